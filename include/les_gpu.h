@@ -1,0 +1,149 @@
+/*
+ * C-ABI of the MI355X LES time-step library (liblesgpu.so).
+ *
+ * Drop-in boundary for the reference's LES_engine / compute_* Fortran module
+ * API (fmargairaz/new-repo).  The reference's procedures take Fortran arrays
+ * by reference and keep module state; here the whole domain is device
+ * resident, fields are named by `les_field` ids, and every entry point returns
+ * an int status that mirrors the reference's decomp_2d_abort codes:
+ *   0 ok, 1 invalid option/model, 3 out of memory, 4 double initialisation,
+ *   5 HIP runtime error, 9 NaN found / singular solve, 10 bad shape or id.
+ * Calls come from one host thread; all kernels run on one HIP stream; host
+ * buffers are synchronised only by download/status calls.  Not reentrant,
+ * like the reference's module state.
+ *
+ * Layout of every 3D field: (nx, ny, nz+2) float64 on the host, Fortran
+ * order (x fastest), z levels 0..nz+1 -- the reference's
+ * dimension(nx, ny, 0:nz+1) arrays with their ghost planes.  Fields the
+ * reference dimensions 1:nz+1 (cs_opt2, nu_t, F_*, Beta_mum, S_ij) use the
+ * same shape with level 0 unused.
+ */
+#ifndef LES_GPU_H
+#define LES_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Parameters after the reference's nondimensionalisation
+ * (src/parameters_IO.f90:240-298); replaces the namelist globals. */
+typedef struct les_params {
+  int nx, ny, nz;
+  int sgs_model;   /* 0 cst_Smag, 2 dyn_LagScaleDep (src/compute_sgs.f90:149-175) */
+  int cs_count;    /* LASD update period (src/compute_LagScaleDep.f90:237) */
+  int dyn_init;    /* first LASD step */
+  int lagran_init; /* restart with F_* present (src/parameters_IO.f90:77) */
+  int wall_model;  /* 0 wall_filter, 1 wall_inst (src/compute_wall_law.f90:119-151) */
+  int ubc_u_neu, ubc_v_neu; /* top BC type of u, v: 1 Neumann, 0 Dirichlet */
+  int halo_mode;   /* Lagrangian stencil halo: 0 one-rank reference semantics, 1 periodic */
+  int coriolis;    /* coriolis_flag */
+  int precision;   /* 0 FP64, 1 mixed (FP32 flow + FP64 pressure/averages/AB2), 2 mixed + learned SGS */
+  int judge_every; /* learned SGS: steps between plain-LASD judge updates */
+  double lx, ly, lz, dt, z0, pfx, pfy, cs0, coriol, ug, vg, ubc_u, ubc_v, ubc_w;
+} les_params;
+
+/* Field ids (system_variables, compute_sgs, compute_LagScaleDep globals). */
+enum les_field {
+  LES_U = 0, LES_V, LES_W,
+  LES_DUDX, LES_DUDY, LES_DUDZ, LES_DVDX, LES_DVDY, LES_DVDZ, LES_DWDX, LES_DWDY, LES_DWDZ,
+  LES_P, LES_DPDX, LES_DPDY, LES_DPDZ,
+  LES_RHSX, LES_RHSY, LES_RHSZ, LES_RHSX_F, LES_RHSY_F, LES_RHSZ_F,
+  LES_TXX, LES_TXY, LES_TXZ, LES_TYY, LES_TYZ, LES_TZZ,
+  LES_CS_OPT2, LES_NU_T, LES_F_LM, LES_F_MM, LES_F_QN, LES_F_NN, LES_BETA,
+  LES_U_LAG, LES_V_LAG, LES_W_LAG,
+  LES_S11, LES_S12, LES_S13, LES_S22, LES_S23, LES_S33,
+  LES_NFIELDS
+};
+
+/* Surface (nx, ny) arrays. */
+enum les_layer { LES_LBC_U = 0, LES_LBC_V, LES_USTAR, LES_BOTTOM_WK, LES_TOP_WK, LES_Z0, LES_NLAYERS };
+
+/* --- set-up / tear-down -------------------------------------------------- */
+/* Replaces create_system + fft_init + decomp_2d_fft_2d_init + deriv_init +
+ * conv_init + press_init + wall_init + test_filter_init/create + sgs_init
+ * (src/LES_engine.f90:125-153; src/system.f90:66-203). */
+int les_init(const les_params* p);
+/* clean_system + *_finalize (src/LES_engine.f90:413-422). */
+int les_finalize(void);
+const char* les_last_error(void);
+
+/* set_initial_conditions(ic_type='files') + BCs (src/initial_conditions.f90:130-208):
+ * u, v, w are (nx, ny, nz) float64 interiors. */
+int les_set_ic(const double* u, const double* v, const double* w);
+
+/* Host mirrors: whole (nx, ny, nz+2) field / (nx, ny) layer. */
+int les_upload_field(int id, const double* host);
+int les_download_field(int id, double* host);
+int les_upload_layer(int id, const double* host);
+int les_download_layer(int id, double* host);
+/* Device pointer and element size (4 or 8) of a field, for zero-copy use. */
+int les_field_ptr(int id, void** dptr, int* elem_bytes);
+
+/* --- the time step -------------------------------------------------------- */
+/* nsteps iterations of the body of LES_main_engine's time loop
+ * (src/LES_engine.f90:189-353). */
+int les_step(int nsteps);
+int les_sync(void);
+int les_jt_total(void);
+int les_set_jt_total(int jt_total);
+/* check_NaN (src/LES_engine.f90:583-616): *found = 1 if u, v or w holds a NaN;
+ * returns 9 in that case. */
+int les_check_nan(int* found);
+/* Per-phase device times in ms since les_init or the last reset:
+ * t[0] time_grad, t[1] time_stress, t[2] time_rhs, t[3] time_press,
+ * t[4] time_loop (src/LES_engine.f90:100-107). */
+int les_timers(double* t, int reset);
+/* Enable (1) or disable (0) per-kernel-family HIP-event timing; while on,
+ * les_timers also returns t[5 + 2f] = summed ms and t[6 + 2f] = launches of
+ * kernel family f (0 row r2c, 1 row c2r, 2 column, 3 row resample,
+ * 4 column resample, 5 point-wise, 6 tridiagonal, 7 Lagrangian, 8 MLP).
+ * Timing synchronises after every timed launch. */
+int les_set_timing(int on);
+
+/* --- operator-level entry points (the compute_* module procedures) -------- */
+/* obukhov + wallstress (src/compute_wall_law.f90:107-254). */
+int les_wall_law(void);
+/* compute_vel_grad (src/LES_engine.f90:449-498). */
+int les_compute_vel_grad(void);
+/* grad_velocity_filt(f, dfdx, dfdy) (src/compute_derivatives.f90:94-137). */
+int les_grad_velocity_filt(int f, int dfdx, int dfdy);
+/* ddz_uv(dfdz, f) / ddz_w(dfdz, f) (src/compute_derivatives.f90:375-441). */
+int les_ddz_uv(int dfdz, int f);
+int les_ddz_w(int dfdz, int f);
+/* sgs_stress(jt, jtt) (src/compute_sgs.f90:105-247), includes
+ * compute_LagScaleDep_coeff (src/compute_LagScaleDep.f90:204-298). */
+int les_sgs_stress(int jt, int jtt);
+/* compute_dealias_convec(cx, cy, cz) (src/compute_convective_padd.f90:115-262). */
+int les_compute_dealias_convec(int cx, int cy, int cz);
+/* div_stress(rhs, tx, ty, tz, stag[, bottom_wk_press, top_wk_press])
+ * (src/compute_derivatives.f90:464-577). */
+int les_div_stress(int rhs, int tx, int ty, int tz, int stag, int save_wk);
+/* compute_RHS_adv (src/LES_engine.f90:509-572). */
+int les_compute_rhs_adv(void);
+/* press_stag_array(tadv1) (src/compute_pressure.f90:116-242). */
+int les_press_stag_array(double tadv1);
+/* test_filter(f, G) on levels 1..nz+1, alpha = 2 or 4 (src/testfilter.f90:144-184). */
+int les_test_filter(int f, int alpha);
+
+/* execute_decomp_2d_fft_2d (src/decomp_2d_fft_2d.f90:498-577) on device
+ * buffers of the library's compute precision: r2c scales the input by
+ * `scale`; nplanes planes of (nx, ny) real <-> (nx/2+1, ny) complex. */
+int les_fft2d_r2c(const void* d_in, void* d_out, int nplanes, double scale);
+int les_fft2d_c2r(const void* d_in, void* d_out, int nplanes);
+
+/* --- learned SGS surrogate (no counterpart in the reference) -------------- */
+/* Generate training samples (features + LASD targets) at the current state;
+ * returns the number of samples written (<= max_samples) into d_x (12 floats
+ * per sample) and d_y (2 floats per sample, device pointers). */
+int les_sgs_samples(void* d_x, void* d_y, long max_samples, long* n_written, int stride);
+/* Load MLP weights (float32 host arrays, see les_mlp_layout) and normalisers. */
+int les_mlp_set_weights(const float* w, long nweights);
+int les_mlp_layout(long* nweights);
+/* Judge statistics of the last learned step: [0] rel. L2 of Cs2 vs LASD,
+ * [1] rel. L2 of beta, [2] fraction of points out of training range. */
+int les_mlp_judge_stats(double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,947 @@
+// MI355X (gfx950) LES time step: device state, operator kernels, time-step
+// driver and the C-ABI declared in include/les_gpu.h.
+//
+// One LES time step of the reference (src/LES_engine.f90:189-353) runs as a
+// sequence of kernels on one HIP stream; the whole domain is resident in HBM.
+// The 2DECOMP pencil transposes and MPI halo exchanges of the reference are
+// replaced by strided column passes and by index arithmetic inside kernels.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "les_state.h"
+#include "les_ops.h"
+
+Les* g_les = nullptr;
+static char g_err[512] = "";
+
+void les_set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+template <>
+const Cplx<double>* twpool<double>(const Les& L) { return L.tw64; }
+template <>
+const Cplx<float>* twpool<float>(const Les& L) { return L.tw32; }
+
+static const double BOGUS = -1.23456789e20;
+static const double PI = 3.1415926535897932384626433;
+static const double TWO_PI = 6.283185307179586476925286766559;
+static const double KVONK = 0.4;
+
+// ------------------------------------------------------------------ plans
+struct HostPlan {
+  FftPlan p;
+  std::vector<double> re_f, im_f, re_i, im_i;
+};
+
+static bool build_plan(int n, HostPlan& hp, int pool_off, int* used) {
+  int a = 0, b = 0, m = n;
+  while (m % 2 == 0) { m /= 2; ++a; }
+  while (m % 3 == 0) { m /= 3; ++b; }
+  if (m != 1 || n < 2) return false;
+  FftPlan& p = hp.p;
+  p.n = n;
+  p.nst = 0;
+  while (a >= 2) { p.R[p.nst++] = 4; a -= 2; }
+  if (a == 1) p.R[p.nst++] = 2;
+  for (int t = 0; t < b; ++t) p.R[p.nst++] = 3;
+  if (p.nst > LES_MAX_STAGES) return false;
+  int span = 1, o = 0;
+  for (int s = 0; s < p.nst; ++s) {
+    p.P[s] = span;
+    p.off[s] = o;
+    o += (p.R[s] - 1) * span;
+    span *= p.R[s];
+  }
+  int sz = o > 0 ? o : 1;
+  hp.re_f.assign(sz, 0.0); hp.im_f.assign(sz, 0.0); hp.re_i.assign(sz, 0.0); hp.im_i.assign(sz, 0.0);
+  for (int s = 0; s < p.nst; ++s)
+    for (int r = 1; r < p.R[s]; ++r)
+      for (int k = 0; k < p.P[s]; ++k) {
+        double ang = TWO_PI * (double)(r * k) / (double)(p.P[s] * p.R[s]);
+        double c = std::cos(ang), sn = std::sin(ang);
+        int q = p.off[s] + (r - 1) * p.P[s] + k;
+        hp.re_f[q] = c; hp.im_f[q] = -sn;
+        hp.re_i[q] = c; hp.im_i[q] = sn;
+      }
+  p.twf = pool_off;
+  p.twi = pool_off + sz;
+  *used = 2 * sz;
+  return true;
+}
+
+// ------------------------------------------------------------------ memory
+static void* dalloc(Les& L, size_t bytes) {
+  void* p = nullptr;
+  if (bytes == 0) bytes = 8;
+  if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+  hipMemset(p, 0, bytes);
+  L.allocs.push_back(p);
+  return p;
+}
+
+static bool field_is_fp64(int id) {
+  switch (id) {
+    case LES_U: case LES_V: case LES_W: case LES_P: case LES_DPDX: case LES_DPDY: case LES_DPDZ:
+    case LES_RHSX: case LES_RHSY: case LES_RHSZ: case LES_RHSX_F: case LES_RHSY_F: case LES_RHSZ_F:
+    case LES_F_LM: case LES_F_MM: case LES_F_QN: case LES_F_NN:
+      return true;
+  }
+  return false;
+}
+
+// ------------------------------------------------------------------ timing
+static void kt_begin(Les& L, int fam) {
+  if (L.timing) hipEventRecord(L.ev_a[fam], L.stream);
+}
+static void kt_end(Les& L, int fam) {
+  if (L.timing) {
+    hipEventRecord(L.ev_b[fam], L.stream);
+    hipEventSynchronize(L.ev_b[fam]);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, L.ev_a[fam], L.ev_b[fam]);
+    L.kt_ms[fam] += ms;
+    L.kt_calls[fam] += 1;
+  }
+}
+
+// ------------------------------------------------------------------ launch helpers
+template <typename T>
+static int lines_per_block(int n) {
+  int l = (sizeof(T) == 8 ? 1024 : 2048) / n;
+  return l < 1 ? 1 : l;
+}
+
+template <typename TI, typename T>
+static void rows_r2c(Les& L, const TI* const* src, Cplx<T>* const* dst, int nf, int k0, int npl, T scale, int mode = LOAD_SCALE,
+                     T div = (T)1, int ny_override = 0) {
+  RowR2CArgs<TI, T> a;
+  for (int f = 0; f < nf; ++f) { a.src[f] = src[f]; a.dst[f] = dst[f]; }
+  a.nx = L.nx; a.ny = ny_override ? ny_override : L.ny; a.lhx = L.lhx; a.k0 = k0; a.nplanes = npl;
+  a.lpb = lines_per_block<T>(L.nx);
+  a.mode = mode; a.scale = scale; a.div = div;
+  long jobs = (long)npl * (a.ny / 2);
+  dim3 grid((unsigned)((jobs + a.lpb - 1) / a.lpb), nf);
+  size_t sh = 2 * (size_t)a.lpb * L.nx * sizeof(Cplx<T>);
+  kt_begin(L, KT_ROWS_R2C);
+  hipLaunchKernelGGL((k_rows_r2c<TI, T>), grid, dim3(LES_BLOCK), sh, L.stream, a, L.px, twpool<T>(L));
+  kt_end(L, KT_ROWS_R2C);
+}
+
+template <typename T, typename TO>
+static void rows_c2r(Les& L, const Cplx<T>* const* src, TO* const* dst, int nf, int k0, int npl) {
+  RowC2RArgs<T, TO> a;
+  for (int f = 0; f < nf; ++f) { a.src[f] = src[f]; a.dst[f] = dst[f]; }
+  a.nx = L.nx; a.ny = L.ny; a.lhx = L.lhx; a.k0 = k0; a.nplanes = npl;
+  a.lpb = lines_per_block<T>(L.nx);
+  long jobs = (long)npl * (L.ny / 2);
+  dim3 grid((unsigned)((jobs + a.lpb - 1) / a.lpb), nf);
+  size_t sh = 2 * (size_t)a.lpb * L.nx * sizeof(Cplx<T>);
+  kt_begin(L, KT_ROWS_C2R);
+  hipLaunchKernelGGL((k_rows_c2r<T, TO>), grid, dim3(LES_BLOCK), sh, L.stream, a, L.px, twpool<T>(L));
+  kt_end(L, KT_ROWS_C2R);
+}
+
+template <typename T>
+static int cols_per_block(int ny) {
+  int c = (sizeof(T) == 8 ? 1024 : 2048) / ny;
+  return c < 1 ? 1 : c;
+}
+
+template <typename T>
+static void cols(Les& L, int op, const Cplx<T>* in0, const Cplx<T>* in1, Cplx<T>* o0, Cplx<T>* o1, Cplx<T>* o2,
+                 int nout, int fwd, int inv, int k0, int npl, const T* g0 = nullptr, const T* g1 = nullptr) {
+  ColArgs<T> a;
+  a.in[0] = in0; a.in[1] = in1;
+  a.out[0] = o0; a.out[1] = o1; a.out[2] = o2;
+  a.kkx = sizeof(T) == 8 ? (const T*)L.kkx64 : (const T*)L.kkx32;
+  a.kky = sizeof(T) == 8 ? (const T*)L.kky64 : (const T*)L.kky32;
+  a.g0 = g0; a.g1 = g1;
+  a.lhx = L.lhx; a.ny = L.ny; a.k0 = k0; a.nplanes = npl;
+  a.cb = cols_per_block<T>(L.ny);
+  a.ncb = (L.lhx + a.cb - 1) / a.cb;
+  a.nin = in1 ? 2 : 1;
+  a.nout = nout; a.op = op; a.fwd = fwd; a.inv = inv;
+  size_t sh = 4 * (size_t)a.cb * L.ny * sizeof(Cplx<T>);
+  kt_begin(L, KT_COLS);
+  hipLaunchKernelGGL((k_cols<T>), dim3((unsigned)(a.ncb * npl)), dim3(LES_BLOCK), sh, L.stream, a, L.py, twpool<T>(L));
+  kt_end(L, KT_COLS);
+}
+
+// ------------------------------------------------------------------ field access
+template <typename T>
+static T* F(Les& L, int id) { return (T*)L.f[id]; }
+static double* D(Les& L, int id) { return (double*)L.f[id]; }
+
+static unsigned nblk(long n, int b = LES_BLOCK) { return (unsigned)((n + b - 1) / b); }
+
+// ------------------------------------------------------------------ operators
+// grad_velocity_filt for one velocity component (FP64 path: the filtered
+// field is written back as in src/compute_derivatives.f90:134).
+template <typename T>
+static void grad_filt(Les& L, int fid, int dx_id, int dy_id) {
+  Cplx<T>* s0 = (Cplx<T>*)L.wk_spec[0];
+  Cplx<T>* s1 = (Cplx<T>*)L.wk_spec[1];
+  Cplx<T>* s2 = (Cplx<T>*)L.wk_spec[2];
+  Cplx<T>* s3 = (Cplx<T>*)L.wk_spec[3];
+  const double* src[1] = {D(L, fid)};
+  Cplx<T>* dst[1] = {s0};
+  T c = (T)(1.0 / (double)(L.nx * L.ny));
+  rows_r2c<double, T>(L, src, dst, 1, 0, L.nzg, c);
+  cols<T>(L, COP_GRAD, s0, nullptr, s1, s2, s3, L.mixed ? 2 : 3, 1, 1, 0, L.nzg);
+  const Cplx<T>* cs[2] = {s1, s2};
+  T* outs[2] = {F<T>(L, dx_id), F<T>(L, dy_id)};
+  rows_c2r<T, T>(L, cs, outs, 2, 0, L.nzg);
+  if (!L.mixed) {
+    const Cplx<T>* c3[1] = {s3};
+    double* o3[1] = {D(L, fid)};
+    rows_c2r<T, double>(L, c3, o3, 1, 0, L.nzg);
+  }
+}
+
+template <typename T>
+static void nyquist_remove_fp64(Les& L, int fid);
+
+template <typename T>
+static void compute_vel_grad(Les& L) {
+  grad_filt<T>(L, LES_U, LES_DUDX, LES_DUDY);
+  grad_filt<T>(L, LES_V, LES_DVDX, LES_DVDY);
+  grad_filt<T>(L, LES_W, LES_DWDX, LES_DWDY);
+  if (L.mixed) {
+    nyquist_remove_fp64<T>(L, LES_U);
+    nyquist_remove_fp64<T>(L, LES_V);
+    nyquist_remove_fp64<T>(L, LES_W);
+  }
+  BcDdzArgs<T> a;
+  a.u = D(L, LES_U); a.v = D(L, LES_V); a.w = D(L, LES_W);
+  a.dudz = F<T>(L, LES_DUDZ); a.dvdz = F<T>(L, LES_DVDZ); a.dwdz = F<T>(L, LES_DWDZ);
+  a.lbc_u = L.layer[LES_LBC_U]; a.lbc_v = L.layer[LES_LBC_V];
+  a.np = L.np; a.nz = L.nz; a.dz = L.dz; a.cz = 1.0 / L.dz;
+  a.ubc_u_neu = L.P.ubc_u_neu; a.ubc_v_neu = L.P.ubc_v_neu;
+  a.ubc_u = L.P.ubc_u; a.ubc_v = L.P.ubc_v; a.ubc_w = L.P.ubc_w;
+  kt_begin(L, KT_POINTWISE);
+  hipLaunchKernelGGL((k_bc_ddz<T>), dim3(nblk(L.np * (L.nz + 1))), dim3(LES_BLOCK), 0, L.stream, a);
+  kt_end(L, KT_POINTWISE);
+}
+
+// 2-delta filtered first level (get_layer_filter, src/utility_tools.f90:261-307)
+// then the wall law (src/compute_wall_law.f90:107-254).
+template <typename T>
+static void wall_law(Les& L) {
+  T* u1 = (T*)L.wk_layer[0];
+  T* v1 = (T*)L.wk_layer[1];
+  if (L.P.wall_model == 0) {
+    Cplx<T>* sa = (Cplx<T>*)L.wk_layer[2];
+    Cplx<T>* sb = (Cplx<T>*)L.wk_layer[3];
+    Cplx<T>* sc = (Cplx<T>*)L.wk_layer[4];
+    Cplx<T>* sd = (Cplx<T>*)L.wk_layer[5];
+    const double* src[2] = {D(L, LES_U) + L.np, D(L, LES_V) + L.np};
+    Cplx<T>* dst[2] = {sa, sb};
+    rows_r2c<double, T>(L, src, dst, 2, 0, 1, (T)1);
+    const T* g2 = sizeof(T) == 8 ? (const T*)L.g2_64 : (const T*)L.g2_32;
+    cols<T>(L, COP_FILT1, sa, nullptr, sc, nullptr, nullptr, 1, 1, 1, 0, 1, g2);
+    cols<T>(L, COP_FILT1, sb, nullptr, sd, nullptr, nullptr, 1, 1, 1, 0, 1, g2);
+    const Cplx<T>* cs[2] = {sc, sd};
+    T* outs[2] = {u1, v1};
+    rows_c2r<T, T>(L, cs, outs, 2, 0, 1);
+  }
+  WallArgs<T> a;
+  a.u1 = u1; a.v1 = v1;
+  a.uraw = D(L, LES_U) + L.np; a.vraw = D(L, LES_V) + L.np;
+  a.inst = L.P.wall_model == 1;
+  a.logterm = L.logterm;
+  a.ustar = L.layer[LES_USTAR]; a.lbc_u = L.layer[LES_LBC_U]; a.lbc_v = L.layer[LES_LBC_V];
+  a.dudz1 = F<T>(L, LES_DUDZ) + L.np; a.dvdz1 = F<T>(L, LES_DVDZ) + L.np;
+  a.txz1 = F<T>(L, LES_TXZ) + L.np; a.tyz1 = F<T>(L, LES_TYZ) + L.np;
+  a.c1 = 0.5 * L.dz * KVONK;
+  a.np = L.np;
+  kt_begin(L, KT_POINTWISE);
+  hipLaunchKernelGGL((k_wall<T>), dim3(nblk(L.np)), dim3(LES_BLOCK), 0, L.stream, a);
+  kt_end(L, KT_POINTWISE);
+}
+
+template <typename T>
+static SgsArgs<T> sgs_args(Les& L) {
+  SgsArgs<T> a;
+  a.dudx = F<T>(L, LES_DUDX); a.dudy = F<T>(L, LES_DUDY); a.dudz = F<T>(L, LES_DUDZ);
+  a.dvdx = F<T>(L, LES_DVDX); a.dvdy = F<T>(L, LES_DVDY); a.dvdz = F<T>(L, LES_DVDZ);
+  a.dwdx = F<T>(L, LES_DWDX); a.dwdy = F<T>(L, LES_DWDY); a.dwdz = F<T>(L, LES_DWDZ);
+  a.cs = F<T>(L, LES_CS_OPT2); a.nu_t = F<T>(L, LES_NU_T);
+  a.txx = F<T>(L, LES_TXX); a.txy = F<T>(L, LES_TXY); a.txz = F<T>(L, LES_TXZ);
+  a.tyy = F<T>(L, LES_TYY); a.tyz = F<T>(L, LES_TYZ); a.tzz = F<T>(L, LES_TZZ);
+  a.s11 = F<T>(L, LES_S11); a.s12 = F<T>(L, LES_S12); a.s13 = F<T>(L, LES_S13);
+  a.s22 = F<T>(L, LES_S22); a.s23 = F<T>(L, LES_S23); a.s33 = F<T>(L, LES_S33);
+  a.smod = (T*)L.wk_real[63];
+  a.u = D(L, LES_U); a.v = D(L, LES_V); a.w = D(L, LES_W);
+  a.ulag = F<T>(L, LES_U_LAG); a.vlag = F<T>(L, LES_V_LAG); a.wlag = F<T>(L, LES_W_LAG);
+  a.lvec = L.lvec;
+  a.np = L.np; a.nz = L.nz;
+  a.accumulate = 0; a.write_sij = 0;
+  return a;
+}
+
+template <typename T>
+static void lasd_update(Les& L, int jtt);
+
+template <typename T>
+static void sgs_stress(Les& L, int jtt) {
+  SgsArgs<T> a = sgs_args<T>(L);
+  bool lasd = L.P.sgs_model == 2;
+  bool acc = lasd && jtt >= L.P.dyn_init;
+  bool update = lasd && jtt >= L.P.dyn_init && (jtt % L.P.cs_count) == 0;
+  if (L.P.sgs_model == 0) {
+    CsFillArgs<T> c;
+    c.cs = F<T>(L, LES_CS_OPT2); c.val = (T)(L.P.cs0 * L.P.cs0); c.np = L.np; c.nz = L.nz;
+    hipLaunchKernelGGL((k_cs_fill<T>), dim3(nblk(L.np * (L.nz + 1))), dim3(LES_BLOCK), 0, L.stream, c);
+  } else if (lasd && jtt < L.P.dyn_init) {
+    CsFillArgs<T> c;
+    c.cs = F<T>(L, LES_CS_OPT2); c.val = (T)0.03; c.np = L.np; c.nz = L.nz;
+    hipLaunchKernelGGL((k_cs_fill<T>), dim3(nblk(L.np * (L.nz + 1))), dim3(LES_BLOCK), 0, L.stream, c);
+  }
+  if (update) {
+    a.accumulate = acc ? 1 : 0;
+    a.write_sij = 1;
+    kt_begin(L, KT_POINTWISE);
+    hipLaunchKernelGGL((k_sij<T>), dim3(nblk(L.np * (L.nz + 1))), dim3(LES_BLOCK), 0, L.stream, a);
+    kt_end(L, KT_POINTWISE);
+    lasd_update<T>(L, jtt);
+    a.accumulate = 0;
+  } else {
+    a.accumulate = acc ? 1 : 0;
+  }
+  a.write_sij = 0;
+  kt_begin(L, KT_POINTWISE);
+  hipLaunchKernelGGL((k_sgs<T>), dim3(nblk(L.np * (L.nz + 1))), dim3(LES_BLOCK), 0, L.stream, a);
+  kt_end(L, KT_POINTWISE);
+}
+
+#include "les_lasd.h"
+#include "les_rhs.h"
+#include "les_press.h"
+
+// ---------------------------------------------------------------- mixed-mode Nyquist filter
+// In the mixed-precision mode the velocity stays FP64, so the Nyquist-free
+// velocity of grad_velocity_filt (src/compute_derivatives.f90:105-134) is
+// formed directly in FP64 by removing the projection on the kx = nx/2 and
+// ky = ny/2 modes: f -= (-1)^i A_j + (-1)^j B_i - (-1)^(i+j) C, with A, B, C
+// the normalised alternating sums along x, y and both.
+__global__ void k_nyq_rowsum(const double* f, double* A, int nx, long nrows) {
+  __shared__ double red[LES_BLOCK];
+  const long row = blockIdx.x;
+  if (row >= nrows) return;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nx; i += blockDim.x) s += (i & 1) ? -f[row * nx + i] : f[row * nx + i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) A[row] = red[0] / (double)nx;
+}
+
+__global__ void k_nyq_colsum(const double* f, double* B, int nx, int ny, int nplanes) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)nx * nplanes) return;
+  const int i = (int)(t % nx), k = (int)(t / nx);
+  double s = 0.0;
+  for (int j = 0; j < ny; ++j) {
+    const double v = f[((long)k * ny + j) * nx + i];
+    s += (j & 1) ? -v : v;
+  }
+  B[t] = s / (double)ny;
+}
+
+__global__ void k_nyq_csum(const double* A, double* C, int ny, int nplanes) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nplanes) return;
+  double s = 0.0;
+  for (int j = 0; j < ny; ++j) s += (j & 1) ? -A[(long)k * ny + j] : A[(long)k * ny + j];
+  C[k] = s / (double)ny;
+}
+
+__global__ void k_nyq_apply(double* f, const double* A, const double* B, const double* C, int nx, int ny, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = (int)(t % nx);
+  const long r = t / nx;
+  const int j = (int)(r % ny);
+  const long k = r / ny;
+  const double si = (i & 1) ? -1.0 : 1.0, sj = (j & 1) ? -1.0 : 1.0;
+  f[t] -= si * A[r] + sj * B[k * nx + i] - si * sj * C[k];
+}
+
+template <typename T>
+static void nyquist_remove_fp64(Les& L, int fid) {
+  double* f = D(L, fid);
+  double* A = L.nyq_rows;
+  double* B = A + (long)L.ny * L.nzg;
+  double* C = B + (long)L.nx * L.nzg;
+  const long nrows = (long)L.ny * L.nzg;
+  kt_begin(L, KT_POINTWISE);
+  hipLaunchKernelGGL(k_nyq_rowsum, dim3((unsigned)nrows), dim3(LES_BLOCK), 0, L.stream, f, A, L.nx, nrows);
+  hipLaunchKernelGGL(k_nyq_colsum, dim3(nblk((long)L.nx * L.nzg)), dim3(LES_BLOCK), 0, L.stream, f, B, L.nx, L.ny, L.nzg);
+  hipLaunchKernelGGL(k_nyq_csum, dim3(nblk(L.nzg)), dim3(LES_BLOCK), 0, L.stream, A, C, L.ny, L.nzg);
+  hipLaunchKernelGGL(k_nyq_apply, dim3(nblk(L.nf)), dim3(LES_BLOCK), 0, L.stream, f, A, B, C, L.nx, L.ny, L.nf);
+  kt_end(L, KT_POINTWISE);
+}
+
+// ------------------------------------------------------------------ step
+template <typename T>
+static int les_step_t(Les& L) {
+  L.jt_total += 1;
+  const int jtt = L.jt_total;
+  if (L.timing) hipEventRecord(L.ph_ev[0], L.stream);
+  wall_law<T>(L);
+  compute_vel_grad<T>(L);
+  if (L.timing) hipEventRecord(L.ph_ev[1], L.stream);
+  sgs_stress<T>(L, jtt);
+  if (L.timing) hipEventRecord(L.ph_ev[2], L.stream);
+  compute_rhs_adv<T>(L);
+  if (L.timing) hipEventRecord(L.ph_ev[3], L.stream);
+  double tadv1 = (jtt == 1) ? 1.0 : 1.5;
+  double tadv2 = (jtt == 1) ? 0.0 : 1.0 - tadv1;
+  {
+    Ab2Args a;
+    a.u = D(L, LES_U); a.v = D(L, LES_V); a.w = D(L, LES_W);
+    a.rx = D(L, LES_RHSX); a.ry = D(L, LES_RHSY); a.rz = D(L, LES_RHSZ);
+    a.fx = D(L, LES_RHSX_F); a.fy = D(L, LES_RHSY_F); a.fz = D(L, LES_RHSZ_F);
+    a.dt = L.P.dt; a.t1 = tadv1; a.t2 = tadv2; a.np = L.np; a.nz = L.nz;
+    kt_begin(L, KT_POINTWISE);
+    hipLaunchKernelGGL(k_ab2, dim3(nblk(L.np * (L.nz + 2))), dim3(LES_BLOCK), 0, L.stream, a);
+    kt_end(L, KT_POINTWISE);
+  }
+  press_stag_array<T>(L, tadv1);
+  projection(L, tadv1);
+  if (L.timing) {
+    hipEventRecord(L.ph_ev[4], L.stream);
+    hipEventSynchronize(L.ph_ev[4]);
+    float ms;
+    hipEventElapsedTime(&ms, L.ph_ev[0], L.ph_ev[1]); L.phase_ms[0] += ms;
+    hipEventElapsedTime(&ms, L.ph_ev[1], L.ph_ev[2]); L.phase_ms[1] += ms;
+    hipEventElapsedTime(&ms, L.ph_ev[2], L.ph_ev[3]); L.phase_ms[2] += ms;
+    hipEventElapsedTime(&ms, L.ph_ev[3], L.ph_ev[4]); L.phase_ms[3] += ms;
+    hipEventElapsedTime(&ms, L.ph_ev[0], L.ph_ev[4]); L.phase_ms[4] += ms;
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------------ set-up
+static int setup_kernel_attrs() {
+  const int big = 160 * 1024;
+  hipFuncSetAttribute((const void*)k_rows_r2c<double, double>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+  hipFuncSetAttribute((const void*)k_rows_r2c<double, float>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+  hipFuncSetAttribute((const void*)k_rows_r2c<float, float>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+  hipFuncSetAttribute((const void*)k_rows_c2r<double, double>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+  hipFuncSetAttribute((const void*)k_rows_c2r<float, float>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+  hipFuncSetAttribute((const void*)k_rows_c2r<float, double>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+  hipFuncSetAttribute((const void*)k_cols<double>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+  hipFuncSetAttribute((const void*)k_cols<float>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+  hipFuncSetAttribute((const void*)k_rows_resample<double, double, double>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+  hipFuncSetAttribute((const void*)k_rows_resample<float, double, float>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+  hipFuncSetAttribute((const void*)k_rows_resample<float, float, float>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+  hipFuncSetAttribute((const void*)k_rows_resample<float, double, double>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+  hipFuncSetAttribute((const void*)k_cols_resample<double>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+  hipFuncSetAttribute((const void*)k_cols_resample<float>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+  return 0;
+}
+
+static void free_all(Les* L) {
+  if (!L) return;
+  for (void* p : L->allocs) hipFree(p);
+  L->allocs.clear();
+  for (int i = 0; i < KT_COUNT; ++i) { hipEventDestroy(L->ev_a[i]); hipEventDestroy(L->ev_b[i]); }
+  for (int i = 0; i < 6; ++i) hipEventDestroy(L->ph_ev[i]);
+  hipStreamDestroy(L->stream);
+}
+
+extern "C" {
+
+const char* les_last_error(void) { return g_err; }
+
+int les_init(const les_params* p) {
+  if (g_les) { les_set_error("les_init: already initialised"); return 4; }
+  if (!p) { les_set_error("les_init: null params"); return 1; }
+  const les_params P = *p;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) { les_set_error("les_init: no HIP device"); return 5; }
+  for (int n : {P.nx, P.ny}) {
+    int m = n;
+    while (m % 2 == 0) m /= 2;
+    while (m % 3 == 0) m /= 3;
+    if (m != 1 || n % 4 || n < 8) { les_set_error("les_init: nx, ny must be 2^a 3^b, multiples of 4, >= 8"); return 10; }
+  }
+  if (P.nz < 3) { les_set_error("les_init: nz must be >= 3"); return 10; }
+  if (P.sgs_model != 0 && P.sgs_model != 2) { les_set_error("Invalid SGS model"); return 1; }
+  if (P.cs_count < 1) { les_set_error("cs_count must be >= 1"); return 1; }
+  if (P.wall_model != 0 && P.wall_model != 1) { les_set_error("Invalid wall law type"); return 1; }
+  if (P.precision < 0 || P.precision > 2) { les_set_error("precision must be 0, 1 or 2"); return 1; }
+  Les* L = new Les();
+  std::memset((void*)L, 0, sizeof(Les) - sizeof(std::vector<void*>));
+  new (&L->allocs) std::vector<void*>();
+  L->P = P;
+  L->nx = P.nx; L->ny = P.ny; L->nz = P.nz; L->nzg = P.nz + 2; L->lhx = P.nx / 2 + 1;
+  L->nx2 = 3 * P.nx / 2; L->ny2 = 3 * P.ny / 2;
+  L->np = (long)P.nx * P.ny; L->nf = L->np * L->nzg; L->ns = (long)L->lhx * P.ny; L->nsp = L->ns * L->nzg;
+  L->nbp = (long)L->nx2 * L->ny2;
+  L->dx = P.lx / (double)P.nx; L->dy = P.ly / (double)P.ny; L->dz = P.lz / (double)P.nz;
+  L->delta = std::pow(L->dx * L->dy * L->dz, 1.0 / 3.0);
+  L->jt_total = 0;
+  L->F_coeff_init = P.lagran_init != 0;
+  L->mixed = P.precision != 0;
+  if (hipStreamCreate(&L->stream) != hipSuccess) { delete L; les_set_error("stream"); return 5; }
+  for (int i = 0; i < KT_COUNT; ++i) { hipEventCreate(&L->ev_a[i]); hipEventCreate(&L->ev_b[i]); }
+  for (int i = 0; i < 6; ++i) hipEventCreate(&L->ph_ev[i]);
+  g_les = L;
+  // ---- plans and twiddles
+  HostPlan hp[4];
+  int sizes[4] = {L->nx, L->ny, L->nx2, L->ny2};
+  int off = 0;
+  for (int i = 0; i < 4; ++i) {
+    int used = 0;
+    if (!build_plan(sizes[i], hp[i], off, &used)) { les_set_error("fft plan %d", sizes[i]); les_finalize(); return 10; }
+    off += used;
+  }
+  std::vector<Cplx<double>> t64(off);
+  std::vector<Cplx<float>> t32(off);
+  for (int i = 0; i < 4; ++i) {
+    int sz = (int)hp[i].re_f.size();
+    for (int q = 0; q < sz; ++q) {
+      t64[hp[i].p.twf + q] = {hp[i].re_f[q], hp[i].im_f[q]};
+      t64[hp[i].p.twi + q] = {hp[i].re_i[q], hp[i].im_i[q]};
+    }
+  }
+  for (int q = 0; q < off; ++q) t32[q] = {(float)t64[q].x, (float)t64[q].y};
+  L->px = hp[0].p; L->py = hp[1].p; L->px2 = hp[2].p; L->py2 = hp[3].p;
+  L->tw64 = (Cplx<double>*)dalloc(*L, sizeof(Cplx<double>) * off);
+  L->tw32 = (Cplx<float>*)dalloc(*L, sizeof(Cplx<float>) * off);
+  if (!L->tw64 || !L->tw32) { les_finalize(); les_set_error("out of memory"); return 3; }
+  hipMemcpy(L->tw64, t64.data(), sizeof(Cplx<double>) * off, hipMemcpyHostToDevice);
+  hipMemcpy(L->tw32, t32.data(), sizeof(Cplx<float>) * off, hipMemcpyHostToDevice);
+  // ---- spectral tables (src/fft_engine.f90:97-145; src/testfilter.f90:101-131)
+  int nx = L->nx, ny = L->ny, lhx = L->lhx;
+  std::vector<double> kkx(L->ns), kky(L->ns), kk2(L->ns), G2(L->ns), G4(L->ns);
+  double cx = 2.0 * PI / P.lx, cy = 2.0 * PI / P.ly;
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < lhx; ++i) {
+      double kx = (double)i;
+      int m = (j + ny / 2) % ny;
+      double ky = (double)(m - ny / 2);
+      if (i == lhx - 1 || j == ny / 2) { kx = 0.0; ky = 0.0; }
+      kx = cx * kx;
+      ky = cy * ky;
+      long q = i + (long)lhx * j;
+      kkx[q] = kx; kky[q] = ky; kk2[q] = kx * kx + ky * ky;
+    }
+  for (int a = 0; a < 2; ++a) {
+    double alpha = a == 0 ? 2.0 : 4.0;
+    std::vector<double>& G = a == 0 ? G2 : G4;
+    double dl = alpha * std::sqrt(L->dx * L->dy);
+    double t = PI / dl;
+    double kc2 = t * t;
+    for (long q = 0; q < L->ns; ++q) G[q] = kk2[q] >= kc2 ? 0.0 : 1.0 / (double)(nx * ny);
+    for (int j = 0; j < ny; ++j) G[(lhx - 1) + (long)lhx * j] = 0.0;
+    for (int i = 0; i < lhx; ++i) G[i + (long)lhx * (ny / 2)] = 0.0;
+  }
+  std::vector<float> f32(L->ns);
+  double** d64[4] = {&L->kkx64, &L->kky64, &L->g2_64, &L->g4_64};
+  float** d32[4] = {&L->kkx32, &L->kky32, &L->g2_32, &L->g4_32};
+  std::vector<double>* hsrc[4] = {&kkx, &kky, &G2, &G4};
+  for (int t = 0; t < 4; ++t) {
+    *d64[t] = (double*)dalloc(*L, sizeof(double) * L->ns);
+    *d32[t] = (float*)dalloc(*L, sizeof(float) * L->ns);
+    if (!*d64[t] || !*d32[t]) { les_finalize(); les_set_error("out of memory"); return 3; }
+    hipMemcpy(*d64[t], hsrc[t]->data(), sizeof(double) * L->ns, hipMemcpyHostToDevice);
+    for (long q = 0; q < L->ns; ++q) f32[q] = (float)(*hsrc[t])[q];
+    hipMemcpy(*d32[t], f32.data(), sizeof(float) * L->ns, hipMemcpyHostToDevice);
+  }
+  // ---- surface: z0 and log(0.5 dz / z0)
+  std::vector<double> lt(L->np, std::log((0.5 * L->dz) / P.z0)), z0(L->np, P.z0);
+  L->logterm = (double*)dalloc(*L, sizeof(double) * L->np);
+  for (int i = 0; i < LES_NLAYERS; ++i) L->layer[i] = (double*)dalloc(*L, sizeof(double) * L->np);
+  hipMemcpy(L->logterm, lt.data(), sizeof(double) * L->np, hipMemcpyHostToDevice);
+  hipMemcpy(L->layer[LES_Z0], z0.data(), sizeof(double) * L->np, hipMemcpyHostToDevice);
+  // ---- mixing length per level (src/compute_sgs.f90:142-163)
+  std::vector<double> lv(L->nzg, L->delta);
+  if (P.sgs_model == 0) {
+    double n = 2.0, z = 0.5 * L->dz;
+    lv[1] = std::pow(std::pow(P.cs0, n) * std::pow(KVONK * z, -n) + std::pow(L->delta, -n), -1.0 / n);
+    for (int k = 2; k <= L->nz + 1; ++k) {
+      z = ((double)k - 1.0) * L->dz;
+      lv[k] = std::pow(std::pow(P.cs0, n) * std::pow(KVONK * z, -n) + std::pow(L->delta, -n), -1.0 / n);
+    }
+  }
+  L->lvec = (double*)dalloc(*L, sizeof(double) * L->nzg);
+  hipMemcpy(L->lvec, lv.data(), sizeof(double) * L->nzg, hipMemcpyHostToDevice);
+  // ---- fields
+  size_t tb = L->mixed ? 4 : 8;
+  for (int id = 0; id < LES_NFIELDS; ++id) {
+    L->fbytes[id] = field_is_fp64(id) ? 8 : (int)tb;
+    L->f[id] = dalloc(*L, (size_t)L->fbytes[id] * L->nf);
+    if (!L->f[id]) { les_finalize(); les_set_error("out of memory (fields)"); return 3; }
+  }
+  // w(:,:,0) = BOGUS (src/system.f90:196-198) is set by les_set_ic.
+  // ---- workspaces
+  for (int i = 0; i < 8; ++i) L->wk_spec[i] = dalloc(*L, 2 * tb * L->nsp);
+  for (int i = 0; i < 64; ++i) L->wk_real[i] = dalloc(*L, tb * L->nf);
+  for (int i = 0; i < 6; ++i) L->wk_hyb[i] = dalloc(*L, tb * (size_t)L->nx2 * L->ny * L->nzg);
+  for (int i = 0; i < 6; ++i) L->wk_big[i] = dalloc(*L, tb * (size_t)L->nbp * L->nzg);
+  for (int i = 0; i < 3; ++i) L->wk_prod[i] = dalloc(*L, tb * (size_t)L->nbp * L->nzg);
+  for (int i = 0; i < 3; ++i) L->wk_hyb3[i] = dalloc(*L, tb * (size_t)L->nx2 * L->ny * L->nzg);
+  for (int i = 0; i < 5; ++i) L->pspec[i] = (double*)dalloc(*L, 16 * (size_t)L->nsp);
+  for (int i = 0; i < 4; ++i) L->playr[i] = (double*)dalloc(*L, 16 * (size_t)L->ns);
+  for (int i = 0; i < 6; ++i) L->wk_layer[i] = dalloc(*L, 2 * tb * (size_t)L->ns + 2 * tb * L->np);
+  for (int i = 0; i < 4; ++i) L->Fold[i] = (double*)dalloc(*L, 8 * L->nf);
+  L->nyq_rows = (double*)dalloc(*L, 8 * (size_t)(L->nx + L->ny + 2) * L->nzg);
+  L->nanflag = (int*)dalloc(*L, 64);
+  for (void* q : L->allocs)
+    if (!q) { les_finalize(); les_set_error("out of memory (workspaces)"); return 3; }
+  for (int i = 0; i < 64; ++i) if (!L->wk_real[i]) { les_finalize(); les_set_error("out of memory"); return 3; }
+  setup_kernel_attrs();
+  if (hipDeviceSynchronize() != hipSuccess) { les_set_error("init sync failed"); les_finalize(); return 5; }
+  return 0;
+}
+
+int les_finalize(void) {
+  if (!g_les) return 0;
+  hipStreamSynchronize(g_les->stream);
+  free_all(g_les);
+  g_les->allocs.~vector();
+  ::operator delete((void*)g_les);
+  g_les = nullptr;
+  return 0;
+}
+
+int les_sync(void) {
+  if (!g_les) return 1;
+  LES_CHECK(hipStreamSynchronize(g_les->stream));
+  return 0;
+}
+
+int les_jt_total(void) { return g_les ? g_les->jt_total : -1; }
+int les_set_jt_total(int jt) {
+  if (!g_les) return 1;
+  g_les->jt_total = jt;
+  return 0;
+}
+
+// Host <-> device copies with precision conversion.
+__global__ void k_cvt_d2f(const double* a, float* b, long n) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
+}
+__global__ void k_cvt_f2d(const float* a, double* b, long n) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (double)a[i];
+}
+
+static int copy_field(int id, double* host, int up) {
+  Les* L = g_les;
+  if (!L) { les_set_error("not initialised"); return 1; }
+  if (id < 0 || id >= LES_NFIELDS) { les_set_error("bad field id %d", id); return 10; }
+  size_t n = (size_t)L->nf;
+  if (L->fbytes[id] == 8) {
+    if (up) { LES_CHECK(hipMemcpyAsync(L->f[id], host, 8 * n, hipMemcpyHostToDevice, L->stream)); }
+    else { LES_CHECK(hipMemcpyAsync(host, L->f[id], 8 * n, hipMemcpyDeviceToHost, L->stream)); }
+  } else {
+    double* stage = nullptr;
+    LES_CHECK(hipMallocAsync((void**)&stage, 8 * n, L->stream));
+    if (up) {
+      LES_CHECK(hipMemcpyAsync(stage, host, 8 * n, hipMemcpyHostToDevice, L->stream));
+      hipLaunchKernelGGL(k_cvt_d2f, dim3(nblk((long)n)), dim3(LES_BLOCK), 0, L->stream, stage, (float*)L->f[id], (long)n);
+    } else {
+      hipLaunchKernelGGL(k_cvt_f2d, dim3(nblk((long)n)), dim3(LES_BLOCK), 0, L->stream, (const float*)L->f[id], stage, (long)n);
+      LES_CHECK(hipMemcpyAsync(host, stage, 8 * n, hipMemcpyDeviceToHost, L->stream));
+    }
+    LES_CHECK(hipFreeAsync(stage, L->stream));
+  }
+  LES_CHECK(hipStreamSynchronize(L->stream));
+  return 0;
+}
+
+int les_upload_field(int id, const double* host) { return copy_field(id, (double*)host, 1); }
+int les_download_field(int id, double* host) { return copy_field(id, host, 0); }
+
+int les_field_ptr(int id, void** dptr, int* elem_bytes) {
+  if (!g_les) return 1;
+  if (id < 0 || id >= LES_NFIELDS) return 10;
+  *dptr = g_les->f[id];
+  *elem_bytes = g_les->fbytes[id];
+  return 0;
+}
+
+int les_upload_layer(int id, const double* host) {
+  Les* L = g_les;
+  if (!L) return 1;
+  if (id < 0 || id >= LES_NLAYERS) return 10;
+  LES_CHECK(hipMemcpy(L->layer[id], host, 8 * L->np, hipMemcpyHostToDevice));
+  if (id == LES_Z0) {
+    std::vector<double> z0(L->np), lt(L->np);
+    std::memcpy(z0.data(), host, 8 * L->np);
+    for (long q = 0; q < L->np; ++q) lt[q] = std::log((0.5 * L->dz) / z0[q]);
+    LES_CHECK(hipMemcpy(L->logterm, lt.data(), 8 * L->np, hipMemcpyHostToDevice));
+  }
+  return 0;
+}
+
+int les_download_layer(int id, double* host) {
+  Les* L = g_les;
+  if (!L) return 1;
+  if (id < 0 || id >= LES_NLAYERS) return 10;
+  LES_CHECK(hipStreamSynchronize(L->stream));
+  LES_CHECK(hipMemcpy(host, L->layer[id], 8 * L->np, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int les_set_ic(const double* u0, const double* v0, const double* w0) {
+  Les* L = g_les;
+  if (!L) return 1;
+  std::vector<double> h(L->nf, 0.0);
+  const double* src[3] = {u0, v0, w0};
+  for (int c = 0; c < 3; ++c) {
+    std::fill(h.begin(), h.end(), 0.0);
+    for (int k = 1; k <= L->nz; ++k) std::memcpy(&h[(size_t)k * L->np], src[c] + (size_t)(k - 1) * L->np, 8 * L->np);
+    int rc = les_upload_field(LES_U + c, h.data());
+    if (rc) return rc;
+  }
+  IcBcArgs a;
+  a.u = D(*L, LES_U); a.v = D(*L, LES_V); a.w = D(*L, LES_W);
+  a.np = L->np; a.nz = L->nz; a.dz = L->dz;
+  a.ubc_u_neu = L->P.ubc_u_neu; a.ubc_v_neu = L->P.ubc_v_neu;
+  a.ubc_u = L->P.ubc_u; a.ubc_v = L->P.ubc_v; a.ubc_w = L->P.ubc_w;
+  a.bogus = BOGUS;
+  hipLaunchKernelGGL(k_ic_bc, dim3(nblk(L->np)), dim3(LES_BLOCK), 0, L->stream, a);
+  LES_CHECK(hipGetLastError());
+  LES_CHECK(hipStreamSynchronize(L->stream));
+  return 0;
+}
+
+int les_step(int nsteps) {
+  Les* L = g_les;
+  if (!L) { les_set_error("not initialised"); return 1; }
+  for (int s = 0; s < nsteps; ++s) {
+    int rc = L->mixed ? les_step_t<float>(*L) : les_step_t<double>(*L);
+    if (rc) return rc;
+  }
+  LES_CHECK(hipGetLastError());
+  return 0;
+}
+
+int les_check_nan(int* found) {
+  Les* L = g_les;
+  if (!L) return 1;
+  LES_CHECK(hipMemsetAsync(L->nanflag, 0, 4, L->stream));
+  NanArgs a;
+  a.u = D(*L, LES_U); a.v = D(*L, LES_V); a.w = D(*L, LES_W);
+  a.n = L->np * L->nz; a.off = L->np; a.flag = L->nanflag;
+  hipLaunchKernelGGL(k_nan, dim3(nblk(a.n)), dim3(LES_BLOCK), 0, L->stream, a);
+  int h = 0;
+  LES_CHECK(hipMemcpyAsync(&h, L->nanflag, 4, hipMemcpyDeviceToHost, L->stream));
+  LES_CHECK(hipStreamSynchronize(L->stream));
+  if (found) *found = h;
+  if (h) { les_set_error("I got a NaN"); return 9; }
+  return 0;
+}
+
+int les_timers(double* t, int reset) {
+  Les* L = g_les;
+  if (!L) return 1;
+  if (t) {
+    for (int i = 0; i < 5; ++i) t[i] = L->phase_ms[i];
+    for (int i = 0; i < KT_COUNT; ++i) {
+      t[5 + 2 * i] = L->kt_ms[i];
+      t[6 + 2 * i] = (double)L->kt_calls[i];
+    }
+  }
+  if (reset) {
+    for (int i = 0; i < 5; ++i) L->phase_ms[i] = 0;
+    for (int i = 0; i < KT_COUNT; ++i) { L->kt_ms[i] = 0; L->kt_calls[i] = 0; }
+  }
+  return 0;
+}
+
+int les_set_timing(int on) {
+  if (!g_les) return 1;
+  g_les->timing = on;
+  return 0;
+}
+
+// ---- operator-level entry points
+int les_wall_law(void) {
+  if (!g_les) return 1;
+  if (g_les->mixed) wall_law<float>(*g_les); else wall_law<double>(*g_les);
+  LES_CHECK(hipGetLastError());
+  return 0;
+}
+int les_compute_vel_grad(void) {
+  if (!g_les) return 1;
+  if (g_les->mixed) compute_vel_grad<float>(*g_les); else compute_vel_grad<double>(*g_les);
+  LES_CHECK(hipGetLastError());
+  return 0;
+}
+int les_grad_velocity_filt(int f, int dfdx, int dfdy) {
+  if (!g_les) return 1;
+  if (f < LES_U || f > LES_W) { les_set_error("grad_velocity_filt: f must be u, v or w"); return 10; }
+  if (g_les->mixed) grad_filt<float>(*g_les, f, dfdx, dfdy); else grad_filt<double>(*g_les, f, dfdx, dfdy);
+  LES_CHECK(hipGetLastError());
+  return 0;
+}
+int les_sgs_stress(int jt, int jtt) {
+  (void)jt;
+  if (!g_les) return 1;
+  if (g_les->mixed) sgs_stress<float>(*g_les, jtt); else sgs_stress<double>(*g_les, jtt);
+  LES_CHECK(hipGetLastError());
+  return 0;
+}
+int les_compute_rhs_adv(void) {
+  if (!g_les) return 1;
+  if (g_les->mixed) compute_rhs_adv<float>(*g_les); else compute_rhs_adv<double>(*g_les);
+  LES_CHECK(hipGetLastError());
+  return 0;
+}
+int les_press_stag_array(double tadv1) {
+  if (!g_les) return 1;
+  if (g_les->mixed) press_stag_array<float>(*g_les, tadv1); else press_stag_array<double>(*g_les, tadv1);
+  LES_CHECK(hipGetLastError());
+  return 0;
+}
+
+int les_fft2d_r2c(const void* d_in, void* d_out, int nplanes, double scale) {
+  Les* L = g_les;
+  if (!L) return 1;
+  if (L->mixed) {
+    const float* s[1] = {(const float*)d_in};
+    Cplx<float>* d[1] = {(Cplx<float>*)d_out};
+    rows_r2c<float, float>(*L, s, d, 1, 0, nplanes, (float)scale);
+    cols<float>(*L, COP_COPY, d[0], nullptr, d[0], nullptr, nullptr, 1, 1, 0, 0, nplanes);
+  } else {
+    const double* s[1] = {(const double*)d_in};
+    Cplx<double>* d[1] = {(Cplx<double>*)d_out};
+    rows_r2c<double, double>(*L, s, d, 1, 0, nplanes, scale);
+    cols<double>(*L, COP_COPY, d[0], nullptr, d[0], nullptr, nullptr, 1, 1, 0, 0, nplanes);
+  }
+  LES_CHECK(hipGetLastError());
+  LES_CHECK(hipStreamSynchronize(L->stream));
+  return 0;
+}
+
+int les_fft2d_c2r(const void* d_in, void* d_out, int nplanes) {
+  Les* L = g_les;
+  if (!L) return 1;
+  // the input is preserved (unlike FFTW's c2r): the y pass goes to a workspace
+  size_t need = (size_t)nplanes * L->ns;
+  if (need > (size_t)L->nsp) { les_set_error("les_fft2d_c2r: at most nz+2 planes"); return 10; }
+  if (L->mixed) {
+    Cplx<float>* w = (Cplx<float>*)L->wk_spec[7];
+    cols<float>(*L, COP_COPY, (const Cplx<float>*)d_in, nullptr, w, nullptr, nullptr, 1, 0, 1, 0, nplanes);
+    const Cplx<float>* s[1] = {w};
+    float* d[1] = {(float*)d_out};
+    rows_c2r<float, float>(*L, s, d, 1, 0, nplanes);
+  } else {
+    Cplx<double>* w = (Cplx<double>*)L->wk_spec[7];
+    cols<double>(*L, COP_COPY, (const Cplx<double>*)d_in, nullptr, w, nullptr, nullptr, 1, 0, 1, 0, nplanes);
+    const Cplx<double>* s[1] = {w};
+    double* d[1] = {(double*)d_out};
+    rows_c2r<double, double>(*L, s, d, 1, 0, nplanes);
+  }
+  LES_CHECK(hipGetLastError());
+  LES_CHECK(hipStreamSynchronize(L->stream));
+  return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- generic ddz
+// ddz_uv (uvp -> w, levels 1..nz+1) and ddz_w (w -> uvp, levels 1..nz, top
+// level zero), src/compute_derivatives.f90:375-441, on fields of either
+// precision.
+template <typename TI, typename TO>
+__global__ void k_ddz(const TI* f, TO* d, long np, int nz, double cz, int w_nodes) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= np * (nz + 1)) return;
+  const long c = np + t;
+  const int k = 1 + (int)(t / np);
+  if (w_nodes) {
+    d[c] = (k <= nz) ? (TO)(cz * ((double)f[c + np] - (double)f[c])) : (TO)0;
+  } else {
+    d[c] = (TO)(cz * ((double)f[c] - (double)f[c - np]));
+  }
+}
+
+static int ddz_any(int dfdz, int f, int w_nodes) {
+  Les* L = g_les;
+  if (!L) return 1;
+  if (dfdz < 0 || dfdz >= LES_NFIELDS || f < 0 || f >= LES_NFIELDS) return 10;
+  const double cz = 1.0 / L->dz;
+  dim3 g(nblk(L->np * (L->nz + 1)));
+  const bool fi = L->fbytes[f] == 8, fo = L->fbytes[dfdz] == 8;
+  if (fi && fo) hipLaunchKernelGGL((k_ddz<double, double>), g, dim3(LES_BLOCK), 0, L->stream, (const double*)L->f[f], (double*)L->f[dfdz], L->np, L->nz, cz, w_nodes);
+  else if (fi) hipLaunchKernelGGL((k_ddz<double, float>), g, dim3(LES_BLOCK), 0, L->stream, (const double*)L->f[f], (float*)L->f[dfdz], L->np, L->nz, cz, w_nodes);
+  else if (fo) hipLaunchKernelGGL((k_ddz<float, double>), g, dim3(LES_BLOCK), 0, L->stream, (const float*)L->f[f], (double*)L->f[dfdz], L->np, L->nz, cz, w_nodes);
+  else hipLaunchKernelGGL((k_ddz<float, float>), g, dim3(LES_BLOCK), 0, L->stream, (const float*)L->f[f], (float*)L->f[dfdz], L->np, L->nz, cz, w_nodes);
+  LES_CHECK(hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+static int test_filter_t(Les& L, int fid, int alpha) {
+  Cplx<T>* s0 = (Cplx<T>*)L.wk_spec[0];
+  Cplx<T>* s1 = (Cplx<T>*)L.wk_spec[1];
+  const T* src[1] = {F<T>(L, fid)};
+  Cplx<T>* dst[1] = {s0};
+  rows_r2c<T, T>(L, src, dst, 1, 1, L.nz + 1, (T)1);
+  const T* g = alpha == 2 ? (sizeof(T) == 8 ? (const T*)L.g2_64 : (const T*)L.g2_32)
+                          : (sizeof(T) == 8 ? (const T*)L.g4_64 : (const T*)L.g4_32);
+  cols<T>(L, COP_FILT1, s0, nullptr, s1, nullptr, nullptr, 1, 1, 1, 1, L.nz + 1, g);
+  const Cplx<T>* cs[1] = {s1};
+  T* outs[1] = {F<T>(L, fid)};
+  rows_c2r<T, T>(L, cs, outs, 1, 1, L.nz + 1);
+  return 0;
+}
+
+extern "C" {
+int les_ddz_uv(int dfdz, int f) { return ddz_any(dfdz, f, 0); }
+int les_ddz_w(int dfdz, int f) { return ddz_any(dfdz, f, 1); }
+
+int les_compute_dealias_convec(int cx, int cy, int cz) {
+  Les* L = g_les;
+  if (!L) return 1;
+  for (int id : {cx, cy, cz})
+    if (id < 0 || id >= LES_NFIELDS || L->fbytes[id] != 8) { les_set_error("convective output must be an FP64 field"); return 10; }
+  if (L->mixed) convection<float>(*L, D(*L, cx), D(*L, cy), D(*L, cz));
+  else convection<double>(*L, D(*L, cx), D(*L, cy), D(*L, cz));
+  LES_CHECK(hipGetLastError());
+  return 0;
+}
+
+int les_div_stress(int rhs, int tx, int ty, int tz, int stag, int save_wk) {
+  Les* L = g_les;
+  if (!L) return 1;
+  if (L->fbytes[rhs] != 8) { les_set_error("div_stress: rhs must be an FP64 field"); return 10; }
+  for (int id : {tx, ty, tz})
+    if (id < 0 || id >= LES_NFIELDS || field_is_fp64(id)) { les_set_error("div_stress: stress fields expected"); return 10; }
+  if (L->mixed) div_stress<float>(*L, rhs, tx, ty, tz, stag, save_wk, 0);
+  else div_stress<double>(*L, rhs, tx, ty, tz, stag, save_wk, 0);
+  LES_CHECK(hipGetLastError());
+  return 0;
+}
+
+int les_test_filter(int f, int alpha) {
+  Les* L = g_les;
+  if (!L) return 1;
+  if (f < 0 || f >= LES_NFIELDS || field_is_fp64(f)) { les_set_error("test_filter: flow-precision field expected"); return 10; }
+  if (alpha != 2 && alpha != 4) { les_set_error("test_filter: alpha must be 2 or 4"); return 1; }
+  if (L->mixed) test_filter_t<float>(*L, f, alpha); else test_filter_t<double>(*L, f, alpha);
+  LES_CHECK(hipGetLastError());
+  return 0;
+}
+}  // extern "C"

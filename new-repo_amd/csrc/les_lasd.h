@@ -1,0 +1,227 @@
+// Lagrangian scale-dependent dynamic model: test filtering of the Germano
+// inputs, Lagrangian pathline interpolation of F_LM/F_MM/F_QN/F_NN, the
+// relaxation update and the dynamic coefficients
+// (src/compute_LagScaleDep.f90:204-707).  Included by les_gpu.hip.
+#pragma once
+
+// FF(i, j, k) of the level-2 halo array built by update_halo3 and the z
+// fix-ups of compute_interpolag_coeff (src/decomp_2d_custom.f90:902-1024,
+// src/compute_LagScaleDep.f90:439-452) for 1-based i in [0, nx+1],
+// j in [0, ny+1], k in [0, nz+2].  halo_mode 0 reproduces a one-rank run:
+// y halo rows are shifted copies (row 0 <- row 2, row ny+1 <- row ny-1),
+// plane 0 is plane 1 and the halo cells of planes nz+1, nz+2 are those of
+// planes nz-1, nz.  halo_mode 1 is periodic in x and y with clamped z.
+__device__ __forceinline__ long ff_index(int i, int j, int k, int nx, int ny, int nz, int mode) {
+  int kk = k;
+  if (mode == 0) {
+    const bool inter = (i >= 1 && i <= nx && j >= 1 && j <= ny);
+    if (k == 0) kk = 1;
+    if (k >= nz + 1) {
+      if (inter) return ((long)(nz + 1) * ny + (j - 1)) * nx + (i - 1);
+      kk = k - 2;
+    }
+    int jj = j == 0 ? 2 : (j == ny + 1 ? ny - 1 : j);
+    int ii = i == 0 ? nx : (i == nx + 1 ? 1 : i);
+    return ((long)kk * ny + (jj - 1)) * nx + (ii - 1);
+  }
+  if (k == 0) kk = 1;
+  if (k >= nz + 1) kk = nz + 1;
+  int jj = j == 0 ? ny : (j == ny + 1 ? 1 : j);
+  int ii = i == 0 ? nx : (i == nx + 1 ? 1 : i);
+  return ((long)kk * ny + (jj - 1)) * nx + (ii - 1);
+}
+
+__device__ __forceinline__ double fmax32(double a, float b) {
+  const float fa = (float)a;
+  return (double)(fa > b ? fa : b);
+}
+
+// tmp**(-1/8) from correctly rounded operations only
+__device__ __forceinline__ double pow_m18(double t) { return 1.0 / sqrt(sqrt(sqrt(t))); }
+
+template <typename T>
+struct LagrArgs {
+  const T *ulag, *vlag, *wlag;
+  const double* Fo[4];  // previous F_LM, F_MM, F_QN, F_NN
+  double* Fn[4];        // updated
+  const T *LM, *MM, *QN, *NN;
+  T *cs, *beta;
+  double dt, csc, dx, dy, dz, opftdelta, lagran_dt;
+  int nx, ny, nz;
+  long np;
+  int interp;  // 0 on the first update (F initialised from MM, NN)
+  int halo_mode;
+};
+
+template <typename T>
+__global__ void k_lagr(LagrArgs<T> a) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.np * (a.nz + 1)) return;
+  const int k = 1 + (int)(t / a.np);
+  const long q = t % a.np;
+  const int i = 1 + (int)(q % a.nx), j = 1 + (int)(q / a.nx);
+  const long c = (long)k * a.np + q;
+  double F[4];
+  if (a.interp) {
+    // compute_interpolag_weight (src/compute_LagScaleDep.f90:309-411)
+    const double cs = a.csc;
+    double uw, vw, ww;
+    if (k == 1) {
+      uw = (double)a.ulag[c] / cs;
+      vw = (double)a.vlag[c] / cs;
+      ww = 0.25 * ((double)a.wlag[c + a.np] / cs);
+    } else {
+      uw = 0.5 * ((double)a.ulag[c] / cs + (double)a.ulag[c - a.np] / cs);
+      vw = 0.5 * ((double)a.vlag[c] / cs + (double)a.vlag[c - a.np] / cs);
+      ww = (double)a.wlag[c] / cs;
+    }
+    const double xp = -(uw * a.dt * cs / a.dx);
+    const double yp = -(vw * a.dt * cs / a.dy);
+    double zp;
+    if (k <= 2) {
+      const double w2 = (double)a.wlag[2 * a.np + q] / cs;
+      double z2 = -(w2 * a.dt * cs / a.dz);
+      z2 = z2 + fmin(z2, 0.0);
+      z2 = fmin(1.0, z2);
+      zp = fmax(-1.0, z2);
+    } else {
+      zp = -(ww * a.dt * cs / a.dz);
+    }
+    const int ax = i + (signbit(xp) ? -1 : 1);
+    const int ay = j + (signbit(yp) ? -1 : 1);
+    const int az = k + (signbit(zp) ? -1 : 1);
+    const double cx = fabs(xp), cy = fabs(yp), cz = fabs(zp);
+    const double fx = 1.0 - cx, fy = 1.0 - cy, fz = 1.0 - cz;
+    const int nx = a.nx, ny = a.ny, nz = a.nz, hm = a.halo_mode;
+    const long i000 = ff_index(i, j, k, nx, ny, nz, hm), i001 = ff_index(i, j, az, nx, ny, nz, hm);
+    const long i010 = ff_index(i, ay, k, nx, ny, nz, hm), i011 = ff_index(i, ay, az, nx, ny, nz, hm);
+    const long i100 = ff_index(ax, j, k, nx, ny, nz, hm), i101 = ff_index(ax, j, az, nx, ny, nz, hm);
+    const long i110 = ff_index(ax, ay, k, nx, ny, nz, hm), i111 = ff_index(ax, ay, az, nx, ny, nz, hm);
+    // compute_interpolag_coeff (src/compute_LagScaleDep.f90:455-469)
+    for (int f = 0; f < 4; ++f) {
+      const double* G = a.Fo[f];
+      F[f] = fx * fy * (G[i000] * fz + G[i001] * cz) + fx * cy * (G[i010] * fz + G[i011] * cz) +
+             cx * fy * (G[i100] * fz + G[i101] * cz) + cx * cy * (G[i110] * fz + G[i111] * cz);
+    }
+  } else {
+    // first update (src/compute_LagScaleDep.f90:264-277)
+    const double mm = (double)a.MM[c], nn = (double)a.NN[c];
+    F[1] = mm; F[0] = 0.03 * mm;
+    F[3] = nn; F[2] = 0.03 * nn;
+  }
+  // update_interpolag_coeff (src/compute_LagScaleDep.f90:607-640)
+  for (int s = 0; s < 2; ++s) {
+    const double ab = (double)(s == 0 ? a.LM[c] : a.QN[c]);
+    const double bb = (double)(s == 0 ? a.MM[c] : a.NN[c]);
+    double fab = F[2 * s], fbb = F[2 * s + 1];
+    double tmp = fmax32(fab * fbb, 1e-24f);
+    tmp = a.opftdelta * pow_m18(tmp);
+    tmp = fmax32(tmp, 1e-24f);
+    const double dumfac = a.lagran_dt / tmp;
+    const double epsi = dumfac / (1.0 + dumfac);
+    const double nab = epsi * ab + (1.0 - epsi) * fab;
+    const double nbb = epsi * bb + (1.0 - epsi) * fbb;
+    F[2 * s] = fmax32(nab, 1e-24f);
+    F[2 * s + 1] = nbb;
+  }
+  for (int f = 0; f < 4; ++f) a.Fn[f][c] = F[f];
+  // compute_dynamic_coeff (src/compute_LagScaleDep.f90:651-707); the beta
+  // exponent log(2)/(log(4)-log(2)) is exactly 1 in IEEE double.
+  const double c2 = fmax32(F[0] / F[1], 1e-24f);
+  const double c4 = fmax32(F[2] / F[3], 1e-24f);
+  double b = fmax32(c4 / c2, 0.125f);
+  if (k == a.nz + 1) b = 1.0;
+  a.beta[c] = (T)b;
+  a.cs[c] = (T)fmax32(c2 / b, 1e-24f);
+}
+
+// compute_LagScaleDep_coeff on an update step (src/compute_LagScaleDep.f90:237-292)
+template <typename T>
+static void lasd_update(Les& L, int jtt) {
+  (void)jtt;
+  const long npt = L.np * (L.nz + 1);
+  // 15 materialised inputs + the 6 S_ij fields
+  T* in[21];
+  for (int i = 0; i < 9; ++i) in[i] = (T*)L.wk_real[i];
+  in[9] = F<T>(L, LES_S11); in[10] = F<T>(L, LES_S12); in[11] = F<T>(L, LES_S13);
+  in[12] = F<T>(L, LES_S22); in[13] = F<T>(L, LES_S23); in[14] = F<T>(L, LES_S33);
+  for (int i = 15; i < 21; ++i) in[i] = (T*)L.wk_real[i - 6];
+  LasdInArgs<T> ia;
+  for (int i = 0; i < 9; ++i) ia.o[i] = in[i];
+  for (int i = 0; i < 6; ++i) ia.o[9 + i] = in[15 + i];
+  ia.u = D(L, LES_U); ia.v = D(L, LES_V); ia.w = D(L, LES_W);
+  for (int i = 0; i < 6; ++i) ia.s[i] = in[9 + i];
+  ia.smod = (T*)L.wk_real[63];
+  ia.np = L.np; ia.nz = L.nz;
+  kt_begin(L, KT_POINTWISE);
+  hipLaunchKernelGGL((k_lasd_inputs<T>), dim3(nblk(npt)), dim3(LES_BLOCK), 0, L.stream, ia);
+  kt_end(L, KT_POINTWISE);
+  // test filters at 2 delta and 4 delta from one forward transform per field
+  const T* g2 = sizeof(T) == 8 ? (const T*)L.g2_64 : (const T*)L.g2_32;
+  const T* g4 = sizeof(T) == 8 ? (const T*)L.g4_64 : (const T*)L.g4_32;
+  T* fl2[21];
+  T* fl4[21];
+  for (int f = 0; f < 21; ++f) { fl2[f] = (T*)L.wk_real[15 + f]; fl4[f] = (T*)L.wk_real[36 + f]; }
+  for (int f0 = 0; f0 < 21; f0 += 3) {
+    const T* src[3];
+    Cplx<T>* sp[3];
+    for (int t = 0; t < 3; ++t) { src[t] = in[f0 + t]; sp[t] = (Cplx<T>*)L.wk_spec[t]; }
+    rows_r2c<T, T>(L, src, sp, 3, 1, L.nz + 1, (T)1);
+    for (int t = 0; t < 3; ++t) {
+      Cplx<T>* o2 = (Cplx<T>*)L.wk_spec[3 + 2 * t];
+      Cplx<T>* o4 = (Cplx<T>*)L.wk_spec[4 + 2 * t];
+      cols<T>(L, COP_FILT2, sp[t], nullptr, o2, o4, nullptr, 2, 1, 1, 1, L.nz + 1, g2, g4);
+    }
+    const Cplx<T>* cs[6];
+    T* outs[6];
+    for (int t = 0; t < 3; ++t) {
+      cs[2 * t] = (Cplx<T>*)L.wk_spec[3 + 2 * t];
+      cs[2 * t + 1] = (Cplx<T>*)L.wk_spec[4 + 2 * t];
+      outs[2 * t] = fl2[f0 + t];
+      outs[2 * t + 1] = fl4[f0 + t];
+    }
+    rows_c2r<T, T>(L, cs, outs, 6, 1, L.nz + 1);
+  }
+  T* LM = (T*)L.wk_real[57];
+  T* MM = (T*)L.wk_real[58];
+  T* QN = (T*)L.wk_real[59];
+  T* NN = (T*)L.wk_real[60];
+  for (int s = 0; s < 2; ++s) {
+    GermanoArgs<T> ga;
+    for (int f = 0; f < 21; ++f) ga.fl[f] = s == 0 ? fl2[f] : fl4[f];
+    ga.AB = s == 0 ? LM : QN;
+    ga.BB = s == 0 ? MM : NN;
+    ga.tf2 = (T)(s == 0 ? 4.0 : 16.0);
+    ga.cst = (T)(2.0 * (L.delta * L.delta));
+    ga.np = L.np; ga.nz = L.nz;
+    kt_begin(L, KT_POINTWISE);
+    hipLaunchKernelGGL((k_germano<T>), dim3(nblk(npt)), dim3(LES_BLOCK), 0, L.stream, ga);
+    kt_end(L, KT_POINTWISE);
+  }
+  // Lagrangian interpolation, relaxation and coefficients; F_* double buffered
+  for (int f = 0; f < 4; ++f) {
+    void* tmp = L.f[LES_F_LM + f];
+    L.f[LES_F_LM + f] = (void*)L.Fold[f];
+    L.Fold[f] = (double*)tmp;
+  }
+  LagrArgs<T> la;
+  la.ulag = F<T>(L, LES_U_LAG); la.vlag = F<T>(L, LES_V_LAG); la.wlag = F<T>(L, LES_W_LAG);
+  for (int f = 0; f < 4; ++f) { la.Fo[f] = L.Fold[f]; la.Fn[f] = D(L, LES_F_LM + f); }
+  la.LM = LM; la.MM = MM; la.QN = QN; la.NN = NN;
+  la.cs = F<T>(L, LES_CS_OPT2); la.beta = F<T>(L, LES_BETA);
+  la.dt = L.P.dt; la.csc = (double)L.P.cs_count;
+  la.dx = L.dx; la.dy = L.dy; la.dz = L.dz;
+  la.opftdelta = 1.5 * L.delta;
+  la.lagran_dt = L.P.dt * (double)L.P.cs_count;
+  la.nx = L.nx; la.ny = L.ny; la.nz = L.nz; la.np = L.np;
+  la.interp = L.F_coeff_init;
+  la.halo_mode = L.P.halo_mode;
+  kt_begin(L, KT_LAGR);
+  hipLaunchKernelGGL((k_lagr<T>), dim3(nblk(npt)), dim3(LES_BLOCK), 0, L.stream, la);
+  kt_end(L, KT_LAGR);
+  L.F_coeff_init = 1;
+  const size_t tb = sizeof(T) * (size_t)L.nf;
+  hipMemsetAsync(L.f[LES_U_LAG], 0, tb, L.stream);
+  hipMemsetAsync(L.f[LES_V_LAG], 0, tb, L.stream);
+  hipMemsetAsync(L.f[LES_W_LAG], 0, tb, L.stream);
+}

@@ -1,0 +1,173 @@
+// Pressure Poisson solve and projection (src/compute_pressure.f90:116-372,
+// src/LES_engine.f90:299-352).  The pressure path runs in FP64 in every
+// precision mode.  Included by les_gpu.hip.
+#pragma once
+
+// Per-(kx, ky) tridiagonal system of length nz+2 solved by the Thomas
+// algorithm, one thread per mode, with the divergence right-hand side built
+// on the fly (src/compute_pressure.f90:205-216, 256-305, 448-493, 544-619).
+struct TriArgs {
+  const Cplx<double> *Hx, *Hy, *Hz;   // y-transformed spectra, planes 1..nz
+  const Cplx<double> *bot, *top;      // BC layer spectra
+  const double *kkx, *kky;
+  Cplx<double>* phat;                 // planes 0..nz+1
+  double* gam;                        // (nz+3) * ns scratch
+  int lhx, ny, nz;
+  long ns;
+  double dz, idz2;
+};
+
+__global__ void k_tridiag(TriArgs a) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= a.ns) return;
+  const int i = (int)(q % a.lhx), j = (int)(q / a.lhx);
+  const int nz = a.nz;
+  const long ns = a.ns;
+  if (i == a.lhx - 1 || j == a.ny / 2) {
+    for (int k = 0; k <= nz + 1; ++k) a.phat[(long)k * ns + q] = {0.0, 0.0};
+    return;
+  }
+  const double dz = a.dz, idz2 = a.idz2;
+  const double kx = a.kkx[q], ky = a.kky[q];
+  const double bmid = -((kx * kx + ky * ky) + 2.0 / (dz * dz));
+  const bool mean = (i == 0 && j == 0);
+  // row 1: bottom layer (mean mode pinned)
+  Cplx<double> r1 = mean ? Cplx<double>{0.0, 0.0} : Cplx<double>{a.bot[q].x * dz, a.bot[q].y * dz};
+  double bet = mean ? 1.0 : -1.0;
+  const double c1 = mean ? 0.0 : 1.0;
+  Cplx<double> x = {r1.x / bet, r1.y / bet};
+  a.phat[q] = x;
+  Cplx<double> hz = a.Hz[ns + q];
+  for (int k = 2; k <= nz + 2; ++k) {
+    Cplx<double> r;
+    double am, bk;
+    if (k <= nz + 1) {
+      const long p = (long)(k - 1) * ns + q;
+      const Cplx<double> hx = a.Hx[p], hy = a.Hy[p];
+      const Cplx<double> hzp = (k - 1 < nz) ? a.Hz[p + ns] : Cplx<double>{0.0, 0.0};
+      const double sr = kx * hx.x + ky * hy.x;
+      const double si = kx * hx.y + ky * hy.y;
+      r = {-si + (hzp.x - hz.x) / dz, sr + (hzp.y - hz.y) / dz};
+      hz = hzp;
+      am = idz2;
+      bk = bmid;
+    } else {
+      r = {a.top[q].x * dz, a.top[q].y * dz};
+      am = -1.0;
+      bk = 1.0;
+    }
+    const double cprev = (k == 2) ? c1 : idz2;
+    const double g = cprev / bet;
+    a.gam[(long)k * ns + q] = g;
+    bet = bk - am * g;
+    x = {(r.x - am * x.x) / bet, (r.y - am * x.y) / bet};
+    if (k <= nz + 1) a.phat[(long)(k - 1) * ns + q] = x;
+  }
+  // back substitution: x holds x(nz+2)
+  for (int k = nz + 1; k >= 1; --k) {
+    const long p = (long)(k - 1) * ns + q;
+    const double g = a.gam[(long)(k + 1) * ns + q];
+    const Cplx<double> xk = a.phat[p];
+    x = {xk.x - g * x.x, xk.y - g * x.y};
+    a.phat[p] = x;
+  }
+  a.phat[(long)(nz + 1) * ns + q] = {0.0, 0.0};
+}
+
+template <typename T>
+static void press_stag_array(Les& L, double tadv1) {
+  const int nz = L.nz;
+  const double c = 1.0 / (double)(L.nx * L.ny);
+  Cplx<double>* Hx = (Cplx<double>*)L.pspec[0];
+  Cplx<double>* Hy = (Cplx<double>*)L.pspec[1];
+  Cplx<double>* Hz = (Cplx<double>*)L.pspec[2];
+  Cplx<double>* ph = (Cplx<double>*)L.pspec[3];
+  // rH = const/tadv1 * (u/dt) on levels 1..nz (src/compute_pressure.f90:136-149)
+  const double* src[3] = {D(L, LES_U), D(L, LES_V), D(L, LES_W)};
+  Cplx<double>* dst[3] = {Hx, Hy, Hz};
+  rows_r2c<double, double>(L, src, dst, 3, 1, nz, c / tadv1, LOAD_SCALE_DIV, L.P.dt);
+  for (int f = 0; f < 3; ++f) cols<double>(L, COP_COPY, dst[f], nullptr, dst[f], nullptr, nullptr, 1, 1, 0, 1, nz);
+  // bottom and top layers (src/compute_pressure.f90:152-165, 196-203)
+  Cplx<double>* lb = (Cplx<double>*)L.playr[0];
+  Cplx<double>* lt = (Cplx<double>*)L.playr[1];
+  const double* lsrc[2] = {L.layer[LES_BOTTOM_WK], L.layer[LES_TOP_WK]};
+  Cplx<double>* ldst[2] = {lb, lt};
+  rows_r2c<double, double>(L, lsrc, ldst, 2, 0, 1, c);
+  cols<double>(L, COP_COPY, lb, nullptr, lb, nullptr, nullptr, 1, 1, 0, 0, 1);
+  cols<double>(L, COP_COPY, lt, nullptr, lt, nullptr, nullptr, 1, 1, 0, 0, 1);
+  // tridiagonal solve per mode
+  TriArgs t;
+  t.Hx = Hx; t.Hy = Hy; t.Hz = Hz; t.bot = lb; t.top = lt;
+  t.kkx = L.kkx64; t.kky = L.kky64;
+  t.phat = ph;
+  t.gam = L.pspec[4];
+  t.lhx = L.lhx; t.ny = L.ny; t.nz = nz; t.ns = L.ns;
+  t.dz = L.dz; t.idz2 = 1.0 / (L.dz * L.dz);
+  kt_begin(L, KT_TRIDIAG);
+  hipLaunchKernelGGL(k_tridiag, dim3(nblk(L.ns, 64)), dim3(64), 0, L.stream, t);
+  kt_end(L, KT_TRIDIAG);
+  // gradient: i kx p, i ky p and p back to physical space (src/compute_pressure.f90:320-357)
+  cols<double>(L, COP_PGRAD, ph, nullptr, Hx, Hy, Hz, 3, 0, 1, 0, nz + 2);
+  const Cplx<double>* cs[3] = {Hx, Hy, Hz};
+  double* outs[3] = {D(L, LES_DPDX), D(L, LES_DPDY), D(L, LES_P)};
+  rows_c2r<double, double>(L, cs, outs, 3, 0, nz + 2);
+}
+
+// dpdz, projection, rhs -= grad p, w BCs and set_bc
+// (src/compute_pressure.f90:353-369; src/LES_engine.f90:299-352)
+struct ProjArgs {
+  double *u, *v, *w, *p, *dpdx, *dpdy, *dpdz, *rx, *ry, *rz;
+  const double *lbc_u, *lbc_v;
+  double dtt, cz, dz;
+  long np;
+  int nz;
+  int ubc_u_neu, ubc_v_neu;
+  double ubc_u, ubc_v, ubc_w, bogus;
+};
+
+__global__ void k_proj(ProjArgs a) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.np * a.nz) return;
+  const int k = 1 + (int)(t / a.np);
+  const long q = t % a.np;
+  const long c = (long)k * a.np + q;
+  const double gz = a.cz * (a.p[c] - a.p[c - a.np]);
+  a.dpdz[c] = gz;
+  const double gx = a.dpdx[c], gy = a.dpdy[c];
+  const double un = a.u[c] - a.dtt * gx;
+  const double vn = a.v[c] - a.dtt * gy;
+  double wn = a.w[c] - a.dtt * gz;
+  a.rx[c] = a.rx[c] - gx;
+  a.ry[c] = a.ry[c] - gy;
+  a.rz[c] = a.rz[c] - gz;
+  if (k == 1) {
+    wn = 0.0;
+    a.u[q] = un - a.lbc_u[q] * a.dz;
+    a.v[q] = vn - a.lbc_v[q] * a.dz;
+  }
+  a.u[c] = un;
+  a.v[c] = vn;
+  a.w[c] = wn;
+  if (k == a.nz) {
+    const long tp = c + a.np;
+    a.u[tp] = a.ubc_u_neu ? un + a.ubc_u * a.dz : 2.0 * a.ubc_u - un;
+    a.v[tp] = a.ubc_v_neu ? vn + a.ubc_v * a.dz : 2.0 * a.ubc_v - vn;
+    a.w[tp] = a.ubc_w;
+    a.p[tp] = a.bogus;
+  }
+}
+
+static void projection(Les& L, double tadv1) {
+  ProjArgs a;
+  a.u = D(L, LES_U); a.v = D(L, LES_V); a.w = D(L, LES_W); a.p = D(L, LES_P);
+  a.dpdx = D(L, LES_DPDX); a.dpdy = D(L, LES_DPDY); a.dpdz = D(L, LES_DPDZ);
+  a.rx = D(L, LES_RHSX); a.ry = D(L, LES_RHSY); a.rz = D(L, LES_RHSZ);
+  a.lbc_u = L.layer[LES_LBC_U]; a.lbc_v = L.layer[LES_LBC_V];
+  a.dtt = L.P.dt * tadv1; a.cz = 1.0 / L.dz; a.dz = L.dz;
+  a.np = L.np; a.nz = L.nz;
+  a.ubc_u_neu = L.P.ubc_u_neu; a.ubc_v_neu = L.P.ubc_v_neu;
+  a.ubc_u = L.P.ubc_u; a.ubc_v = L.P.ubc_v; a.ubc_w = L.P.ubc_w; a.bogus = BOGUS;
+  kt_begin(L, KT_POINTWISE);
+  hipLaunchKernelGGL(k_proj, dim3(nblk(L.np * L.nz)), dim3(LES_BLOCK), 0, L.stream, a);
+  kt_end(L, KT_POINTWISE);
+}

@@ -1,0 +1,228 @@
+// Right-hand side of the momentum equations: 3/2-padded rotational
+// convection and the divergence of the SGS stress (src/LES_engine.f90:509-572).
+// Included by les_gpu.hip.
+#pragma once
+
+// Products on the padded grid (src/compute_convective_padd.f90:179-249);
+// velocities on the padded grid are the negated ones, as in the reference.
+template <typename T>
+struct ProdArgs {
+  const T *ub, *vb, *wb, *o1, *o2, *o3;
+  T *px, *py, *pz;
+  long nbp;
+  int nz;
+  T c2;
+};
+
+template <typename T>
+__global__ void k_conv_prod(ProdArgs<T> a) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.nbp * a.nz) return;
+  const int k = 1 + (int)(t / a.nbp);
+  const long q = t % a.nbp;
+  const long c = (long)k * a.nbp + q, up = c + a.nbp, dn = c - a.nbp;
+  const T c2 = a.c2, h = (T)0.5;
+  T x, y, z;
+  if (k == 1) {
+    x = c2 * (-(a.vb[c] * a.o3[c]) + h * a.wb[up] * a.o2[up]);
+    y = c2 * (a.ub[c] * a.o3[c] + h * a.wb[up] * (-a.o1[up]));
+    z = (T)0;
+  } else {
+    x = c2 * (-(a.vb[c] * a.o3[c]) + h * (a.wb[up] * a.o2[up] + a.wb[c] * a.o2[c]));
+    y = c2 * (a.ub[c] * a.o3[c] + h * (a.wb[c] * (-a.o1[c]) + a.wb[up] * (-a.o1[up])));
+    z = c2 * h * ((a.ub[c] + a.ub[dn]) * (-a.o2[c]) + (a.vb[c] + a.vb[dn]) * a.o1[c]);
+  }
+  a.px[c] = x;
+  a.py[c] = y;
+  a.pz[c] = z;
+}
+
+struct SetPlaneArgs {
+  double *a, *b, *c;
+  long np;
+  int k0, k1;
+  double val;
+};
+
+__global__ void k_set_planes(SetPlaneArgs s) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= s.np) return;
+  s.a[(long)s.k0 * s.np + q] = s.val; s.b[(long)s.k0 * s.np + q] = s.val; s.c[(long)s.k0 * s.np + q] = s.val;
+  s.a[(long)s.k1 * s.np + q] = s.val; s.b[(long)s.k1 * s.np + q] = s.val; s.c[(long)s.k1 * s.np + q] = s.val;
+}
+
+// compute_dealias_convec (src/compute_convective_padd.f90:115-262)
+template <typename T>
+static void convection(Les& L, double* cx, double* cy, double* cz) {
+  const int nx = L.nx, ny = L.ny, nz = L.nz, nx2 = L.nx2, ny2 = L.ny2;
+  const T c = (T)(1.0 / (double)(nx * ny));
+  // pad in x: planes 1..nz+1 of u, v, w and the vorticities
+  {
+    RowResArgs<T, T, T> a;
+    a.cs.u = D(L, LES_U); a.cs.v = D(L, LES_V); a.cs.w = D(L, LES_W);
+    a.cs.dudy = F<T>(L, LES_DUDY); a.cs.dudz = F<T>(L, LES_DUDZ);
+    a.cs.dvdx = F<T>(L, LES_DVDX); a.cs.dvdz = F<T>(L, LES_DVDZ);
+    a.cs.dwdx = F<T>(L, LES_DWDX); a.cs.dwdy = F<T>(L, LES_DWDY);
+    a.cconv = c;
+    a.src_mode = 1;
+    for (int f = 0; f < 6; ++f) { a.src[f] = nullptr; a.dst[f] = (T*)L.wk_hyb[f]; }
+    a.n1 = nx; a.n2 = nx2; a.ny = ny; a.k0 = 1; a.nplanes = nz + 1;
+    a.src_plane = (long)nx * ny; a.dst_plane = (long)nx2 * ny;
+    const int nm = nx2;
+    a.lpb = lines_per_block<T>(nm);
+    long jobs = (long)a.nplanes * (ny / 2);
+    size_t sh = 2 * (size_t)a.lpb * nm * sizeof(Cplx<T>);
+    kt_begin(L, KT_ROWS_RES);
+    hipLaunchKernelGGL((k_rows_resample<T, T, T>), dim3((unsigned)((jobs + a.lpb - 1) / a.lpb), 6), dim3(LES_BLOCK), sh,
+                       L.stream, a, L.px, L.px2, twpool<T>(L));
+    kt_end(L, KT_ROWS_RES);
+  }
+  // pad in y
+  {
+    ColResArgs<T> a;
+    for (int f = 0; f < 6; ++f) { a.src[f] = (const T*)L.wk_hyb[f]; a.dst[f] = (T*)L.wk_big[f]; }
+    a.W = nx2; a.h1 = ny; a.h2 = ny2; a.k0 = 1; a.nplanes = nz + 1;
+    a.src_plane = (long)nx2 * ny; a.dst_plane = L.nbp;
+    a.cp = lines_per_block<T>(ny2);
+    a.ncp = (nx2 / 2 + a.cp - 1) / a.cp;
+    size_t sh = 2 * (size_t)a.cp * ny2 * sizeof(Cplx<T>);
+    kt_begin(L, KT_COLS_RES);
+    hipLaunchKernelGGL((k_cols_resample<T>), dim3((unsigned)(a.ncp * a.nplanes), 6), dim3(LES_BLOCK), sh, L.stream, a,
+                       L.py, L.py2, twpool<T>(L));
+    kt_end(L, KT_COLS_RES);
+  }
+  // products
+  {
+    ProdArgs<T> a;
+    a.ub = (const T*)L.wk_big[0]; a.vb = (const T*)L.wk_big[1]; a.wb = (const T*)L.wk_big[2];
+    a.o1 = (const T*)L.wk_big[3]; a.o2 = (const T*)L.wk_big[4]; a.o3 = (const T*)L.wk_big[5];
+    a.px = (T*)L.wk_prod[0]; a.py = (T*)L.wk_prod[1]; a.pz = (T*)L.wk_prod[2];
+    a.nbp = L.nbp; a.nz = nz;
+    a.c2 = (T)(1.0 / (double)(nx2 * ny2));
+    kt_begin(L, KT_POINTWISE);
+    hipLaunchKernelGGL((k_conv_prod<T>), dim3(nblk(L.nbp * nz)), dim3(LES_BLOCK), 0, L.stream, a);
+    kt_end(L, KT_POINTWISE);
+  }
+  // truncate in y
+  {
+    ColResArgs<T> a;
+    for (int f = 0; f < 3; ++f) { a.src[f] = (const T*)L.wk_prod[f]; a.dst[f] = (T*)L.wk_hyb3[f]; }
+    a.W = nx2; a.h1 = ny2; a.h2 = ny; a.k0 = 1; a.nplanes = nz;
+    a.src_plane = L.nbp; a.dst_plane = (long)nx2 * ny;
+    a.cp = lines_per_block<T>(ny2);
+    a.ncp = (nx2 / 2 + a.cp - 1) / a.cp;
+    size_t sh = 2 * (size_t)a.cp * ny2 * sizeof(Cplx<T>);
+    kt_begin(L, KT_COLS_RES);
+    hipLaunchKernelGGL((k_cols_resample<T>), dim3((unsigned)(a.ncp * a.nplanes), 3), dim3(LES_BLOCK), sh, L.stream, a,
+                       L.py2, L.py, twpool<T>(L));
+    kt_end(L, KT_COLS_RES);
+  }
+  // truncate in x, straight into rhs
+  {
+    RowResArgs<T, T, double> a;
+    a.src[0] = (const T*)L.wk_hyb3[0]; a.src[1] = (const T*)L.wk_hyb3[1]; a.src[2] = (const T*)L.wk_hyb3[2];
+    a.dst[0] = cx; a.dst[1] = cy; a.dst[2] = cz;
+    a.cconv = (T)0;
+    a.src_mode = 0;
+    a.n1 = nx2; a.n2 = nx; a.ny = ny; a.k0 = 1; a.nplanes = nz;
+    a.src_plane = (long)nx2 * ny; a.dst_plane = L.np;
+    a.lpb = lines_per_block<T>(nx2);
+    long jobs = (long)nz * (ny / 2);
+    size_t sh = 2 * (size_t)a.lpb * nx2 * sizeof(Cplx<T>);
+    kt_begin(L, KT_ROWS_RES);
+    hipLaunchKernelGGL((k_rows_resample<T, T, double>), dim3((unsigned)((jobs + a.lpb - 1) / a.lpb), 3), dim3(LES_BLOCK),
+                       sh, L.stream, a, L.px2, L.px, twpool<T>(L));
+    kt_end(L, KT_ROWS_RES);
+  }
+  SetPlaneArgs sp;
+  sp.a = cx; sp.b = cy; sp.c = cz; sp.np = L.np; sp.k0 = 0; sp.k1 = nz + 1; sp.val = BOGUS;
+  hipLaunchKernelGGL(k_set_planes, dim3(nblk(L.np)), dim3(LES_BLOCK), 0, L.stream, sp);
+}
+
+// rhs(k) += d/dx tx + d/dy ty + d/dz tz for k = 1..nz, plus the forcing of
+// compute_RHS_adv; optionally the pressure BC layers
+// (src/compute_derivatives.f90:545-574; src/LES_engine.f90:547-562).
+template <typename T>
+struct DivEpiArgs {
+  double* rhs;
+  const T* divh;
+  const T* tz;
+  int stag, save_wk;
+  double *bottom_wk, *top_wk;
+  double cz;
+  long np;
+  int nz;
+  int forcing;  // 1: x forcing, 2: y forcing
+  double pf, coriol, geo;
+  const double* vel;  // v for x, u for y (Coriolis)
+  int coriolis;
+};
+
+template <typename T>
+__global__ void k_div_epi(DivEpiArgs<T> a) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.np * a.nz) return;
+  const int k = 1 + (int)(t / a.np);
+  const long q = t % a.np;
+  const long c = (long)k * a.np + q;
+  const double dzt = a.stag ? a.cz * ((double)a.tz[c + a.np] - (double)a.tz[c])
+                            : a.cz * ((double)a.tz[c] - (double)a.tz[c - a.np]);
+  double r = a.rhs[c] + (double)a.divh[c] + dzt;
+  if (a.forcing == 1) {
+    r = r + a.pf;
+    if (a.coriolis) r = r + a.coriol * a.vel[c] - a.coriol * a.geo;
+  } else if (a.forcing == 2) {
+    r = r + a.pf;
+    if (a.coriolis) r = r - a.coriol * a.vel[c] + a.coriol * a.geo;
+  }
+  a.rhs[c] = r;
+  if (a.save_wk && k == 1) {
+    a.bottom_wk[q] = (double)a.divh[c] + a.cz * ((double)a.tz[c] - (double)a.tz[q]);
+    a.top_wk[q] = (double)a.divh[(long)(a.nz + 1) * a.np + q];
+  }
+}
+
+template <typename T>
+static void div_stress(Les& L, int rhs_id, int tx, int ty, int tz, int stag, int save_wk, int forcing) {
+  Cplx<T>* s0 = (Cplx<T>*)L.wk_spec[0];
+  Cplx<T>* s1 = (Cplx<T>*)L.wk_spec[1];
+  Cplx<T>* s2 = (Cplx<T>*)L.wk_spec[2];
+  const T* src[2] = {F<T>(L, tx), F<T>(L, ty)};
+  Cplx<T>* dst[2] = {s0, s1};
+  const T c = (T)(1.0 / (double)(L.nx * L.ny));
+  rows_r2c<T, T>(L, src, dst, 2, 1, L.nz + 1, c);
+  cols<T>(L, COP_DIV, s0, s1, s2, nullptr, nullptr, 1, 1, 1, 1, L.nz + 1);
+  T* divh = (T*)L.wk_real[62];
+  const Cplx<T>* cs[1] = {s2};
+  T* outs[1] = {divh};
+  rows_c2r<T, T>(L, cs, outs, 1, 1, L.nz + 1);
+  DivEpiArgs<T> a;
+  a.rhs = D(L, rhs_id); a.divh = divh; a.tz = F<T>(L, tz);
+  a.stag = stag; a.save_wk = save_wk;
+  a.bottom_wk = L.layer[LES_BOTTOM_WK]; a.top_wk = L.layer[LES_TOP_WK];
+  a.cz = 1.0 / L.dz; a.np = L.np; a.nz = L.nz;
+  a.forcing = forcing;
+  a.pf = forcing == 1 ? L.P.pfx : L.P.pfy;
+  a.coriol = L.P.coriol;
+  a.geo = forcing == 1 ? L.P.vg : L.P.ug;
+  a.vel = forcing == 1 ? D(L, LES_V) : D(L, LES_U);
+  a.coriolis = L.P.coriolis;
+  kt_begin(L, KT_POINTWISE);
+  hipLaunchKernelGGL((k_div_epi<T>), dim3(nblk(L.np * L.nz)), dim3(LES_BLOCK), 0, L.stream, a);
+  kt_end(L, KT_POINTWISE);
+}
+
+template <typename T>
+static void compute_rhs_adv(Les& L) {
+  // rhs_f <- rhs (src/LES_engine.f90:517-522) as a buffer swap: the new rhs
+  // is fully overwritten by the convective term below.
+  for (int c = 0; c < 3; ++c) {
+    void* t = L.f[LES_RHSX + c];
+    L.f[LES_RHSX + c] = L.f[LES_RHSX_F + c];
+    L.f[LES_RHSX_F + c] = t;
+  }
+  convection<T>(L, D(L, LES_RHSX), D(L, LES_RHSY), D(L, LES_RHSZ));
+  div_stress<T>(L, LES_RHSX, LES_TXX, LES_TXY, LES_TXZ, 1, 0, 1);
+  div_stress<T>(L, LES_RHSY, LES_TXY, LES_TYY, LES_TYZ, 1, 0, 2);
+  div_stress<T>(L, LES_RHSZ, LES_TXZ, LES_TYZ, LES_TZZ, 0, 1, 0);
+}

@@ -1,0 +1,76 @@
+// Device state of the LES library: one global instance (the reference keeps
+// the same data as Fortran module globals, src/system.f90:21-47).
+#pragma once
+#include <vector>
+
+#include "../../include/les_gpu.h"
+#include "fft_kernels.h"
+
+// Kernel families timed with HIP events when timing is enabled.
+enum {
+  KT_ROWS_R2C = 0, KT_ROWS_C2R, KT_COLS, KT_ROWS_RES, KT_COLS_RES, KT_POINTWISE, KT_TRIDIAG, KT_LAGR, KT_MLP, KT_COUNT
+};
+
+struct Les {
+  les_params P;
+  int nx, ny, nz, nzg, lhx, nx2, ny2;
+  long np, nf, ns, nsp;  // nx*ny, nx*ny*nzg, lhx*ny, lhx*ny*nzg
+  long nbp;              // nx2*ny2
+  double dx, dy, dz, delta;
+  int jt_total;
+  int F_coeff_init;
+  hipStream_t stream;
+  int mixed;             // flow kernels in FP32
+
+  // host-built FFT plans and twiddles
+  FftPlan px, py, px2, py2;
+  Cplx<double>* tw64;
+  Cplx<float>* tw32;
+
+  // tables (double and float copies)
+  double *kkx64, *kky64, *g2_64, *g4_64;
+  float *kkx32, *kky32, *g2_32, *g4_32;
+  double* logterm;  // log(0.5 dz / z0) per surface point
+  double* lvec;     // mixing length per level (nz+2)
+
+  // fields: index by les_field; elem size per field
+  void* f[LES_NFIELDS];
+  int fbytes[LES_NFIELDS];
+  double* layer[LES_NLAYERS];
+  double* z0;
+
+  // workspaces
+  void* wk_spec[8];       // spectral (lhx, ny, nzg) of flow precision
+  void* wk_real[64];      // real (nx, ny, nzg) of flow precision
+  void* wk_hyb[6];        // (nx2, ny, nzg)
+  void* wk_big[6];        // (nx2, ny2, nzg)
+  void* wk_prod[3];       // (nx2, ny2, nzg)
+  void* wk_hyb3[3];       // (nx2, ny, nzg)
+  double* pspec[5];       // pressure spectra (double): Hx, Hy, Hz, phat, tmp
+  double* playr[4];       // bottom/top layer spectra and scratch
+  void* wk_layer[6];      // (nx, ny) planes of flow precision and spectra
+  double* Fold[4];        // previous F_* for the Lagrangian gather
+  double* nyq_rows;       // mixed-mode FP64 Nyquist removal partial sums
+  int* nanflag;
+
+  // learned SGS
+  float* mlp_w;           // packed weights (device)
+  long mlp_nw;
+  int mlp_ready;
+  double judge[4];
+
+  // timing
+  int timing;
+  hipEvent_t ev_a[KT_COUNT], ev_b[KT_COUNT];
+  double kt_ms[KT_COUNT];
+  long kt_calls[KT_COUNT];
+  hipEvent_t ph_ev[6];
+  double phase_ms[5];
+  std::vector<void*> allocs;
+};
+
+extern Les* g_les;
+
+// helpers implemented in les_gpu.hip
+template <typename T>
+const Cplx<T>* twpool(const Les& L);
