@@ -1,0 +1,187 @@
+"""Benchmark of the MI355X LES time step (one GPU).
+
+Metric: Mpoint-steps/s of the LES time step at 256^3 (nx*ny*nz*steps /
+seconds / 1e6), the reference's `time_loop` region (src/LES_engine.f90:191-353:
+wall law, gradients, SGS with the Lagrangian dynamic model, padded
+convection, divergence of stress, AB2, pressure and projection).  Inputs are
+already resident in HBM when the timed region starts.
+
+The headline value is the fastest available variant in the order
+learned (FP32 + learned SGS) > mixed (FP32 flow, FP64 pressure/averages/AB2)
+> fp64; every variant that runs is reported, with the FP64 step as the
+baseline of the speed-up.  Output: one JSON line on stdout.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import os
+import pathlib
+import sys
+import time
+
+import numpy as np
+
+ROOT = pathlib.Path(__file__).resolve().parent
+PEAK_HBM = 8.0e12  # B/s, MI355X
+FAMILIES = ["rows_r2c", "rows_c2r", "cols", "rows_resample", "cols_resample", "pointwise", "tridiag", "lagrangian",
+            "mlp"]
+# compulsory bytes per point-step of the whole step (cs_count = 5)
+STEP_BYTES = {"fp64": 836.8, "mixed": 623.2}
+
+
+def _load(name, path, search=None):
+    if name in sys.modules:
+        return sys.modules[name]
+    spec = importlib.util.spec_from_file_location(name, path, submodule_search_locations=search)
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def load_pkg():
+    return _load("new_repo_amd", ROOT / "new-repo_amd" / "__init__.py", [str(ROOT / "new-repo_amd")])
+
+
+def run_variant(pkg, P, ic, precision, steps, warmup, nx, ny, nz, kernel_stats=True, seed=1234):
+    prm = P.LesParams(nx=nx, ny=ny, nz=nz, precision=precision)
+    prm.validate()
+    u, v, w = ic.log_law_ic(nx, ny, nz, prm.lz, prm.z0, 0.10, seed=seed)
+    res = {}
+    with pkg.LesGpu(prm) as g:
+        g.set_ic(u, v, w)
+        g.step(warmup)
+        g.sync()
+        t0 = time.perf_counter()
+        g.step(steps)
+        g.sync()
+        t1 = time.perf_counter()
+        if g.check_nan():
+            raise RuntimeError("NaN in velocity after the timed steps")
+        secs = t1 - t0
+        res["seconds"] = secs
+        res["ms_per_step"] = 1e3 * secs / steps
+        res["mpts"] = nx * ny * nz * steps / secs / 1e6
+        if kernel_stats:
+            # second pass with HIP events around every launch on the library's
+            # stream; one full LASD cycle so every kernel family appears
+            g.timers(reset=True)
+            g.set_timing(True)
+            g.step(prm.cs_count)
+            g.sync()
+            t = g.timers(reset=True)
+            g.set_timing(False)
+            fam = {}
+            for i, name in enumerate(FAMILIES):
+                ms, calls, nbytes = t[5 + 3 * i], t[6 + 3 * i], t[7 + 3 * i]
+                if calls > 0:
+                    fam[name] = {"ms": ms, "calls": int(calls), "bytes": nbytes,
+                                 "avg_ms": ms / calls, "gbs": nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0}
+            res["families"] = fam
+            res["phase_ms"] = [float(x) / prm.cs_count for x in t[:5]]
+    return res
+
+
+def cpu_baseline(P, ic, nx, ny, nz, nsteps):
+    """The FP64 CPU restatement of the step (oracle/) on this host's cores,
+    timed over one LASD cycle at the benchmark grid."""
+    O = _load("les_oracle_py", ROOT / "oracle" / "oracle.py")
+    prm = P.LesParams(nx=nx, ny=ny, nz=nz)
+    threads = O.Oracle.threads(int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
+    o = O.Oracle(prm)
+    u, v, w = ic.log_law_ic(nx, ny, nz, prm.lz, prm.z0, 0.10, seed=1234)
+    o.set_ic(u, v, w)
+    t0 = time.perf_counter()
+    o.step(nsteps)
+    secs = time.perf_counter() - t0
+    return {"value": nx * ny * nz * nsteps / secs / 1e6, "unit": "Mpt-steps/s", "cores": threads, "kind": "port",
+            "sample": f"FP64 oracle (oracle/les_oracle.cpp, OpenMP) steps 1-{nsteps} at {nx}x{ny}x{nz}, "
+                      f"one LASD update included, {secs:.1f} s"}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--variants", default="auto", help="comma list of fp64,mixed,learned or auto")
+    ap.add_argument("--grid", default="256,256,256")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-steps", type=int, default=5)
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}): the whole domain lives on a single MI355X "
+              f"and no multi-GPU path is built.", file=sys.stderr)
+        return 2
+    nx, ny, nz = (int(x) for x in args.grid.split(","))
+    pkg = load_pkg()
+    P = pkg.load_params_module()
+    ic = pkg.load_ic_module()
+    order = {"learned": P.PREC_LEARNED, "mixed": P.PREC_MIXED, "fp64": P.PREC_FP64}
+    if args.variants == "auto":
+        wanted = ["fp64", "mixed"]
+        if os.environ.get("LES_BENCH_LEARNED", "1") == "1" and getattr(pkg, "HAS_LEARNED", False):
+            wanted.append("learned")
+    else:
+        wanted = args.variants.split(",")
+    results = {}
+    for name in wanted:
+        try:
+            results[name] = run_variant(pkg, P, ic, order[name], args.steps, args.warmup, nx, ny, nz)
+        except Exception as e:  # a variant that fails is reported, not hidden
+            results[name] = {"error": str(e)}
+        print(f"# {name}: {results[name].get('mpts', results[name].get('error'))}", file=sys.stderr, flush=True)
+    ok = [n for n in ("learned", "mixed", "fp64") if n in results and "mpts" in results[n]]
+    if not ok:
+        print(json.dumps({"metric": "Mpt-steps/s at 256^3", "error": results}))
+        return 1
+    head = ok[0]
+    hr = results[head]
+    fam = hr.get("families", {})
+    dom = max(fam, key=lambda k: fam[k]["ms"]) if fam else None
+    roof = None
+    if dom:
+        d = fam[dom]
+        ach = d["bytes"] / d["calls"] / (d["avg_ms"] * 1e-3)
+        roof = {"bound": "hbm", "achieved": ach / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                "frac": ach / PEAK_HBM, "traffic": None, "kernel_family": dom,
+                "avg_launch_ms": d["avg_ms"], "bytes_per_launch": d["bytes"] / d["calls"]}
+    step_key = "fp64" if head == "fp64" else "mixed"
+    step_bytes = STEP_BYTES[step_key] * nx * ny * nz
+    out = {
+        "metric": "Mpt-steps/s at 256^3: FP32+learned-SGS vs plain FP64 dynamic SGS, matched error",
+        "value": hr["mpts"],
+        "unit": "Mpt-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": hr["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"fp64": "f64", "mixed": "f32+f64", "learned": "f32+f64+bf16"}[head],
+        "data": "synthetic: seeded log-law initial field (u* = 1, z0 = 1e-4, +-10% perturbation in z <= 0.5)",
+        "config": {"workload": f"LES time step, {head}, LASD cs_count=5", "grid": [nx, ny, nz], "variant": head,
+                   "parallelism": "single GPU, whole domain resident"},
+        "roofline": roof,
+        "step_roofline": {"bytes_per_point_step": STEP_BYTES[step_key],
+                          "achieved_GBs": step_bytes / (hr["ms_per_step"] * 1e-3) / 1e9,
+                          "frac": step_bytes / (hr["ms_per_step"] * 1e-3) / PEAK_HBM},
+        "variants": {k: {kk: vv for kk, vv in v.items() if kk != "families"} for k, v in results.items()},
+        "kernel_families": fam,
+    }
+    if "fp64" in results and "mpts" in results["fp64"]:
+        out["speedup_vs_fp64"] = hr["mpts"] / results["fp64"]["mpts"]
+    if not args.no_cpu:
+        try:
+            out["cpu_baseline"] = cpu_baseline(P, ic, nx, ny, nz, args.cpu_steps)
+        except Exception as e:
+            out["cpu_baseline"] = {"error": str(e)}
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

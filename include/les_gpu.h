@@ -94,10 +94,11 @@ int les_check_nan(int* found);
  * t[4] time_loop (src/LES_engine.f90:100-107). */
 int les_timers(double* t, int reset);
 /* Enable (1) or disable (0) per-kernel-family HIP-event timing; while on,
- * les_timers also returns t[5 + 2f] = summed ms and t[6 + 2f] = launches of
- * kernel family f (0 row r2c, 1 row c2r, 2 column, 3 row resample,
- * 4 column resample, 5 point-wise, 6 tridiagonal, 7 Lagrangian, 8 MLP).
- * Timing synchronises after every timed launch. */
+ * les_timers also returns, for kernel family f (0 row r2c, 1 row c2r,
+ * 2 column, 3 row resample, 4 column resample, 5 point-wise, 6 tridiagonal,
+ * 7 Lagrangian, 8 MLP): t[5 + 3f] summed ms, t[6 + 3f] launches and
+ * t[7 + 3f] algorithmic bytes (inputs read once + outputs written once).
+ * t must hold 32 doubles.  Timing synchronises after every timed launch. */
 int les_set_timing(int on);
 
 /* --- operator-level entry points (the compute_* module procedures) -------- */

@@ -174,7 +174,7 @@ class LesGpu:
         check(self.lib.les_set_timing(int(on)))
 
     def timers(self, reset: bool = False) -> np.ndarray:
-        t = np.zeros(5 + 2 * 9)
+        t = np.zeros(5 + 3 * 9)
         check(self.lib.les_timers(_dp(t), int(reset)))
         return t
 

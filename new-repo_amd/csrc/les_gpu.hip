@@ -83,7 +83,10 @@ static bool build_plan(int n, HostPlan& hp, int pool_off, int* used) {
 static void* dalloc(Les& L, size_t bytes) {
   void* p = nullptr;
   if (bytes == 0) bytes = 8;
-  if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess) {
+    L.alloc_failed = 1;
+    return nullptr;
+  }
   hipMemset(p, 0, bytes);
   L.allocs.push_back(p);
   return p;
@@ -103,8 +106,9 @@ static bool field_is_fp64(int id) {
 static void kt_begin(Les& L, int fam) {
   if (L.timing) hipEventRecord(L.ev_a[fam], L.stream);
 }
-static void kt_end(Les& L, int fam) {
+static void kt_end(Les& L, int fam, double bytes = 0.0) {
   if (L.timing) {
+    L.kt_bytes[fam] += bytes;
     hipEventRecord(L.ev_b[fam], L.stream);
     hipEventSynchronize(L.ev_b[fam]);
     float ms = 0.f;
@@ -134,7 +138,7 @@ static void rows_r2c(Les& L, const TI* const* src, Cplx<T>* const* dst, int nf, 
   size_t sh = 2 * (size_t)a.lpb * L.nx * sizeof(Cplx<T>);
   kt_begin(L, KT_ROWS_R2C);
   hipLaunchKernelGGL((k_rows_r2c<TI, T>), grid, dim3(LES_BLOCK), sh, L.stream, a, L.px, twpool<T>(L));
-  kt_end(L, KT_ROWS_R2C);
+  kt_end(L, KT_ROWS_R2C, (double)nf * npl * a.ny * ((double)L.nx * sizeof(TI) + (double)L.lhx * sizeof(Cplx<T>)));
 }
 
 template <typename T, typename TO>
@@ -148,7 +152,7 @@ static void rows_c2r(Les& L, const Cplx<T>* const* src, TO* const* dst, int nf, 
   size_t sh = 2 * (size_t)a.lpb * L.nx * sizeof(Cplx<T>);
   kt_begin(L, KT_ROWS_C2R);
   hipLaunchKernelGGL((k_rows_c2r<T, TO>), grid, dim3(LES_BLOCK), sh, L.stream, a, L.px, twpool<T>(L));
-  kt_end(L, KT_ROWS_C2R);
+  kt_end(L, KT_ROWS_C2R, (double)nf * npl * L.ny * ((double)L.lhx * sizeof(Cplx<T>) + (double)L.nx * sizeof(TO)));
 }
 
 template <typename T>
@@ -174,7 +178,7 @@ static void cols(Les& L, int op, const Cplx<T>* in0, const Cplx<T>* in1, Cplx<T>
   size_t sh = 4 * (size_t)a.cb * L.ny * sizeof(Cplx<T>);
   kt_begin(L, KT_COLS);
   hipLaunchKernelGGL((k_cols<T>), dim3((unsigned)(a.ncb * npl)), dim3(LES_BLOCK), sh, L.stream, a, L.py, twpool<T>(L));
-  kt_end(L, KT_COLS);
+  kt_end(L, KT_COLS, (double)npl * L.ns * sizeof(Cplx<T>) * (a.nin + nout));
 }
 
 // ------------------------------------------------------------------ field access
@@ -230,7 +234,7 @@ static void compute_vel_grad(Les& L) {
   a.ubc_u = L.P.ubc_u; a.ubc_v = L.P.ubc_v; a.ubc_w = L.P.ubc_w;
   kt_begin(L, KT_POINTWISE);
   hipLaunchKernelGGL((k_bc_ddz<T>), dim3(nblk(L.np * (L.nz + 1))), dim3(LES_BLOCK), 0, L.stream, a);
-  kt_end(L, KT_POINTWISE);
+  kt_end(L, KT_POINTWISE, (double)L.np * (L.nz + 1) * (3 * 8.0 + 3.0 * sizeof(T)));
 }
 
 // 2-delta filtered first level (get_layer_filter, src/utility_tools.f90:261-307)
@@ -266,7 +270,7 @@ static void wall_law(Les& L) {
   a.np = L.np;
   kt_begin(L, KT_POINTWISE);
   hipLaunchKernelGGL((k_wall<T>), dim3(nblk(L.np)), dim3(LES_BLOCK), 0, L.stream, a);
-  kt_end(L, KT_POINTWISE);
+  kt_end(L, KT_POINTWISE, (double)L.np * (2.0 * sizeof(T) + 8 * 4.0 + 4.0 * sizeof(T)));
 }
 
 template <typename T>
@@ -312,7 +316,7 @@ static void sgs_stress(Les& L, int jtt) {
     a.write_sij = 1;
     kt_begin(L, KT_POINTWISE);
     hipLaunchKernelGGL((k_sij<T>), dim3(nblk(L.np * (L.nz + 1))), dim3(LES_BLOCK), 0, L.stream, a);
-    kt_end(L, KT_POINTWISE);
+    kt_end(L, KT_POINTWISE, (double)L.np * (L.nz + 1) * (16.0 * sizeof(T) + (a.accumulate ? 3 * 8.0 + 6.0 * sizeof(T) : 0.0)));
     lasd_update<T>(L, jtt);
     a.accumulate = 0;
   } else {
@@ -321,7 +325,7 @@ static void sgs_stress(Les& L, int jtt) {
   a.write_sij = 0;
   kt_begin(L, KT_POINTWISE);
   hipLaunchKernelGGL((k_sgs<T>), dim3(nblk(L.np * (L.nz + 1))), dim3(LES_BLOCK), 0, L.stream, a);
-  kt_end(L, KT_POINTWISE);
+  kt_end(L, KT_POINTWISE, (double)L.np * (L.nz + 1) * (17.0 * sizeof(T) + (a.accumulate ? 3 * 8.0 + 6.0 * sizeof(T) : 0.0)));
 }
 
 #include "les_lasd.h"
@@ -392,7 +396,7 @@ static void nyquist_remove_fp64(Les& L, int fid) {
   hipLaunchKernelGGL(k_nyq_colsum, dim3(nblk((long)L.nx * L.nzg)), dim3(LES_BLOCK), 0, L.stream, f, B, L.nx, L.ny, L.nzg);
   hipLaunchKernelGGL(k_nyq_csum, dim3(nblk(L.nzg)), dim3(LES_BLOCK), 0, L.stream, A, C, L.ny, L.nzg);
   hipLaunchKernelGGL(k_nyq_apply, dim3(nblk(L.nf)), dim3(LES_BLOCK), 0, L.stream, f, A, B, C, L.nx, L.ny, L.nf);
-  kt_end(L, KT_POINTWISE);
+  kt_end(L, KT_POINTWISE, 24.0 * L.nf);
 }
 
 // ------------------------------------------------------------------ step
@@ -418,7 +422,7 @@ static int les_step_t(Les& L) {
     a.dt = L.P.dt; a.t1 = tadv1; a.t2 = tadv2; a.np = L.np; a.nz = L.nz;
     kt_begin(L, KT_POINTWISE);
     hipLaunchKernelGGL(k_ab2, dim3(nblk(L.np * (L.nz + 2))), dim3(LES_BLOCK), 0, L.stream, a);
-    kt_end(L, KT_POINTWISE);
+    kt_end(L, KT_POINTWISE, 12.0 * 8 * L.np * L.nz);
   }
   press_stag_array<T>(L, tadv1);
   projection(L, tadv1);
@@ -591,7 +595,7 @@ int les_init(const les_params* p) {
   }
   // w(:,:,0) = BOGUS (src/system.f90:196-198) is set by les_set_ic.
   // ---- workspaces
-  for (int i = 0; i < 8; ++i) L->wk_spec[i] = dalloc(*L, 2 * tb * L->nsp);
+  for (int i = 0; i < LES_NSPEC; ++i) L->wk_spec[i] = dalloc(*L, 2 * tb * L->nsp);
   for (int i = 0; i < 64; ++i) L->wk_real[i] = dalloc(*L, tb * L->nf);
   for (int i = 0; i < 6; ++i) L->wk_hyb[i] = dalloc(*L, tb * (size_t)L->nx2 * L->ny * L->nzg);
   for (int i = 0; i < 6; ++i) L->wk_big[i] = dalloc(*L, tb * (size_t)L->nbp * L->nzg);
@@ -603,8 +607,7 @@ int les_init(const les_params* p) {
   for (int i = 0; i < 4; ++i) L->Fold[i] = (double*)dalloc(*L, 8 * L->nf);
   L->nyq_rows = (double*)dalloc(*L, 8 * (size_t)(L->nx + L->ny + 2) * L->nzg);
   L->nanflag = (int*)dalloc(*L, 64);
-  for (void* q : L->allocs)
-    if (!q) { les_finalize(); les_set_error("out of memory (workspaces)"); return 3; }
+  if (L->alloc_failed) { les_finalize(); les_set_error("out of memory (workspaces)"); return 3; }
   for (int i = 0; i < 64; ++i) if (!L->wk_real[i]) { les_finalize(); les_set_error("out of memory"); return 3; }
   setup_kernel_attrs();
   if (hipDeviceSynchronize() != hipSuccess) { les_set_error("init sync failed"); les_finalize(); return 5; }
@@ -758,13 +761,14 @@ int les_timers(double* t, int reset) {
   if (t) {
     for (int i = 0; i < 5; ++i) t[i] = L->phase_ms[i];
     for (int i = 0; i < KT_COUNT; ++i) {
-      t[5 + 2 * i] = L->kt_ms[i];
-      t[6 + 2 * i] = (double)L->kt_calls[i];
+      t[5 + 3 * i] = L->kt_ms[i];
+      t[6 + 3 * i] = (double)L->kt_calls[i];
+      t[7 + 3 * i] = L->kt_bytes[i];
     }
   }
   if (reset) {
     for (int i = 0; i < 5; ++i) L->phase_ms[i] = 0;
-    for (int i = 0; i < KT_COUNT; ++i) { L->kt_ms[i] = 0; L->kt_calls[i] = 0; }
+    for (int i = 0; i < KT_COUNT; ++i) { L->kt_ms[i] = 0; L->kt_calls[i] = 0; L->kt_bytes[i] = 0; }
   }
   return 0;
 }

@@ -155,13 +155,15 @@ static void lasd_update(Les& L, int jtt) {
   ia.np = L.np; ia.nz = L.nz;
   kt_begin(L, KT_POINTWISE);
   hipLaunchKernelGGL((k_lasd_inputs<T>), dim3(nblk(npt)), dim3(LES_BLOCK), 0, L.stream, ia);
-  kt_end(L, KT_POINTWISE);
+  kt_end(L, KT_POINTWISE, (double)npt * (3 * 8.0 + 22.0 * sizeof(T)));
   // test filters at 2 delta and 4 delta from one forward transform per field
   const T* g2 = sizeof(T) == 8 ? (const T*)L.g2_64 : (const T*)L.g2_32;
   const T* g4 = sizeof(T) == 8 ? (const T*)L.g4_64 : (const T*)L.g4_32;
   T* fl2[21];
   T* fl4[21];
   for (int f = 0; f < 21; ++f) { fl2[f] = (T*)L.wk_real[15 + f]; fl4[f] = (T*)L.wk_real[36 + f]; }
+  // three fields at a time: inputs in spectral slots 0..2, filtered outputs in 3..8
+  static_assert(LES_NSPEC >= 9, "Germano filtering needs 9 spectral workspaces");
   for (int f0 = 0; f0 < 21; f0 += 3) {
     const T* src[3];
     Cplx<T>* sp[3];
@@ -196,7 +198,7 @@ static void lasd_update(Les& L, int jtt) {
     ga.np = L.np; ga.nz = L.nz;
     kt_begin(L, KT_POINTWISE);
     hipLaunchKernelGGL((k_germano<T>), dim3(nblk(npt)), dim3(LES_BLOCK), 0, L.stream, ga);
-    kt_end(L, KT_POINTWISE);
+    kt_end(L, KT_POINTWISE, (double)npt * 23.0 * sizeof(T));
   }
   // Lagrangian interpolation, relaxation and coefficients; F_* double buffered
   for (int f = 0; f < 4; ++f) {
@@ -218,7 +220,7 @@ static void lasd_update(Les& L, int jtt) {
   la.halo_mode = L.P.halo_mode;
   kt_begin(L, KT_LAGR);
   hipLaunchKernelGGL((k_lagr<T>), dim3(nblk(npt)), dim3(LES_BLOCK), 0, L.stream, la);
-  kt_end(L, KT_LAGR);
+  kt_end(L, KT_LAGR, (double)npt * (3.0 * sizeof(T) + 4 * 8.0 + 4.0 * sizeof(T) + 4 * 8.0 + 2.0 * sizeof(T)));
   L.F_coeff_init = 1;
   const size_t tb = sizeof(T) * (size_t)L.nf;
   hipMemsetAsync(L.f[LES_U_LAG], 0, tb, L.stream);

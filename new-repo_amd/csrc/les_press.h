@@ -105,7 +105,7 @@ static void press_stag_array(Les& L, double tadv1) {
   t.dz = L.dz; t.idz2 = 1.0 / (L.dz * L.dz);
   kt_begin(L, KT_TRIDIAG);
   hipLaunchKernelGGL(k_tridiag, dim3(nblk(L.ns, 64)), dim3(64), 0, L.stream, t);
-  kt_end(L, KT_TRIDIAG);
+  kt_end(L, KT_TRIDIAG, 16.0 * L.ns * (3.0 * nz + 2.0 + (nz + 2.0)));
   // gradient: i kx p, i ky p and p back to physical space (src/compute_pressure.f90:320-357)
   cols<double>(L, COP_PGRAD, ph, nullptr, Hx, Hy, Hz, 3, 0, 1, 0, nz + 2);
   const Cplx<double>* cs[3] = {Hx, Hy, Hz};
@@ -169,5 +169,5 @@ static void projection(Les& L, double tadv1) {
   a.ubc_u = L.P.ubc_u; a.ubc_v = L.P.ubc_v; a.ubc_w = L.P.ubc_w; a.bogus = BOGUS;
   kt_begin(L, KT_POINTWISE);
   hipLaunchKernelGGL(k_proj, dim3(nblk(L.np * L.nz)), dim3(LES_BLOCK), 0, L.stream, a);
-  kt_end(L, KT_POINTWISE);
+  kt_end(L, KT_POINTWISE, 8.0 * L.np * L.nz * 16.0);
 }

@@ -75,7 +75,7 @@ static void convection(Les& L, double* cx, double* cy, double* cz) {
     kt_begin(L, KT_ROWS_RES);
     hipLaunchKernelGGL((k_rows_resample<T, T, T>), dim3((unsigned)((jobs + a.lpb - 1) / a.lpb), 6), dim3(LES_BLOCK), sh,
                        L.stream, a, L.px, L.px2, twpool<T>(L));
-    kt_end(L, KT_ROWS_RES);
+    kt_end(L, KT_ROWS_RES, (double)a.nplanes * L.np * (3.0 * 8 + 6.0 * sizeof(T)) + 6.0 * a.nplanes * nx2 * ny * sizeof(T));
   }
   // pad in y
   {
@@ -89,7 +89,7 @@ static void convection(Les& L, double* cx, double* cy, double* cz) {
     kt_begin(L, KT_COLS_RES);
     hipLaunchKernelGGL((k_cols_resample<T>), dim3((unsigned)(a.ncp * a.nplanes), 6), dim3(LES_BLOCK), sh, L.stream, a,
                        L.py, L.py2, twpool<T>(L));
-    kt_end(L, KT_COLS_RES);
+    kt_end(L, KT_COLS_RES, 6.0 * a.nplanes * nx2 * (double)(ny + ny2) * sizeof(T));
   }
   // products
   {
@@ -101,7 +101,7 @@ static void convection(Les& L, double* cx, double* cy, double* cz) {
     a.c2 = (T)(1.0 / (double)(nx2 * ny2));
     kt_begin(L, KT_POINTWISE);
     hipLaunchKernelGGL((k_conv_prod<T>), dim3(nblk(L.nbp * nz)), dim3(LES_BLOCK), 0, L.stream, a);
-    kt_end(L, KT_POINTWISE);
+    kt_end(L, KT_POINTWISE, 9.0 * L.nbp * nz * sizeof(T));
   }
   // truncate in y
   {
@@ -115,7 +115,7 @@ static void convection(Les& L, double* cx, double* cy, double* cz) {
     kt_begin(L, KT_COLS_RES);
     hipLaunchKernelGGL((k_cols_resample<T>), dim3((unsigned)(a.ncp * a.nplanes), 3), dim3(LES_BLOCK), sh, L.stream, a,
                        L.py2, L.py, twpool<T>(L));
-    kt_end(L, KT_COLS_RES);
+    kt_end(L, KT_COLS_RES, 3.0 * a.nplanes * nx2 * (double)(ny + ny2) * sizeof(T));
   }
   // truncate in x, straight into rhs
   {
@@ -132,7 +132,7 @@ static void convection(Les& L, double* cx, double* cy, double* cz) {
     kt_begin(L, KT_ROWS_RES);
     hipLaunchKernelGGL((k_rows_resample<T, T, double>), dim3((unsigned)((jobs + a.lpb - 1) / a.lpb), 3), dim3(LES_BLOCK),
                        sh, L.stream, a, L.px2, L.px, twpool<T>(L));
-    kt_end(L, KT_ROWS_RES);
+    kt_end(L, KT_ROWS_RES, 3.0 * nz * ny * ((double)nx2 * sizeof(T) + (double)nx * 8));
   }
   SetPlaneArgs sp;
   sp.a = cx; sp.b = cy; sp.c = cz; sp.np = L.np; sp.k0 = 0; sp.k1 = nz + 1; sp.val = BOGUS;
@@ -209,7 +209,7 @@ static void div_stress(Les& L, int rhs_id, int tx, int ty, int tz, int stag, int
   a.coriolis = L.P.coriolis;
   kt_begin(L, KT_POINTWISE);
   hipLaunchKernelGGL((k_div_epi<T>), dim3(nblk(L.np * L.nz)), dim3(LES_BLOCK), 0, L.stream, a);
-  kt_end(L, KT_POINTWISE);
+  kt_end(L, KT_POINTWISE, (double)L.np * L.nz * (16.0 + 2.0 * sizeof(T)));
 }
 
 template <typename T>

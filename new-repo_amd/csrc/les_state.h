@@ -6,6 +6,9 @@
 #include "../../include/les_gpu.h"
 #include "fft_kernels.h"
 
+// Number of spectral workspaces (the Germano filtering uses slots 0..8).
+#define LES_NSPEC 12
+
 // Kernel families timed with HIP events when timing is enabled.
 enum {
   KT_ROWS_R2C = 0, KT_ROWS_C2R, KT_COLS, KT_ROWS_RES, KT_COLS_RES, KT_POINTWISE, KT_TRIDIAG, KT_LAGR, KT_MLP, KT_COUNT
@@ -40,7 +43,7 @@ struct Les {
   double* z0;
 
   // workspaces
-  void* wk_spec[8];       // spectral (lhx, ny, nzg) of flow precision
+  void* wk_spec[LES_NSPEC]; // spectral (lhx, ny, nzg) of flow precision
   void* wk_real[64];      // real (nx, ny, nzg) of flow precision
   void* wk_hyb[6];        // (nx2, ny, nzg)
   void* wk_big[6];        // (nx2, ny2, nzg)
@@ -52,6 +55,7 @@ struct Les {
   double* Fold[4];        // previous F_* for the Lagrangian gather
   double* nyq_rows;       // mixed-mode FP64 Nyquist removal partial sums
   int* nanflag;
+  int alloc_failed;
 
   // learned SGS
   float* mlp_w;           // packed weights (device)
@@ -63,6 +67,7 @@ struct Les {
   int timing;
   hipEvent_t ev_a[KT_COUNT], ev_b[KT_COUNT];
   double kt_ms[KT_COUNT];
+  double kt_bytes[KT_COUNT];
   long kt_calls[KT_COUNT];
   hipEvent_t ph_ev[6];
   double phase_ms[5];

@@ -1142,6 +1142,8 @@ int orc_op(int which, double arg) {
     case 4: pressure(o, arg); break;
     case 5: compute_sij(o); break;
     case 6: convection(o, o.rhsx, o.rhsy, o.rhsz); break;
+    case 7: lagrangian_interp(o); break;
+    case 8: set_bc_all(o); break;
     default: return 2;
   }
   return 0;

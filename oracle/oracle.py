@@ -159,7 +159,8 @@ class Oracle:
         return x[1:self.prm.nz + 3]
 
     def op(self, which, arg=0.0):
-        code = {"wall_law": 0, "vel_grad": 1, "sgs": 2, "rhs": 3, "pressure": 4, "sij": 5, "conv": 6}[which]
+        code = {"wall_law": 0, "vel_grad": 1, "sgs": 2, "rhs": 3, "pressure": 4, "sij": 5, "conv": 6,
+                "lagrangian": 7, "set_bc": 8}[which]
         lib().orc_op(code, float(arg))
 
     def set_jt_total(self, jt):

@@ -63,8 +63,8 @@ def oracle():
 
 
 def rel_l2(a, b):
-    a = np.asarray(a, dtype=np.float64)
-    b = np.asarray(b, dtype=np.float64)
+    a = np.asarray(a)
+    b = np.asarray(b)
     den = np.linalg.norm(b.ravel())
     num = np.linalg.norm((a - b).ravel())
     return num / den if den > 0 else num

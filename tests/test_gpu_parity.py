@@ -88,3 +88,109 @@ def test_step_matches_oracle(grid):
             print(f" after {o.jt_total()} steps", flush=True)
             _compare(g, o, nz)
         assert not g.check_nan()
+
+
+def test_operators_match_oracle_individually():
+    pkg, O = load_pkg(), load_oracle()
+    prm, u, v, w = _setup(32, 32, 16, seed=11)
+    o = O.Oracle(prm)
+    o.set_ic(u, v, w)
+    o.step(6)  # a developed state with a non-zero Cs^2
+    with pkg.LesGpu(prm) as g:
+        for n in ("u", "v", "w", "cs_opt2", "F_LM", "F_MM", "F_QN", "F_NN", "rhsx", "rhsy", "rhsz", "txz", "tyz"):
+            g.upload(n, o.get(n))
+        g.set_jt_total(6)
+        for lay in ("lbc_u", "lbc_v", "bottom_wk", "top_wk"):
+            g.set_layer(lay, o.get2d(lay))
+        print("wall law", flush=True)
+        o.op("wall_law")
+        g.wall_law()
+        for lay in ("ustar", "lbc_u", "lbc_v"):
+            assert np.array_equal(g.layer(lay), o.get2d(lay)), lay
+        print("velocity gradients", flush=True)
+        o.op("vel_grad")
+        g.compute_vel_grad()
+        _compare(g, o, prm.nz, ["u", "v", "w", "dudx", "dudy", "dudz", "dvdx", "dvdy", "dvdz", "dwdx", "dwdy", "dwdz"],
+                 tol=0.0)
+        print("sgs stress (no update at jtt=7)", flush=True)
+        o.op("sgs", 7)
+        g.sgs_stress(7, 7)
+        _compare(g, o, prm.nz, ["txx", "txy", "txz", "tyy", "tyz", "tzz", "nu_t"], tol=0.0)
+        print("pressure", flush=True)
+        o.op("pressure", 1.5)
+        g.press_stag_array(1.5)
+        _compare(g, o, prm.nz, ["p", "dpdx", "dpdy"], tol=0.0)
+
+
+def test_mixed_precision_error_vs_fp64_oracle():
+    pkg, O = load_pkg(), load_oracle()
+    P = load_params()
+    prm, u, v, w = _setup(32, 32, 16, seed=5)
+    o = O.Oracle(prm)
+    o.set_ic(u, v, w)
+    prm.precision = P.PREC_MIXED
+    with pkg.LesGpu(prm) as g:
+        g.set_ic(u, v, w)
+        o.step(12)
+        g.step(12)
+        errs = {}
+        for n in ("u", "v", "w", "p", "cs_opt2", "beta", "F_LM", "F_MM"):
+            errs[n] = rel_l2(_interior(n, g.download(n), prm.nz), _interior(n, o.get(n), prm.nz))
+        print("mixed vs FP64 oracle after 12 steps:", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+        assert errs["u"] < 1e-6 and errs["v"] < 1e-4 and errs["w"] < 1e-4
+        assert errs["p"] < 1e-3 and errs["cs_opt2"] < 1e-2
+
+
+def test_nan_is_reported():
+    pkg = load_pkg()
+    prm, u, v, w = _setup(16, 16, 8)
+    with pkg.LesGpu(prm) as g:
+        g.set_ic(u, v, w)
+        assert not g.check_nan()
+        a = g.download("v")
+        a[3, 5, 7] = np.nan
+        g.upload("v", a)
+        assert g.check_nan()
+
+
+def test_init_rejects_bad_options():
+    pkg = load_pkg()
+    P = load_params()
+    bad = P.LesParams(nx=20, ny=16, nz=8)
+    with pytest.raises(pkg.LesError) as e:
+        pkg.LesGpu(bad)
+    assert e.value.code == 10
+    bad = P.LesParams(nx=16, ny=16, nz=8, sgs_model=1)
+    with pytest.raises(pkg.LesError) as e:
+        pkg.LesGpu(bad)
+    assert e.value.code == 1
+
+
+def test_full_size_invariants():
+    """256x256x128 (the FP64 drop-in grid): size-independent properties after
+    a LASD cycle -- finite fields, discrete divergence-free velocity,
+    Nyquist-free spectral derivatives, float32-valued Cs^2 >= 1e-24 and
+    beta >= 1/8 with beta = 1 on the top level."""
+    pkg = load_pkg()
+    prm, u, v, w = _setup(256, 256, 128, amp=0.05, seed=21)
+    nz, dz = prm.nz, prm.lz / prm.nz
+    with pkg.LesGpu(prm) as g:
+        g.set_ic(u, v, w)
+        print("full size: 10 steps", flush=True)
+        g.step(10)
+        assert not g.check_nan()
+        U, V, W = (np.fft.rfft2(g.download(n)) for n in ("u", "v", "w"))
+        lhx = prm.nx // 2 + 1
+        kx = np.arange(lhx, dtype=float)
+        kx[-1] = 0
+        ky = ((np.arange(prm.ny) + prm.ny // 2) % prm.ny - prm.ny // 2).astype(float)
+        ky[prm.ny // 2] = 0
+        d = 1j * kx[None, None, :] * U[2:nz] + 1j * ky[None, :, None] * V[2:nz] + (W[3:nz + 1] - W[2:nz]) / dz
+        d[:, :, -1] = 0
+        d[:, prm.ny // 2, :] = 0
+        assert np.abs(d).max() < 1e-10 * np.abs(1j * kx[None, None, :] * U[2:nz]).max()
+        D = np.fft.rfft2(g.download("dudx")[1:nz + 1])
+        assert np.abs(D[:, :, -1]).max() < 1e-12 * np.abs(D).max()
+        cs, beta = g.download("cs_opt2")[1:nz + 2], g.download("beta")[1:nz + 2]
+        assert np.array_equal(cs.astype(np.float32).astype(np.float64), cs) and cs.min() >= np.float32(1e-24)
+        assert beta.min() >= 0.125 and np.all(beta[-1] == 1.0)
