@@ -125,6 +125,13 @@ int les_compute_rhs_adv(void);
 int les_press_stag_array(double tadv1);
 /* test_filter(f, G) on levels 1..nz+1, alpha = 2 or 4 (src/testfilter.f90:144-184). */
 int les_test_filter(int f, int alpha);
+/* Advection step u += dt (tadv1 rhs + tadv2 rhs_f) on levels 1..nz and
+ * w = 0 on the ghost levels (src/LES_engine.f90:253-283). */
+int les_ab2(double tadv1, double tadv2);
+/* Projection u -= dt tadv1 grad p, rhs -= grad p, dpdz, w(1) = w(nz+1) = 0
+ * and the bottom/top set_bc (src/LES_engine.f90:299-352;
+ * src/compute_pressure.f90:353-369). */
+int les_projection(double tadv1);
 
 /* execute_decomp_2d_fft_2d (src/decomp_2d_fft_2d.f90:498-577) on device
  * buffers of the library's compute precision: r2c scales the input by

@@ -71,6 +71,7 @@ def library() -> ctypes.CDLL:
             "les_compute_dealias_convec": [ctypes.c_int] * 3, "les_div_stress": [ctypes.c_int] * 6,
             "les_compute_rhs_adv": [], "les_press_stag_array": [ctypes.c_double],
             "les_test_filter": [ctypes.c_int] * 2,
+            "les_ab2": [ctypes.c_double, ctypes.c_double], "les_projection": [ctypes.c_double],
             "les_fft2d_r2c": [vp, vp, ctypes.c_int, ctypes.c_double], "les_fft2d_c2r": [vp, vp, ctypes.c_int],
             "les_sgs_samples": [vp, vp, ctypes.c_long, lp, ctypes.c_int],
             "les_mlp_set_weights": [ctypes.POINTER(ctypes.c_float), ctypes.c_long],
@@ -208,6 +209,12 @@ class LesGpu:
 
     def press_stag_array(self, tadv1: float) -> None:
         check(self.lib.les_press_stag_array(float(tadv1)))
+
+    def ab2(self, tadv1: float, tadv2: float) -> None:
+        check(self.lib.les_ab2(float(tadv1), float(tadv2)))
+
+    def projection(self, tadv1: float) -> None:
+        check(self.lib.les_projection(float(tadv1)))
 
     def test_filter(self, f: str, alpha: int) -> None:
         check(self.lib.les_test_filter(FID[f], int(alpha)))

@@ -399,6 +399,18 @@ static void nyquist_remove_fp64(Les& L, int fid) {
   kt_end(L, KT_POINTWISE, 24.0 * L.nf);
 }
 
+// advection step (src/LES_engine.f90:253-283)
+static void ab2(Les& L, double tadv1, double tadv2) {
+  Ab2Args a;
+  a.u = D(L, LES_U); a.v = D(L, LES_V); a.w = D(L, LES_W);
+  a.rx = D(L, LES_RHSX); a.ry = D(L, LES_RHSY); a.rz = D(L, LES_RHSZ);
+  a.fx = D(L, LES_RHSX_F); a.fy = D(L, LES_RHSY_F); a.fz = D(L, LES_RHSZ_F);
+  a.dt = L.P.dt; a.t1 = tadv1; a.t2 = tadv2; a.np = L.np; a.nz = L.nz;
+  kt_begin(L, KT_POINTWISE);
+  hipLaunchKernelGGL(k_ab2, dim3(nblk(L.np * (L.nz + 2))), dim3(LES_BLOCK), 0, L.stream, a);
+  kt_end(L, KT_POINTWISE, 12.0 * 8 * L.np * L.nz);
+}
+
 // ------------------------------------------------------------------ step
 template <typename T>
 static int les_step_t(Les& L) {
@@ -414,16 +426,7 @@ static int les_step_t(Les& L) {
   if (L.timing) hipEventRecord(L.ph_ev[3], L.stream);
   double tadv1 = (jtt == 1) ? 1.0 : 1.5;
   double tadv2 = (jtt == 1) ? 0.0 : 1.0 - tadv1;
-  {
-    Ab2Args a;
-    a.u = D(L, LES_U); a.v = D(L, LES_V); a.w = D(L, LES_W);
-    a.rx = D(L, LES_RHSX); a.ry = D(L, LES_RHSY); a.rz = D(L, LES_RHSZ);
-    a.fx = D(L, LES_RHSX_F); a.fy = D(L, LES_RHSY_F); a.fz = D(L, LES_RHSZ_F);
-    a.dt = L.P.dt; a.t1 = tadv1; a.t2 = tadv2; a.np = L.np; a.nz = L.nz;
-    kt_begin(L, KT_POINTWISE);
-    hipLaunchKernelGGL(k_ab2, dim3(nblk(L.np * (L.nz + 2))), dim3(LES_BLOCK), 0, L.stream, a);
-    kt_end(L, KT_POINTWISE, 12.0 * 8 * L.np * L.nz);
-  }
+  ab2(L, tadv1, tadv2);
   press_stag_array<T>(L, tadv1);
   projection(L, tadv1);
   if (L.timing) {
@@ -935,6 +938,20 @@ int les_div_stress(int rhs, int tx, int ty, int tz, int stag, int save_wk) {
     if (id < 0 || id >= LES_NFIELDS || field_is_fp64(id)) { les_set_error("div_stress: stress fields expected"); return 10; }
   if (L->mixed) div_stress<float>(*L, rhs, tx, ty, tz, stag, save_wk, 0);
   else div_stress<double>(*L, rhs, tx, ty, tz, stag, save_wk, 0);
+  LES_CHECK(hipGetLastError());
+  return 0;
+}
+
+int les_ab2(double tadv1, double tadv2) {
+  if (!g_les) return 1;
+  ab2(*g_les, tadv1, tadv2);
+  LES_CHECK(hipGetLastError());
+  return 0;
+}
+
+int les_projection(double tadv1) {
+  if (!g_les) return 1;
+  projection(*g_les, tadv1);
   LES_CHECK(hipGetLastError());
   return 0;
 }
