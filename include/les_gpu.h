@@ -140,15 +140,38 @@ int les_fft2d_r2c(const void* d_in, void* d_out, int nplanes, double scale);
 int les_fft2d_c2r(const void* d_in, void* d_out, int nplanes);
 
 /* --- learned SGS surrogate (no counterpart in the reference) -------------- */
-/* Generate training samples (features + LASD targets) at the current state;
- * returns the number of samples written (<= max_samples) into d_x (12 floats
- * per sample) and d_y (2 floats per sample, device pointers). */
+/* A per-point MLP 12 -> 64 -> 64 -> 64 -> 2 (GELU, BF16 MFMA, FP32
+ * accumulation) predicting the Lagrangian dynamic model's Cs^2 and beta from
+ * strain/rotation invariants at delta, 2 delta, 4 delta and log(z/delta).
+ * With precision = 2 the update steps of the dynamic procedure use it; every
+ * judge_every steps the plain procedure runs instead and judges it.
+ *
+ * Training samples at the current state (call right after a step with a
+ * plain dynamic update, mixed-precision modes): every stride-th point of
+ * levels 1..nz+1, at most max_samples, raw features to device d_x
+ * (12 float32 per sample) and targets (Cs^2, beta) to device d_y. */
 int les_sgs_samples(void* d_x, void* d_y, long max_samples, long* n_written, int stride);
-/* Load MLP weights (float32 host arrays, see les_mlp_layout) and normalisers. */
-int les_mlp_set_weights(const float* w, long nweights);
+/* Number of float32 values of the compact weight layout: W1[64][12] b1[64]
+ * W2[64][64] b2[64] W3[64][64] b3[64] W4[2][64] b4[2], then the feature
+ * means[12] and scales[12], target means[2] and scales[2], and the
+ * out-of-range limit (in scale units). */
 int les_mlp_layout(long* nweights);
-/* Judge statistics of the last learned step: [0] rel. L2 of Cs2 vs LASD,
- * [1] rel. L2 of beta, [2] fraction of points out of training range. */
+int les_mlp_set_weights(const float* w, long nweights);
+int les_mlp_get_weights(float* w);
+/* Raw samples -> normalised training buffers (device): d_x bf16 [n][16],
+ * d_y float32 [n][16] (targets in columns 0, 1). */
+int les_mlp_prepare(const void* d_raw, const void* d_tgt, long n, void* d_x, void* d_y);
+/* Gradient of 0.5 * mean |y - t|^2 over a batch (n multiple of 16) in the
+ * compact layout without the normalisers (MLP weights and biases only). */
+int les_mlp_grad(const void* d_x, const void* d_y, long n, float* grad, double* loss);
+/* One Adam step (beta1 0.9, beta2 0.999) on a batch; FP32 master weights,
+ * BF16 copies for the MFMA kernels. */
+int les_mlp_train_step(const void* d_x, const void* d_y, long n, double lr, double* loss);
+/* Fused inference on normalised features (device bf16 [n][16]) ->
+ * normalised outputs (device float32 [n][2]). */
+int les_mlp_forward(const void* d_x, long n, void* d_out);
+/* Judge statistics of the last judge step: [0] rel. L2 of Cs^2 (surrogate vs
+ * plain procedure), [1] rel. L2 of beta, [3] the step. */
 int les_mlp_judge_stats(double* out);
 
 #ifdef __cplusplus

@@ -76,6 +76,11 @@ def library() -> ctypes.CDLL:
             "les_sgs_samples": [vp, vp, ctypes.c_long, lp, ctypes.c_int],
             "les_mlp_set_weights": [ctypes.POINTER(ctypes.c_float), ctypes.c_long],
             "les_mlp_layout": [lp], "les_mlp_judge_stats": [dp],
+            "les_mlp_get_weights": [ctypes.POINTER(ctypes.c_float)],
+            "les_mlp_prepare": [vp, vp, ctypes.c_long, vp, vp],
+            "les_mlp_grad": [vp, vp, ctypes.c_long, ctypes.POINTER(ctypes.c_float), dp],
+            "les_mlp_train_step": [vp, vp, ctypes.c_long, ctypes.c_double, dp],
+            "les_mlp_forward": [vp, ctypes.c_long, vp],
         }
         for name, args in sig.items():
             f = getattr(lib, name)
@@ -218,6 +223,50 @@ class LesGpu:
 
     def test_filter(self, f: str, alpha: int) -> None:
         check(self.lib.les_test_filter(FID[f], int(alpha)))
+
+    # ---- learned SGS surrogate
+    def mlp_size(self) -> int:
+        n = ctypes.c_long()
+        check(self.lib.les_mlp_layout(ctypes.byref(n)))
+        return n.value
+
+    def mlp_set_weights(self, w) -> None:
+        a = np.ascontiguousarray(w, dtype=np.float32)
+        check(self.lib.les_mlp_set_weights(a.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), a.size))
+
+    def mlp_get_weights(self) -> np.ndarray:
+        out = np.zeros(self.mlp_size(), dtype=np.float32)
+        check(self.lib.les_mlp_get_weights(out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return out
+
+    def mlp_prepare(self, d_raw: int, d_tgt: int, n: int, d_x: int, d_y: int) -> None:
+        check(self.lib.les_mlp_prepare(ctypes.c_void_p(d_raw), ctypes.c_void_p(d_tgt), n, ctypes.c_void_p(d_x),
+                                       ctypes.c_void_p(d_y)))
+
+    def mlp_grad(self, d_x: int, d_y: int, n: int):
+        g = np.zeros(self.mlp_size() - 29, dtype=np.float32)
+        loss = ctypes.c_double()
+        check(self.lib.les_mlp_grad(ctypes.c_void_p(d_x), ctypes.c_void_p(d_y), n,
+                                    g.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), ctypes.byref(loss)))
+        return g, loss.value
+
+    def mlp_train_step(self, d_x: int, d_y: int, n: int, lr: float) -> float:
+        loss = ctypes.c_double()
+        check(self.lib.les_mlp_train_step(ctypes.c_void_p(d_x), ctypes.c_void_p(d_y), n, float(lr), ctypes.byref(loss)))
+        return loss.value
+
+    def mlp_forward(self, d_x: int, n: int, d_out: int) -> None:
+        check(self.lib.les_mlp_forward(ctypes.c_void_p(d_x), n, ctypes.c_void_p(d_out)))
+
+    def sgs_samples(self, d_x: int, d_y: int, max_samples: int, stride: int) -> int:
+        n = ctypes.c_long()
+        check(self.lib.les_sgs_samples(ctypes.c_void_p(d_x), ctypes.c_void_p(d_y), max_samples, ctypes.byref(n), stride))
+        return n.value
+
+    def judge_stats(self) -> np.ndarray:
+        out = np.zeros(4)
+        check(self.lib.les_mlp_judge_stats(_dp(out)))
+        return out
 
     def fft2d_r2c(self, d_in: int, d_out: int, nplanes: int, scale: float = 1.0) -> None:
         check(self.lib.les_fft2d_r2c(ctypes.c_void_p(d_in), ctypes.c_void_p(d_out), nplanes, float(scale)))

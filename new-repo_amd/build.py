@@ -27,7 +27,7 @@ HIP_FLAGS = [
     "-Wno-unused-value", "-Wno-unused-result",
 ]
 
-SOURCES = ["les_gpu.hip", "les_mlp.hip"]
+SOURCES = ["les_gpu.hip"]
 
 
 def _run(cmd, cwd=None):

@@ -295,6 +295,8 @@ static SgsArgs<T> sgs_args(Les& L) {
 
 template <typename T>
 static void lasd_update(Les& L, int jtt);
+template <typename T>
+static void learned_update(Les& L, int jtt);
 
 template <typename T>
 static void sgs_stress(Les& L, int jtt) {
@@ -317,7 +319,12 @@ static void sgs_stress(Les& L, int jtt) {
     kt_begin(L, KT_POINTWISE);
     hipLaunchKernelGGL((k_sij<T>), dim3(nblk(L.np * (L.nz + 1))), dim3(LES_BLOCK), 0, L.stream, a);
     kt_end(L, KT_POINTWISE, (double)L.np * (L.nz + 1) * (16.0 * sizeof(T) + (a.accumulate ? 3 * 8.0 + 6.0 * sizeof(T) : 0.0)));
-    lasd_update<T>(L, jtt);
+    if constexpr (sizeof(T) == 4) {
+      if (L.P.precision == 2) learned_update<T>(L, jtt);
+      else lasd_update<T>(L, jtt);
+    } else {
+      lasd_update<T>(L, jtt);
+    }
     a.accumulate = 0;
   } else {
     a.accumulate = acc ? 1 : 0;
@@ -331,6 +338,8 @@ static void sgs_stress(Les& L, int jtt) {
 #include "les_lasd.h"
 #include "les_rhs.h"
 #include "les_press.h"
+#include "les_mlp.h"
+#include "les_learn.h"
 
 // ---------------------------------------------------------------- mixed-mode Nyquist filter
 // In the mixed-precision mode the velocity stays FP64, so the Nyquist-free
@@ -464,6 +473,8 @@ static int setup_kernel_attrs() {
 
 static void free_all(Les* L) {
   if (!L) return;
+  delete L->mlp;
+  L->mlp = nullptr;
   for (void* p : L->allocs) hipFree(p);
   L->allocs.clear();
   for (int i = 0; i < KT_COUNT; ++i) { hipEventDestroy(L->ev_a[i]); hipEventDestroy(L->ev_b[i]); }

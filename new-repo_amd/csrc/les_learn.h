@@ -1,0 +1,410 @@
+// Learned SGS driver: feature pipeline, inference inside the time step with
+// the plain dynamic procedure as periodic judge, sample generation and the
+// training step.  Included by les_gpu.hip after les_mlp.h.
+#pragma once
+
+struct MlpState {
+  float* w;        // fp32 master parameters [MLP_NPAR]
+  short* wb;       // bf16 matrices [MLP_NMAT]
+  float *m, *v, *g;
+  float* norm;     // xmean[12] xstd[12]
+  float hn[29];    // host: xmean[12] xstd[12] ymean[2] ystd[2] xlim
+  short* X;        // step features [npts][16]
+  float* red;      // reduction partials
+  long redn;
+  int step;
+  int ready;
+  // training buffers
+  long cap;
+  short *h1, *h2, *h3, *dy, *dz;
+  float *z1, *z2, *z3, *c, *dh, *dyf, *dzf, *part, *lossp;
+};
+
+static void* dalloc(Les& L, size_t bytes);
+
+static int mlp_alloc(Les& L) {
+  if (L.mlp) return 0;
+  MlpState* M = new MlpState();
+  std::memset((void*)M, 0, sizeof(MlpState));
+  M->w = (float*)dalloc(L, 4 * MLP_NPAR);
+  M->wb = (short*)dalloc(L, 2 * MLP_NMAT);
+  M->m = (float*)dalloc(L, 4 * MLP_NPAR);
+  M->v = (float*)dalloc(L, 4 * MLP_NPAR);
+  M->g = (float*)dalloc(L, 4 * MLP_NPAR);
+  M->norm = (float*)dalloc(L, 4 * 32);
+  const long npt = L.np * (L.nz + 1);
+  M->X = (short*)dalloc(L, 2 * 16 * (size_t)npt);
+  M->redn = 2 * (long)nblk(npt) + 64;
+  M->red = (float*)dalloc(L, 4 * (size_t)M->redn);
+  L.mlp = M;
+  if (L.alloc_failed) { les_set_error("out of memory (MLP)"); return 3; }
+  return 0;
+}
+
+static int mlp_train_alloc(Les& L, long n) {
+  MlpState* M = L.mlp;
+  if (n <= M->cap) return 0;
+  const long c = n;
+  M->h1 = (short*)dalloc(L, 2 * 64 * c); M->h2 = (short*)dalloc(L, 2 * 64 * c); M->h3 = (short*)dalloc(L, 2 * 64 * c);
+  M->dy = (short*)dalloc(L, 2 * 16 * c); M->dz = (short*)dalloc(L, 2 * 64 * c);
+  M->z1 = (float*)dalloc(L, 4 * 64 * c); M->z2 = (float*)dalloc(L, 4 * 64 * c); M->z3 = (float*)dalloc(L, 4 * 64 * c);
+  M->c = (float*)dalloc(L, 4 * 64 * c); M->dh = (float*)dalloc(L, 4 * 64 * c);
+  M->dyf = (float*)dalloc(L, 4 * 16 * c); M->dzf = (float*)dalloc(L, 4 * 64 * c);
+  M->part = (float*)dalloc(L, 4 * (size_t)(c / 512 + 2) * 64 * 64);
+  M->lossp = (float*)dalloc(L, 4 * (size_t)(16 * c / 256 + 2));
+  M->cap = c;
+  if (L.alloc_failed) { les_set_error("out of memory (MLP training)"); return 3; }
+  return 0;
+}
+
+// C[M x N] = A[M x K] B[K x N], split-K for long K
+static void gemm(Les& L, const short* A, long sam, long sak, const short* B, long sbk, long sbn, float* C, int M, int N,
+                 int K) {
+  int kchunk = K, nsplit = 1;
+  if (K > 1024) {
+    kchunk = 512;
+    nsplit = (K + kchunk - 1) / kchunk;
+  }
+  GemmArgs g;
+  g.A = A; g.sam = sam; g.sak = sak; g.B = B; g.sbk = sbk; g.sbn = sbn;
+  g.C = nsplit > 1 ? L.mlp->part : C;
+  g.M = M; g.N = N; g.K = K; g.kchunk = kchunk;
+  hipLaunchKernelGGL(k_gemm_bf16, dim3((M / 16) * (N / 16), nsplit), dim3(64), 0, L.stream, g);
+  if (nsplit > 1) {
+    const long n = (long)M * N;
+    hipLaunchKernelGGL(k_sum_splits, dim3(nblk(n)), dim3(LES_BLOCK), 0, L.stream, L.mlp->part, C, n, nsplit, 0);
+  }
+}
+
+// forward + backward over a batch of n points (n multiple of 16); fills
+// L.mlp->g with the gradient of 0.5 * mean ||y - t||^2 and returns the loss
+static int mlp_forward_backward(Les& L, const short* X, const float* Y, long n, double* loss) {
+  MlpState* M = L.mlp;
+  if (n % 16 || n <= 0) { les_set_error("batch must be a positive multiple of 16"); return 10; }
+  int rc = mlp_train_alloc(L, n);
+  if (rc) return rc;
+  const int N = (int)n;
+  const float* b = M->w + MLP_NMAT;
+  const long n64 = n * 64;
+  // forward
+  gemm(L, X, 16, 1, M->wb + MW1, 1, 16, M->c, N, 64, 16);
+  hipLaunchKernelGGL(k_bias_gelu, dim3(nblk(n64)), dim3(LES_BLOCK), 0, L.stream, M->c, b + MB1, M->z1, M->h1, n);
+  gemm(L, M->h1, 64, 1, M->wb + MW2, 1, 64, M->c, N, 64, 64);
+  hipLaunchKernelGGL(k_bias_gelu, dim3(nblk(n64)), dim3(LES_BLOCK), 0, L.stream, M->c, b + MB2, M->z2, M->h2, n);
+  gemm(L, M->h2, 64, 1, M->wb + MW3, 1, 64, M->c, N, 64, 64);
+  hipLaunchKernelGGL(k_bias_gelu, dim3(nblk(n64)), dim3(LES_BLOCK), 0, L.stream, M->c, b + MB3, M->z3, M->h3, n);
+  gemm(L, M->h3, 64, 1, M->wb + MW4, 1, 64, M->c, N, 16, 64);
+  const unsigned nb16 = nblk(n * 16);
+  hipLaunchKernelGGL(k_out_grad, dim3(nb16), dim3(LES_BLOCK), 0, L.stream, M->c, b + MB4, Y, M->dy, M->dyf, M->lossp, n);
+  // backward
+  float* g = M->g;
+  gemm(L, M->dy, 1, 16, M->h3, 64, 1, g + MW4, 16, 64, N);
+  hipLaunchKernelGGL(k_colsum, dim3(16), dim3(LES_BLOCK), 0, L.stream, M->dyf, g + MLP_NMAT + MB4, n, 16);
+  gemm(L, M->dy, 16, 1, M->wb + MW4, 64, 1, M->dh, N, 64, 16);
+  const short* hin[3] = {X, M->h1, M->h2};
+  const float* zs[3] = {M->z1, M->z2, M->z3};
+  const int mw[3] = {MW1, MW2, MW3};
+  const int mb[3] = {MB1, MB2, MB3};
+  const int kin[3] = {16, 64, 64};
+  for (int layer = 2; layer >= 0; --layer) {
+    hipLaunchKernelGGL(k_gelu_back, dim3(nblk(n64)), dim3(LES_BLOCK), 0, L.stream, M->dh, zs[layer], M->dz, M->dzf, n);
+    hipLaunchKernelGGL(k_colsum, dim3(64), dim3(LES_BLOCK), 0, L.stream, M->dzf, g + MLP_NMAT + mb[layer], n, 64);
+    gemm(L, M->dz, 1, 64, hin[layer], kin[layer], 1, g + mw[layer], 64, kin[layer], N);
+    if (layer > 0) gemm(L, M->dz, 64, 1, M->wb + mw[layer], kin[layer], 1, M->dh, N, kin[layer], 64);
+  }
+  if (loss) {
+    std::vector<float> lp(nb16);
+    LES_CHECK(hipMemcpyAsync(lp.data(), M->lossp, 4 * nb16, hipMemcpyDeviceToHost, L.stream));
+    LES_CHECK(hipStreamSynchronize(L.stream));
+    double s = 0.0;
+    for (float x : lp) s += x;
+    *loss = 0.5 * s / (double)n;
+  }
+  return 0;
+}
+
+// features + inference for the current gradients (S_ij already written);
+// cs/beta targets of elem size eb
+template <typename T>
+static void learned_coeffs(Les& L, void* cs, void* beta, int eb) {
+  MlpState* M = L.mlp;
+  const long npt = L.np * (L.nz + 1);
+  SgsArgs<T> a = sgs_args<T>(L);
+  T* rr[3] = {(T*)L.wk_real[0], (T*)L.wk_real[1], (T*)L.wk_real[2]};
+  hipLaunchKernelGGL((k_rij<T>), dim3(nblk(npt)), dim3(LES_BLOCK), 0, L.stream, a, rr[0], rr[1], rr[2]);
+  const T* in9[9] = {F<T>(L, LES_S11), F<T>(L, LES_S12), F<T>(L, LES_S13), F<T>(L, LES_S22),
+                     F<T>(L, LES_S23), F<T>(L, LES_S33), rr[0], rr[1], rr[2]};
+  T* f2[9];
+  T* f4[9];
+  for (int i = 0; i < 9; ++i) { f2[i] = (T*)L.wk_real[15 + i]; f4[i] = (T*)L.wk_real[36 + i]; }
+  const T* g2 = sizeof(T) == 8 ? (const T*)L.g2_64 : (const T*)L.g2_32;
+  const T* g4 = sizeof(T) == 8 ? (const T*)L.g4_64 : (const T*)L.g4_32;
+  for (int f0 = 0; f0 < 9; f0 += 3) {
+    const T* src[3];
+    Cplx<T>* sp[3];
+    for (int t = 0; t < 3; ++t) { src[t] = in9[f0 + t]; sp[t] = (Cplx<T>*)L.wk_spec[t]; }
+    rows_r2c<T, T>(L, src, sp, 3, 1, L.nz + 1, (T)1);
+    for (int t = 0; t < 3; ++t)
+      cols<T>(L, COP_FILT2, sp[t], nullptr, (Cplx<T>*)L.wk_spec[3 + 2 * t], (Cplx<T>*)L.wk_spec[4 + 2 * t], nullptr, 2, 1,
+              1, 1, L.nz + 1, g2, g4);
+    const Cplx<T>* cs6[6];
+    T* outs[6];
+    for (int t = 0; t < 3; ++t) {
+      cs6[2 * t] = (Cplx<T>*)L.wk_spec[3 + 2 * t];
+      cs6[2 * t + 1] = (Cplx<T>*)L.wk_spec[4 + 2 * t];
+      outs[2 * t] = f2[f0 + t];
+      outs[2 * t + 1] = f4[f0 + t];
+    }
+    rows_c2r<T, T>(L, cs6, outs, 6, 1, L.nz + 1);
+  }
+  FeatArgs fa;
+  for (int i = 0; i < 6; ++i) { fa.s[0][i] = (const float*)in9[i]; fa.s[1][i] = (const float*)f2[i]; fa.s[2][i] = (const float*)f4[i]; }
+  for (int i = 0; i < 3; ++i) { fa.r[0][i] = (const float*)in9[6 + i]; fa.r[1][i] = (const float*)f2[6 + i]; fa.r[2][i] = (const float*)f4[6 + i]; }
+  fa.np = L.np; fa.nz = L.nz; fa.dz = (float)L.dz; fa.delta = (float)L.delta;
+  fa.xmean = M->norm; fa.xstd = M->norm + 12; fa.xlim = M->hn[28];
+  fa.X = M->X; fa.raw = nullptr; fa.tgt = nullptr; fa.cs = nullptr; fa.beta = nullptr; fa.elem_bytes = 4;
+  fa.stride = 1; fa.nsel = npt;
+  kt_begin(L, KT_MLP);
+  hipLaunchKernelGGL(k_features, dim3(nblk(npt)), dim3(LES_BLOCK), 0, L.stream, fa);
+  if (cs) {
+    InferArgs ia;
+    ia.X = M->X; ia.Wb = M->wb; ia.bias = M->w + MLP_NMAT;
+    ia.ymean0 = M->hn[24]; ia.ymean1 = M->hn[25]; ia.ystd0 = M->hn[26]; ia.ystd1 = M->hn[27];
+    ia.n = npt; ia.np = L.np; ia.nz = L.nz;
+    ia.cs = cs; ia.beta = beta; ia.elem_bytes = eb;
+    ia.raw_out = nullptr; ia.write_fields = 1;
+    hipLaunchKernelGGL(k_mlp_infer, dim3(2048), dim3(256), 0, L.stream, ia);
+  }
+  kt_end(L, KT_MLP, (double)npt * (27.0 * sizeof(T) + 2.0 * eb));
+}
+
+static double reduce_rel(Les& L, const void* a, const void* b, int ea, int eb) {
+  MlpState* M = L.mlp;
+  const long npt = L.np * (L.nz + 1);
+  DiffArgs d;
+  d.a = a; d.b = b; d.ea = ea; d.eb = eb; d.off = L.np; d.n = npt; d.part = M->red;
+  const unsigned nb = nblk(npt);
+  hipLaunchKernelGGL(k_diff_norm, dim3(nb), dim3(LES_BLOCK), 0, L.stream, d);
+  std::vector<float> h(2 * nb);
+  hipMemcpyAsync(h.data(), M->red, 8 * (size_t)nb, hipMemcpyDeviceToHost, L.stream);
+  hipStreamSynchronize(L.stream);
+  double s1 = 0, s2 = 0;
+  for (unsigned i = 0; i < nb; ++i) { s1 += h[2 * i]; s2 += h[2 * i + 1]; }
+  return s2 > 0 ? std::sqrt(s1 / s2) : std::sqrt(s1);
+}
+
+// Coefficient update of the learned mode on a dynamic-procedure step:
+// judge steps (and every step until weights are loaded) run the plain
+// procedure; the surrogate is evaluated beside it for the judge statistics.
+// Other update steps use the surrogate; out-of-range points keep their
+// last coefficients.
+template <typename T>
+static void learned_update(Les& L, int jtt) {
+  const bool judge = !(L.mlp && L.mlp->ready) || (L.P.judge_every > 0 && jtt % L.P.judge_every == 0);
+  if (judge) {
+    lasd_update<T>(L, jtt);
+    if (L.mlp && L.mlp->ready) {
+      void* tcs = L.wk_real[57];
+      void* tbe = L.wk_real[58];
+      const size_t bytes = sizeof(T) * (size_t)L.nf;
+      hipMemcpyAsync(tcs, L.f[LES_CS_OPT2], bytes, hipMemcpyDeviceToDevice, L.stream);
+      hipMemcpyAsync(tbe, L.f[LES_BETA], bytes, hipMemcpyDeviceToDevice, L.stream);
+      learned_coeffs<T>(L, tcs, tbe, sizeof(T));
+      L.judge[0] = reduce_rel(L, tcs, L.f[LES_CS_OPT2], sizeof(T), sizeof(T));
+      L.judge[1] = reduce_rel(L, tbe, L.f[LES_BETA], sizeof(T), sizeof(T));
+      L.judge[3] = (double)jtt;
+    }
+    return;
+  }
+  learned_coeffs<T>(L, L.f[LES_CS_OPT2], L.f[LES_BETA], sizeof(T));
+  const size_t tb = sizeof(T) * (size_t)L.nf;
+  hipMemsetAsync(L.f[LES_U_LAG], 0, tb, L.stream);
+  hipMemsetAsync(L.f[LES_V_LAG], 0, tb, L.stream);
+  hipMemsetAsync(L.f[LES_W_LAG], 0, tb, L.stream);
+}
+
+// compact public weight layout <-> padded device layout
+static void mlp_unpack(const float* pub, std::vector<float>& dev, float* hn) {
+  dev.assign(MLP_NPAR, 0.f);
+  const float* p = pub;
+  for (int o = 0; o < 64; ++o) for (int i = 0; i < 12; ++i) dev[MW1 + o * 16 + i] = *p++;
+  for (int o = 0; o < 64; ++o) dev[MLP_NMAT + MB1 + o] = *p++;
+  for (int o = 0; o < 64; ++o) for (int i = 0; i < 64; ++i) dev[MW2 + o * 64 + i] = *p++;
+  for (int o = 0; o < 64; ++o) dev[MLP_NMAT + MB2 + o] = *p++;
+  for (int o = 0; o < 64; ++o) for (int i = 0; i < 64; ++i) dev[MW3 + o * 64 + i] = *p++;
+  for (int o = 0; o < 64; ++o) dev[MLP_NMAT + MB3 + o] = *p++;
+  for (int o = 0; o < 2; ++o) for (int i = 0; i < 64; ++i) dev[MW4 + o * 64 + i] = *p++;
+  for (int o = 0; o < 2; ++o) dev[MLP_NMAT + MB4 + o] = *p++;
+  for (int i = 0; i < 29; ++i) hn[i] = *p++;
+}
+
+static void mlp_pack(const std::vector<float>& dev, const float* hn, float* pub) {
+  float* p = pub;
+  for (int o = 0; o < 64; ++o) for (int i = 0; i < 12; ++i) *p++ = dev[MW1 + o * 16 + i];
+  for (int o = 0; o < 64; ++o) *p++ = dev[MLP_NMAT + MB1 + o];
+  for (int o = 0; o < 64; ++o) for (int i = 0; i < 64; ++i) *p++ = dev[MW2 + o * 64 + i];
+  for (int o = 0; o < 64; ++o) *p++ = dev[MLP_NMAT + MB2 + o];
+  for (int o = 0; o < 64; ++o) for (int i = 0; i < 64; ++i) *p++ = dev[MW3 + o * 64 + i];
+  for (int o = 0; o < 64; ++o) *p++ = dev[MLP_NMAT + MB3 + o];
+  for (int o = 0; o < 2; ++o) for (int i = 0; i < 64; ++i) *p++ = dev[MW4 + o * 64 + i];
+  for (int o = 0; o < 2; ++o) *p++ = dev[MLP_NMAT + MB4 + o];
+  for (int i = 0; i < 29; ++i) *p++ = hn[i];
+}
+
+__global__ void k_f2bf(const float* a, short* b, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) b[t] = f2bf(a[t]);
+}
+
+extern "C" {
+
+int les_mlp_layout(long* nweights) {
+  if (nweights) *nweights = MLP_NPUB;
+  return 0;
+}
+
+int les_mlp_set_weights(const float* w, long nweights) {
+  Les* L = g_les;
+  if (!L) return 1;
+  if (nweights != MLP_NPUB) { les_set_error("les_mlp_set_weights: expected %d values", MLP_NPUB); return 10; }
+  int rc = mlp_alloc(*L);
+  if (rc) return rc;
+  MlpState* M = L->mlp;
+  std::vector<float> dev;
+  mlp_unpack(w, dev, M->hn);
+  LES_CHECK(hipMemcpyAsync(M->w, dev.data(), 4 * MLP_NPAR, hipMemcpyHostToDevice, L->stream));
+  LES_CHECK(hipMemcpyAsync(M->norm, M->hn, 4 * 24, hipMemcpyHostToDevice, L->stream));
+  LES_CHECK(hipMemsetAsync(M->m, 0, 4 * MLP_NPAR, L->stream));
+  LES_CHECK(hipMemsetAsync(M->v, 0, 4 * MLP_NPAR, L->stream));
+  hipLaunchKernelGGL(k_f2bf, dim3(nblk(MLP_NMAT)), dim3(LES_BLOCK), 0, L->stream, M->w, M->wb, MLP_NMAT);
+  LES_CHECK(hipStreamSynchronize(L->stream));
+  M->step = 0;
+  M->ready = 1;
+  return 0;
+}
+
+int les_mlp_get_weights(float* w) {
+  Les* L = g_les;
+  if (!L || !L->mlp) return 1;
+  std::vector<float> dev(MLP_NPAR);
+  LES_CHECK(hipStreamSynchronize(L->stream));
+  LES_CHECK(hipMemcpy(dev.data(), L->mlp->w, 4 * MLP_NPAR, hipMemcpyDeviceToHost));
+  mlp_pack(dev, L->mlp->hn, w);
+  return 0;
+}
+
+// raw samples (device: [n][12] features, [n][2] targets) -> normalised
+// training buffers (device: bf16 [n][16], float [n][16])
+int les_mlp_prepare(const void* d_raw, const void* d_tgt, long n, void* d_x, void* d_y) {
+  Les* L = g_les;
+  if (!L || !L->mlp) { les_set_error("les_mlp_prepare: set weights/normalisers first"); return 1; }
+  MlpState* M = L->mlp;
+  hipLaunchKernelGGL(k_prepare, dim3(nblk(n)), dim3(LES_BLOCK), 0, L->stream, (const float*)d_raw, (const float*)d_tgt,
+                     M->norm, M->norm + 12, M->hn[24], M->hn[26], M->hn[25], M->hn[27], (short*)d_x, (float*)d_y, n);
+  LES_CHECK(hipGetLastError());
+  LES_CHECK(hipStreamSynchronize(L->stream));
+  return 0;
+}
+
+// gradient (compact layout, weights and biases only) and loss of a batch
+int les_mlp_grad(const void* d_x, const void* d_y, long n, float* grad, double* loss) {
+  Les* L = g_les;
+  if (!L || !L->mlp) return 1;
+  int rc = mlp_forward_backward(*L, (const short*)d_x, (const float*)d_y, n, loss);
+  if (rc) return rc;
+  std::vector<float> dev(MLP_NPAR);
+  LES_CHECK(hipStreamSynchronize(L->stream));
+  LES_CHECK(hipMemcpy(dev.data(), L->mlp->g, 4 * MLP_NPAR, hipMemcpyDeviceToHost));
+  std::vector<float> pub(MLP_NPUB);
+  mlp_pack(dev, L->mlp->hn, pub.data());
+  std::memcpy(grad, pub.data(), 4 * (MLP_NPUB - 29));
+  return 0;
+}
+
+// one Adam step on a batch
+int les_mlp_train_step(const void* d_x, const void* d_y, long n, double lr, double* loss) {
+  Les* L = g_les;
+  if (!L || !L->mlp) return 1;
+  MlpState* M = L->mlp;
+  int rc = mlp_forward_backward(*L, (const short*)d_x, (const float*)d_y, n, loss);
+  if (rc) return rc;
+  M->step += 1;
+  const float b1c = 1.0f - powf(0.9f, (float)M->step), b2c = 1.0f - powf(0.999f, (float)M->step);
+  hipLaunchKernelGGL(k_adam, dim3(nblk(MLP_NPAR)), dim3(LES_BLOCK), 0, L->stream, M->w, M->m, M->v, M->g, M->wb,
+                     MLP_NPAR, MLP_NMAT, (float)lr, b1c, b2c);
+  LES_CHECK(hipGetLastError());
+  return 0;
+}
+
+// fused inference on a feature batch (device bf16 [n][16]) -> normalised
+// outputs (device float [n][2])
+int les_mlp_forward(const void* d_x, long n, void* d_out) {
+  Les* L = g_les;
+  if (!L || !L->mlp) return 1;
+  MlpState* M = L->mlp;
+  InferArgs ia;
+  ia.X = (const short*)d_x; ia.Wb = M->wb; ia.bias = M->w + MLP_NMAT;
+  ia.ymean0 = 0; ia.ymean1 = 0; ia.ystd0 = 1; ia.ystd1 = 1;
+  ia.n = n; ia.np = L->np; ia.nz = L->nz;
+  ia.cs = nullptr; ia.beta = nullptr; ia.elem_bytes = 4;
+  ia.raw_out = (float*)d_out; ia.write_fields = 0;
+  hipLaunchKernelGGL(k_mlp_infer, dim3(256), dim3(256), 0, L->stream, ia);
+  LES_CHECK(hipGetLastError());
+  LES_CHECK(hipStreamSynchronize(L->stream));
+  return 0;
+}
+
+// training samples at the current state (call right after a step with a
+// plain dynamic-procedure update; mixed-precision modes only)
+int les_sgs_samples(void* d_x, void* d_y, long max_samples, long* n_written, int stride) {
+  Les* L = g_les;
+  if (!L) return 1;
+  if (!L->mixed) { les_set_error("les_sgs_samples: run in a mixed-precision mode"); return 1; }
+  if (stride < 1) stride = 1;
+  int rc = mlp_alloc(*L);
+  if (rc) return rc;
+  const long npt = L->np * (L->nz + 1);
+  long nsel = (npt + stride - 1) / stride;
+  if (nsel > max_samples) nsel = max_samples;
+  SgsArgs<float> a = sgs_args<float>(*L);
+  a.accumulate = 0;
+  a.write_sij = 1;
+  hipLaunchKernelGGL((k_sij<float>), dim3(nblk(npt)), dim3(LES_BLOCK), 0, L->stream, a);
+  // filtered invariants, then raw features and the current coefficients
+  MlpState* M = L->mlp;
+  float save = M->hn[28];
+  learned_coeffs<float>(*L, nullptr, nullptr, 4);  // fills the filtered fields (feature output unused)
+  M->hn[28] = save;
+  FeatArgs fa;
+  const float* rr[3] = {(const float*)L->wk_real[0], (const float*)L->wk_real[1], (const float*)L->wk_real[2]};
+  const int sid[6] = {LES_S11, LES_S12, LES_S13, LES_S22, LES_S23, LES_S33};
+  for (int i = 0; i < 6; ++i) {
+    fa.s[0][i] = (const float*)L->f[sid[i]];
+    fa.s[1][i] = (const float*)L->wk_real[15 + i];
+    fa.s[2][i] = (const float*)L->wk_real[36 + i];
+  }
+  for (int i = 0; i < 3; ++i) {
+    fa.r[0][i] = rr[i];
+    fa.r[1][i] = (const float*)L->wk_real[21 + i];
+    fa.r[2][i] = (const float*)L->wk_real[42 + i];
+  }
+  fa.np = L->np; fa.nz = L->nz; fa.dz = (float)L->dz; fa.delta = (float)L->delta;
+  fa.xmean = M->norm; fa.xstd = M->norm + 12; fa.xlim = 1e30f;
+  fa.X = nullptr; fa.raw = (float*)d_x; fa.tgt = (float*)d_y;
+  fa.cs = L->f[LES_CS_OPT2]; fa.beta = L->f[LES_BETA]; fa.elem_bytes = L->fbytes[LES_CS_OPT2];
+  fa.stride = stride; fa.nsel = nsel;
+  hipLaunchKernelGGL(k_features, dim3(nblk(nsel)), dim3(LES_BLOCK), 0, L->stream, fa);
+  LES_CHECK(hipGetLastError());
+  LES_CHECK(hipStreamSynchronize(L->stream));
+  if (n_written) *n_written = nsel;
+  return 0;
+}
+
+int les_mlp_judge_stats(double* out) {
+  Les* L = g_les;
+  if (!L) return 1;
+  for (int i = 0; i < 4; ++i) out[i] = L->judge[i];
+  return 0;
+}
+
+}  // extern "C"

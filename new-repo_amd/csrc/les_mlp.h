@@ -1,0 +1,438 @@
+// Learned SGS surrogate: a per-point MLP 12 -> 64 -> 64 -> 64 -> 2 (GELU)
+// that predicts the Lagrangian scale-dependent model's Cs^2 and beta from
+// strain/rotation invariants at the grid and test-filter scales.  BF16
+// weights and activations, FP32 accumulation on MFMA
+// (v_mfma_f32_16x16x16_bf16).  No counterpart in the reference; the plain
+// dynamic procedure (les_lasd.h) generates its training targets, serves as
+// the fallback for out-of-range inputs and judges it periodically.
+// Included by les_gpu.hip.
+#pragma once
+
+typedef short s4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+#define MLP_IN 12
+#define MLP_H 64
+#define MLP_OUT 2
+// padded device layout of the matrices (row = output unit, col = input)
+#define MW1 0                      // [64][16]
+#define MW2 (MW1 + 64 * 16)        // [64][64]
+#define MW3 (MW2 + 64 * 64)        // [64][64]
+#define MW4 (MW3 + 64 * 64)        // [16][64], rows 2..15 zero
+#define MLP_NMAT (MW4 + 16 * 64)
+#define MB1 0
+#define MB2 64
+#define MB3 128
+#define MB4 192
+#define MLP_NBIAS 208
+#define MLP_NPAR (MLP_NMAT + MLP_NBIAS)
+// compact public layout: W1[64][12] b1 W2[64][64] b2 W3[64][64] b3 W4[2][64]
+// b4[2] xmean[12] xstd[12] ymean[2] ystd[2] xlim
+#define MLP_NPUB (64 * 12 + 64 + 64 * 64 + 64 + 64 * 64 + 64 + 2 * 64 + 2 + 12 + 12 + 2 + 2 + 1)
+
+__device__ __forceinline__ short f2bf(float f) {
+  unsigned u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (short)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(short b) { return __uint_as_float(((unsigned)(unsigned short)b) << 16); }
+
+__device__ __forceinline__ f4v mfma16(s4v a, s4v b, f4v c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+// GELU, tanh form, and its derivative
+__device__ __forceinline__ float gelu(float x) {
+  const float c = 0.7978845608028654f;
+  return 0.5f * x * (1.0f + tanhf(c * (x + 0.044715f * x * x * x)));
+}
+__device__ __forceinline__ float gelu_d(float x) {
+  const float c = 0.7978845608028654f;
+  const float t = tanhf(c * (x + 0.044715f * x * x * x));
+  return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * c * (1.0f + 3.0f * 0.044715f * x * x);
+}
+
+// ------------------------------------------------------------------ GEMM
+// C[M x N] (fp32) = A[M x K] * B[K x N], bf16 operands with arbitrary
+// strides (so transposes are stride swaps), one 16x16 tile per wave,
+// split-K over blockIdx.y into partial slabs C + split * M * N.
+// M, N, K are multiples of 16.  Lane layout of v_mfma_f32_16x16x16_bf16:
+// A[m = l%16][k = 4(l/16) + j], B[k = 4(l/16) + j][n = l%16],
+// D[m = 4(l/16) + j][n = l%16], j = 0..3.
+struct GemmArgs {
+  const short* A;
+  long sam, sak;
+  const short* B;
+  long sbk, sbn;
+  float* C;
+  int M, N, K, kchunk;
+};
+
+__global__ void __launch_bounds__(64) k_gemm_bf16(GemmArgs g) {
+  const int l = threadIdx.x;
+  const int tn = g.N / 16;
+  const int m0 = (blockIdx.x / tn) * 16, n0 = (blockIdx.x % tn) * 16;
+  const int k0 = blockIdx.y * g.kchunk;
+  int k1 = k0 + g.kchunk;
+  if (k1 > g.K) k1 = g.K;
+  f4v acc = {0.f, 0.f, 0.f, 0.f};
+  const int r = l & 15, q = (l >> 4) * 4;
+  for (int k = k0; k < k1; k += 16) {
+    s4v a, b;
+    for (int j = 0; j < 4; ++j) {
+      a[j] = g.A[(long)(m0 + r) * g.sam + (long)(k + q + j) * g.sak];
+      b[j] = g.B[(long)(k + q + j) * g.sbk + (long)(n0 + r) * g.sbn];
+    }
+    acc = mfma16(a, b, acc);
+  }
+  float* C = g.C + (long)blockIdx.y * g.M * g.N;
+  for (int j = 0; j < 4; ++j) C[(long)(m0 + q + j) * g.N + n0 + r] = acc[j];
+}
+
+__global__ void k_sum_splits(const float* part, float* out, long n, int nsplit, int accumulate) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  float s = 0.f;
+  for (int i = 0; i < nsplit; ++i) s += part[(long)i * n + t];
+  out[t] = accumulate ? out[t] + s : s;
+}
+
+// Z = C + bias (per column), H = gelu(Z) in bf16; rows of width 64
+__global__ void k_bias_gelu(const float* C, const float* bias, float* Z, short* H, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * 64) return;
+  const float z = C[t] + bias[t & 63];
+  Z[t] = z;
+  H[t] = f2bf(gelu(z));
+}
+
+// output layer: Y = C + b4 (16 padded columns, 2 used), dY = (Y - T) / n,
+// per-block loss partials
+__global__ void k_out_grad(const float* C, const float* b4, const float* T, short* dY, float* dYf, float* lossp,
+                           long n) {
+  __shared__ float red[256];
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  float e2 = 0.f;
+  if (t < n * 16) {
+    const int c = (int)(t & 15);
+    float d = 0.f;
+    if (c < MLP_OUT) {
+      const float y = C[t] + b4[c];
+      const float e = y - T[t];
+      e2 = e * e;
+      d = e / (float)n;
+    }
+    dY[t] = f2bf(d);
+    dYf[t] = d;
+  }
+  red[threadIdx.x] = e2;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) lossp[blockIdx.x] = red[0];
+}
+
+// dZ = dH * gelu'(Z)
+__global__ void k_gelu_back(const float* dH, const float* Z, short* dZ, float* dZf, long n) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * 64) return;
+  const float d = dH[t] * gelu_d(Z[t]);
+  dZ[t] = f2bf(d);
+  dZf[t] = d;
+}
+
+// column sums of a [n x w] fp32 matrix (bias gradients)
+__global__ void k_colsum(const float* X, float* out, long n, int w) {
+  __shared__ float red[256];
+  const int c = blockIdx.x;
+  float s = 0.f;
+  for (long r = threadIdx.x; r < n; r += blockDim.x) s += X[r * w + c];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = blockDim.x / 2; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[c] = red[0];
+}
+
+// Adam on the fp32 master parameters; refreshes the bf16 matrix mirror
+__global__ void k_adam(float* w, float* m, float* v, const float* g, short* wb, int n, int nmat, float lr, float b1c,
+                       float b2c) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const float gg = g[t];
+  const float mm = 0.9f * m[t] + 0.1f * gg;
+  const float vv = 0.999f * v[t] + 0.001f * gg * gg;
+  m[t] = mm;
+  v[t] = vv;
+  const float nw = w[t] - lr * (mm / b1c) / (sqrtf(vv / b2c) + 1e-8f);
+  w[t] = nw;
+  if (t < nmat) wb[t] = f2bf(nw);
+}
+
+// ------------------------------------------------------------------ inference
+// Fused forward pass for 16-point tiles, one tile per wave iteration, in the
+// transposed form Z^T = W X^T: the D registers of one layer are the B
+// operand of the next, so activations never leave registers.  Writes Cs^2
+// and beta (float32-valued like the dynamic procedure's casts); points
+// whose features are out of the training range keep the previous values.
+struct InferArgs {
+  const short* X;      // [n][16] normalised bf16 features, slot 15 = out-of-range flag
+  const short* Wb;     // bf16 matrices (MW1..MW4)
+  const float* bias;   // MB1..MB4
+  float ymean0, ystd0, ymean1, ystd1;
+  long n;              // points (levels 1..nz+1 of np each)
+  long np;
+  int nz;
+  void* cs;            // float or double per elem_bytes
+  void* beta;
+  int elem_bytes;
+  float* raw_out;      // optional: normalised outputs [n][2]
+  int write_fields;
+};
+
+__device__ __forceinline__ void store_coef(const InferArgs& a, long p, double cs, double beta) {
+  const long c = a.np + p;  // field index of level 1 + p
+  if (a.elem_bytes == 8) {
+    ((double*)a.cs)[c] = cs;
+    ((double*)a.beta)[c] = beta;
+  } else {
+    ((float*)a.cs)[c] = (float)cs;
+    ((float*)a.beta)[c] = (float)beta;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_mlp_infer(InferArgs a) {
+  const int l = threadIdx.x & 63;
+  const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const long nwaves = ((long)gridDim.x * blockDim.x) >> 6;
+  const int r = l & 15, q = (l >> 4) * 4;
+  // weights as A operands: W[16 rb + r][16 cb + q + j]
+  s4v w1[4], w2[16], w3[16], w4[4];
+  for (int rb = 0; rb < 4; ++rb)
+    for (int j = 0; j < 4; ++j) w1[rb][j] = a.Wb[MW1 + (16 * rb + r) * 16 + q + j];
+  for (int rb = 0; rb < 4; ++rb)
+    for (int cb = 0; cb < 4; ++cb)
+      for (int j = 0; j < 4; ++j) {
+        w2[rb * 4 + cb][j] = a.Wb[MW2 + (16 * rb + r) * 64 + 16 * cb + q + j];
+        w3[rb * 4 + cb][j] = a.Wb[MW3 + (16 * rb + r) * 64 + 16 * cb + q + j];
+      }
+  for (int cb = 0; cb < 4; ++cb)
+    for (int j = 0; j < 4; ++j) w4[cb][j] = a.Wb[MW4 + r * 64 + 16 * cb + q + j];
+  float bb1[4][4], bb2[4][4], bb3[4][4], bb4[4];
+  for (int rb = 0; rb < 4; ++rb)
+    for (int j = 0; j < 4; ++j) {
+      bb1[rb][j] = a.bias[MB1 + 16 * rb + q + j];
+      bb2[rb][j] = a.bias[MB2 + 16 * rb + q + j];
+      bb3[rb][j] = a.bias[MB3 + 16 * rb + q + j];
+    }
+  for (int j = 0; j < 4; ++j) bb4[j] = a.bias[MB4 + q + j];
+  const long ntiles = (a.n + 15) / 16;
+  for (long tile = wave; tile < ntiles; tile += nwaves) {
+    const long p = tile * 16 + r;
+    s4v x = {0, 0, 0, 0};
+    if (p < a.n)
+      for (int j = 0; j < 4; ++j) x[j] = a.X[p * 16 + q + j];
+    s4v h1[4], h2[4], h3[4];
+    for (int rb = 0; rb < 4; ++rb) {
+      f4v acc = {bb1[rb][0], bb1[rb][1], bb1[rb][2], bb1[rb][3]};
+      acc = mfma16(w1[rb], x, acc);
+      for (int j = 0; j < 4; ++j) h1[rb][j] = f2bf(gelu(acc[j]));
+    }
+    for (int rb = 0; rb < 4; ++rb) {
+      f4v acc = {bb2[rb][0], bb2[rb][1], bb2[rb][2], bb2[rb][3]};
+      for (int cb = 0; cb < 4; ++cb) acc = mfma16(w2[rb * 4 + cb], h1[cb], acc);
+      for (int j = 0; j < 4; ++j) h2[rb][j] = f2bf(gelu(acc[j]));
+    }
+    for (int rb = 0; rb < 4; ++rb) {
+      f4v acc = {bb3[rb][0], bb3[rb][1], bb3[rb][2], bb3[rb][3]};
+      for (int cb = 0; cb < 4; ++cb) acc = mfma16(w3[rb * 4 + cb], h2[cb], acc);
+      for (int j = 0; j < 4; ++j) h3[rb][j] = f2bf(gelu(acc[j]));
+    }
+    f4v y = {bb4[0], bb4[1], bb4[2], bb4[3]};
+    for (int cb = 0; cb < 4; ++cb) y = mfma16(w4[cb], h3[cb], y);
+    // lanes 0..15 hold outputs 0..3 of point r; the range flag of point r is
+    // element 3 of lane 48 + r's feature registers
+    const int flag_bits = __shfl((int)(unsigned short)x[3], 48 + r, 64);
+    if (l < 16 && p < a.n) {
+      if (a.raw_out) {
+        a.raw_out[p * 2] = y[0];
+        a.raw_out[p * 2 + 1] = y[1];
+      }
+      if (a.write_fields && bf2f((short)flag_bits) == 0.0f) {
+        const int k = 1 + (int)(p / a.np);
+        const float c2 = y[0] * a.ystd0 + a.ymean0;
+        float b = y[1] * a.ystd1 + a.ymean1;
+        b = b > 0.125f ? b : 0.125f;
+        if (k == a.nz + 1) b = 1.0f;
+        const float cs = c2 > 1e-24f ? c2 : 1e-24f;
+        store_coef(a, p, (double)cs, (double)b);
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ features
+// Rotation R_ij on the nodes of S_ij (src/compute_sgs.f90:260-335 averaging)
+template <typename T>
+__global__ void k_rij(SgsArgs<T> a, T* r12, T* r13, T* r23) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.np * (a.nz + 1)) return;
+  const int k = 1 + (int)(t / a.np);
+  const long q = t % a.np;
+  const long c = (long)k * a.np + q;
+  const T h = (T)0.5;
+  T uy, uz, vx, vz, wx, wy;
+  if (k == 1) {
+    uy = a.dudy[c]; uz = a.dudz[c]; vz = a.dvdz[c]; vx = a.dvdx[c];
+    wx = h * (a.dwdx[c] + a.dwdx[c + a.np]);
+    wy = h * (a.dwdy[c] + a.dwdy[c + a.np]);
+  } else {
+    const long b = c - a.np;
+    uy = h * (a.dudy[c] + a.dudy[b]);
+    uz = a.dudz[c];
+    vx = h * (a.dvdx[c] + a.dvdx[b]);
+    vz = a.dvdz[c];
+    wx = a.dwdx[c];
+    wy = a.dwdy[c];
+  }
+  r12[c] = h * (uy - vx);
+  r13[c] = h * (uz - wx);
+  r23[c] = h * (vz - wy);
+}
+
+struct FeatArgs {
+  const float* s[3][6];  // S11 S12 S13 S22 S23 S33 at delta, 2 delta, 4 delta
+  const float* r[3][3];  // R12 R13 R23
+  long np;
+  int nz;
+  float dz, delta;
+  const float* xmean;
+  const float* xstd;
+  float xlim;
+  short* X;     // normalised bf16 [n][16] (inference / training layout) or null
+  float* raw;   // raw features [nsel][12] or null
+  float* tgt;   // targets [nsel][2] (Cs^2, beta) or null
+  const void* cs;
+  const void* beta;
+  int elem_bytes;
+  long stride;  // sample every stride-th point when raw != null
+  long nsel;
+};
+
+__device__ __forceinline__ void invariants(const float* s, const float* rr, float* out) {
+  // S symmetric, R antisymmetric (R21 = -R12, ...)
+  const float S[3][3] = {{s[0], s[1], s[2]}, {s[1], s[3], s[4]}, {s[2], s[4], s[5]}};
+  const float R[3][3] = {{0.f, rr[0], rr[1]}, {-rr[0], 0.f, rr[2]}, {-rr[1], -rr[2], 0.f}};
+  float SS = 0.f, RR = 0.f, S3 = 0.f, SR2 = 0.f;
+  float S2[3][3], R2[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      SS += S[i][j] * S[i][j];
+      RR += R[i][j] * R[i][j];
+      float a = 0.f, b = 0.f;
+      for (int k = 0; k < 3; ++k) { a += S[i][k] * S[k][j]; b += R[i][k] * R[k][j]; }
+      S2[i][j] = a;
+      R2[i][j] = b;
+    }
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) { S3 += S2[i][j] * S[j][i]; SR2 += S[i][j] * R2[j][i]; }
+  const float mag = sqrtf(2.f * SS);
+  out[0] = mag;
+  out[1] = RR / (SS + 1e-20f);
+  out[2] = S3 / (mag * mag * mag + 1e-30f);
+  out[3] = SR2 / (mag * SS + 1e-30f);
+}
+
+__global__ void k_features(FeatArgs a) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long n = a.np * (a.nz + 1);
+  long p = t;
+  if (a.raw) {
+    if (t >= a.nsel) return;
+    p = t * a.stride;
+  }
+  if (p >= n) return;
+  const long c = a.np + p;
+  const int k = 1 + (int)(p / a.np);
+  float inv[3][4];
+  for (int sc = 0; sc < 3; ++sc) {
+    float s[6], r[3];
+    for (int i = 0; i < 6; ++i) s[i] = a.s[sc][i][c];
+    for (int i = 0; i < 3; ++i) r[i] = a.r[sc][i][c];
+    invariants(s, r, inv[sc]);
+  }
+  const float z = (k == 1) ? 0.5f * a.dz : (float)(k - 1) * a.dz;
+  float f[12];
+  f[0] = logf(inv[0][0] + 1e-12f);
+  f[1] = inv[0][1]; f[2] = inv[0][2]; f[3] = inv[0][3];
+  f[4] = logf((inv[1][0] + 1e-12f) / (inv[0][0] + 1e-12f));
+  f[5] = inv[1][1]; f[6] = inv[1][2]; f[7] = inv[1][3];
+  f[8] = logf((inv[2][0] + 1e-12f) / (inv[0][0] + 1e-12f));
+  f[9] = inv[2][1]; f[10] = inv[2][2];
+  f[11] = logf(z / a.delta);
+  if (a.raw) {
+    for (int i = 0; i < 12; ++i) a.raw[t * 12 + i] = f[i];
+    if (a.tgt) {
+      const double cs = a.elem_bytes == 8 ? ((const double*)a.cs)[c] : (double)((const float*)a.cs)[c];
+      const double be = a.elem_bytes == 8 ? ((const double*)a.beta)[c] : (double)((const float*)a.beta)[c];
+      a.tgt[t * 2] = (float)cs;
+      a.tgt[t * 2 + 1] = (float)be;
+    }
+    return;
+  }
+  bool out = false;
+  for (int i = 0; i < 12; ++i) {
+    const float x = (f[i] - a.xmean[i]) / a.xstd[i];
+    if (!(fabsf(x) <= a.xlim)) out = true;
+    a.X[p * 16 + i] = f2bf(x);
+  }
+  a.X[p * 16 + 12] = 0;
+  a.X[p * 16 + 13] = 0;
+  a.X[p * 16 + 14] = 0;
+  a.X[p * 16 + 15] = out ? f2bf(1.0f) : (short)0;
+}
+
+// raw samples -> normalised bf16 features [n][16] and targets [n][16]
+__global__ void k_prepare(const float* raw, const float* tgt, const float* xmean, const float* xstd, float ym0, float ys0,
+                          float ym1, float ys1, short* X, float* Y, long n) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  for (int i = 0; i < 12; ++i) X[p * 16 + i] = f2bf((raw[p * 12 + i] - xmean[i]) / xstd[i]);
+  for (int i = 12; i < 16; ++i) X[p * 16 + i] = 0;
+  for (int i = 0; i < 16; ++i) Y[p * 16 + i] = 0.f;
+  Y[p * 16] = (tgt[p * 2] - ym0) / ys0;
+  Y[p * 16 + 1] = (tgt[p * 2 + 1] - ym1) / ys1;
+}
+
+// sum of squared differences and of squares over levels 1..nz+1, per block
+struct DiffArgs {
+  const void* a;
+  const void* b;
+  int ea, eb;
+  long off, n;
+  float* part;  // [2 * nblocks]
+};
+
+__global__ void k_diff_norm(DiffArgs d) {
+  __shared__ float r1[256], r2[256];
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  float x = 0.f, y = 0.f;
+  if (t < d.n) {
+    const double va = d.ea == 8 ? ((const double*)d.a)[d.off + t] : (double)((const float*)d.a)[d.off + t];
+    const double vb = d.eb == 8 ? ((const double*)d.b)[d.off + t] : (double)((const float*)d.b)[d.off + t];
+    x = (float)((va - vb) * (va - vb));
+    y = (float)(vb * vb);
+  }
+  r1[threadIdx.x] = x;
+  r2[threadIdx.x] = y;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) { r1[threadIdx.x] += r1[threadIdx.x + w]; r2[threadIdx.x] += r2[threadIdx.x + w]; }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { d.part[2 * blockIdx.x] = r1[0]; d.part[2 * blockIdx.x + 1] = r2[0]; }
+}

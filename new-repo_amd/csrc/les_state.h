@@ -57,11 +57,9 @@ struct Les {
   int* nanflag;
   int alloc_failed;
 
-  // learned SGS
-  float* mlp_w;           // packed weights (device)
-  long mlp_nw;
-  int mlp_ready;
-  double judge[4];
+  // learned SGS surrogate (allocated when weights or samples are first used)
+  struct MlpState* mlp;
+  double judge[4];        // rel. L2 of Cs^2 and beta vs the plain procedure, -, step
 
   // timing
   int timing;

@@ -1,0 +1,182 @@
+"""Train the learned SGS surrogate on the GPU.
+
+1. Run the mixed-precision LES with the plain Lagrangian scale-dependent
+   dynamic procedure and, right after chosen update steps, collect samples
+   (12 invariant features, target Cs^2 and beta) with les_sgs_samples.
+2. Normalise features and targets, then train the 12-64-64-64-2 GELU network
+   with the library's own BF16-MFMA forward/backward kernels and Adam
+   (les_mlp_train_step); torch only provides device buffers and shuffling.
+3. Save the compact weight vector (les_mlp_layout) as .npy.
+
+Usage: python new-repo_amd/train_sgs.py --grid 256,256,256 --updates 3 --out weights/sgs_mlp.npy
+"""
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import pathlib
+import sys
+import time
+
+import numpy as np
+
+PKG_DIR = pathlib.Path(__file__).resolve().parent
+N_IN, N_OUT = 12, 2
+
+
+def _pkg():
+    name = "new_repo_amd"
+    if name in sys.modules:
+        return sys.modules[name]
+    spec = importlib.util.spec_from_file_location(name, PKG_DIR / "__init__.py", submodule_search_locations=[str(PKG_DIR)])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def init_weights(seed: int = 0) -> np.ndarray:
+    """He-initialised compact weight vector without the normalisers."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for shape in [(64, 12), (64,), (64, 64), (64,), (64, 64), (64,), (2, 64), (2,)]:
+        if len(shape) == 2:
+            parts.append(rng.standard_normal(shape) * np.sqrt(2.0 / shape[1]))
+        else:
+            parts.append(np.zeros(shape))
+    return np.concatenate([p.ravel() for p in parts]).astype(np.float32)
+
+
+def normalisers(x, y, xlim: float = 6.0) -> np.ndarray:
+    """means/scales of the raw features and targets (robust to outliers)."""
+    import torch
+    xm = x.mean(0)
+    xs = x.std(0).clamp_min(1e-6)
+    ym = y.mean(0)
+    ys = y.std(0).clamp_min(1e-6)
+    return np.concatenate([xm.cpu().numpy(), xs.cpu().numpy(), ym.cpu().numpy(), ys.cpu().numpy(),
+                           np.array([xlim])]).astype(np.float32)
+
+
+def collect_samples(g, prm, n_updates: int, warm_cycles: int, stride: int, max_per_update: int):
+    """Step the LES and gather samples after n_updates dynamic updates."""
+    import torch
+    npts = prm.nx * prm.ny * (prm.nz + 1)
+    per = min(max_per_update, (npts + stride - 1) // stride)
+    X = torch.empty((n_updates * per, N_IN), dtype=torch.float32, device="cuda")
+    Y = torch.empty((n_updates * per, N_OUT), dtype=torch.float32, device="cuda")
+    g.step(prm.cs_count * warm_cycles)
+    filled = 0
+    for u in range(n_updates):
+        g.step(prm.cs_count)
+        g.sync()
+        torch.cuda.synchronize()
+        n = g.sgs_samples(X[filled:].data_ptr(), Y[filled:].data_ptr(), per, stride)
+        filled += n
+    return X[:filled], Y[:filled]
+
+
+def train(g, X, Y, epochs: int = 3, batch: int = 65536, lr: float = 2e-3, seed: int = 0, xlim: float = 6.0,
+          log=print):
+    """Train on device samples; returns the compact weight vector and history."""
+    import torch
+    n = X.shape[0] - X.shape[0] % 16
+    X, Y = X[:n], Y[:n]
+    w = np.concatenate([init_weights(seed), normalisers(X, Y, xlim)])
+    g.mlp_set_weights(w)
+    xb = torch.empty((n, 16), dtype=torch.int16, device="cuda")
+    yn = torch.empty((n, 16), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    g.mlp_prepare(X.data_ptr(), Y.data_ptr(), n, xb.data_ptr(), yn.data_ptr())
+    batch = max(16, min(batch - batch % 16, n))
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    hist = []
+    steps_per_epoch = n // batch
+    total = epochs * steps_per_epoch
+    t0 = time.time()
+    step = 0
+    for ep in range(epochs):
+        perm = torch.randperm(n, device="cuda", generator=gen)
+        for b in range(steps_per_epoch):
+            idx = perm[b * batch:(b + 1) * batch]
+            bx = xb.index_select(0, idx)
+            by = yn.index_select(0, idx)
+            torch.cuda.synchronize()
+            # cosine-decayed learning rate
+            lr_t = lr * 0.5 * (1 + np.cos(np.pi * step / max(1, total)))
+            loss = g.mlp_train_step(bx.data_ptr(), by.data_ptr(), batch, lr_t)
+            step += 1
+            if b % max(1, steps_per_epoch // 4) == 0:
+                hist.append((ep, b, loss))
+                log(f"  epoch {ep} batch {b}/{steps_per_epoch} loss {loss:.5f} ({time.time() - t0:.1f}s)")
+    return g.mlp_get_weights(), hist
+
+
+def evaluate(g, X, Y):
+    """Relative L2 of Cs^2 and beta: surrogate vs targets, and the constant
+    (mean) predictor that the dynamic procedure would fall back to without
+    the network."""
+    import torch
+    n = X.shape[0] - X.shape[0] % 16
+    X, Y = X[:n], Y[:n]
+    xb = torch.empty((n, 16), dtype=torch.int16, device="cuda")
+    yn = torch.empty((n, 16), dtype=torch.float32, device="cuda")
+    out = torch.empty((n, 2), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    g.mlp_prepare(X.data_ptr(), Y.data_ptr(), n, xb.data_ptr(), yn.data_ptr())
+    g.mlp_forward(xb.data_ptr(), n, out.data_ptr())
+    w = g.mlp_get_weights()
+    ym, ys = torch.tensor(w[-5:-3], device="cuda"), torch.tensor(w[-3:-1], device="cuda")
+    pred = out * ys + ym
+    pred[:, 0] = pred[:, 0].clamp_min(1e-24)
+    pred[:, 1] = pred[:, 1].clamp_min(0.125)
+    res = {}
+    for i, name in enumerate(("cs2", "beta")):
+        t = Y[:, i]
+        res[name] = float(torch.linalg.norm(pred[:, i] - t) / torch.linalg.norm(t))
+        res[name + "_mean_predictor"] = float(torch.linalg.norm(t.mean() - t) / torch.linalg.norm(t))
+    return res
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--grid", default="256,256,256")
+    ap.add_argument("--updates", type=int, default=3)
+    ap.add_argument("--warm-cycles", type=int, default=4)
+    ap.add_argument("--stride", type=int, default=1)
+    ap.add_argument("--max-per-update", type=int, default=20_000_000)
+    ap.add_argument("--epochs", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=65536)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--seed", type=int, default=1234)
+    ap.add_argument("--out", default=str(PKG_DIR / "weights" / "sgs_mlp.npy"))
+    args = ap.parse_args(argv)
+    pkg = _pkg()
+    P = pkg.load_params_module()
+    ic = pkg.load_ic_module()
+    nx, ny, nz = (int(v) for v in args.grid.split(","))
+    prm = P.LesParams(nx=nx, ny=ny, nz=nz, precision=P.PREC_MIXED)
+    u, v, w = ic.log_law_ic(nx, ny, nz, prm.lz, prm.z0, 0.10, seed=args.seed)
+    t0 = time.time()
+    with pkg.LesGpu(prm) as g:
+        g.set_ic(u, v, w)
+        X, Y = collect_samples(g, prm, args.updates + 1, args.warm_cycles, args.stride, args.max_per_update)
+        print(f"samples: {X.shape[0]} from {args.updates + 1} dynamic updates ({time.time() - t0:.1f}s)", flush=True)
+        per = X.shape[0] // (args.updates + 1)
+        Xtr, Ytr, Xte, Yte = X[:per * args.updates], Y[:per * args.updates], X[per * args.updates:], Y[per * args.updates:]
+        weights, hist = train(g, Xtr, Ytr, args.epochs, args.batch, args.lr, args.seed)
+        ev = evaluate(g, Xte, Yte)
+    out = pathlib.Path(args.out)
+    out.parent.mkdir(parents=True, exist_ok=True)
+    np.save(out, weights.astype(np.float32))
+    meta = {"grid": [nx, ny, nz], "train_samples": int(Xtr.shape[0]), "test_samples": int(Xte.shape[0]),
+            "epochs": args.epochs, "batch": args.batch, "lr": args.lr, "held_out": ev,
+            "final_loss": hist[-1][2] if hist else None, "seconds": time.time() - t0}
+    out.with_suffix(".json").write_text(json.dumps(meta, indent=1))
+    print(json.dumps(meta))
+
+
+if __name__ == "__main__":
+    main()
