@@ -40,6 +40,8 @@ typedef struct les_params {
   int precision;   /* 0 FP64, 1 mixed (FP32 flow + FP64 pressure/averages/AB2), 2 mixed + learned SGS */
   int judge_every; /* learned SGS: steps between plain-LASD judge updates */
   double lx, ly, lz, dt, z0, pfx, pfy, cs0, coriol, ug, vg, ubc_u, ubc_v, ubc_w;
+  double learned_zmin; /* learned SGS: levels with z / delta < learned_zmin (the
+                          near-wall band) keep the plain procedure on every update */
 } les_params;
 
 /* Field ids (system_variables, compute_sgs, compute_LagScaleDep globals). */
@@ -143,12 +145,14 @@ int les_fft2d_c2r(const void* d_in, void* d_out, int nplanes);
 /* A per-point MLP 12 -> 64 -> 64 -> 64 -> 2 (GELU, BF16 MFMA, FP32
  * accumulation) predicting the Lagrangian dynamic model's Cs^2 and beta from
  * strain/rotation invariants at delta, 2 delta, 4 delta and log(z/delta).
- * With precision = 2 the update steps of the dynamic procedure use it; every
- * judge_every steps the plain procedure runs instead and judges it.
+ * With precision = 2 the update steps of the dynamic procedure use it above
+ * the near-wall band (z / delta >= learned_zmin, the training range); the band
+ * keeps the plain procedure on every update, and every judge_every steps the
+ * plain procedure runs on the whole column and judges the surrogate.
  *
  * Training samples at the current state (call right after a step with a
- * plain dynamic update, mixed-precision modes): every stride-th point of
- * levels 1..nz+1, at most max_samples, raw features to device d_x
+ * plain dynamic update, mixed-precision modes): every stride-th point of the
+ * levels above the near-wall band, at most max_samples, raw features to device d_x
  * (12 float32 per sample) and targets (Cs^2, beta) to device d_y. */
 int les_sgs_samples(void* d_x, void* d_y, long max_samples, long* n_written, int stride);
 /* Number of float32 values of the compact weight layout: W1[64][12] b1[64]
@@ -170,8 +174,10 @@ int les_mlp_train_step(const void* d_x, const void* d_y, long n, double lr, doub
 /* Fused inference on normalised features (device bf16 [n][16]) ->
  * normalised outputs (device float32 [n][2]). */
 int les_mlp_forward(const void* d_x, long n, void* d_out);
-/* Judge statistics of the last judge step: [0] rel. L2 of Cs^2 (surrogate vs
- * plain procedure), [1] rel. L2 of beta, [3] the step. */
+/* Judge statistics of the last judge step over the levels above the
+ * near-wall band: [0] rel. L2 of Cs^2 (surrogate vs plain procedure), [1]
+ * rel. L2 of beta (both compared up to 8), [2] number of band levels,
+ * [3] the step. */
 int les_mlp_judge_stats(double* out);
 
 #ifdef __cplusplus

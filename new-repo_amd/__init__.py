@@ -43,7 +43,7 @@ class _Params(ctypes.Structure):
                  "ubc_u_neu", "ubc_v_neu", "halo_mode", "coriolis", "precision", "judge_every")] + \
                [(n, ctypes.c_double) for n in
                 ("lx", "ly", "lz", "dt", "z0", "pfx", "pfy", "cs0", "coriol", "ug", "vg",
-                 "ubc_u", "ubc_v", "ubc_w")]
+                 "ubc_u", "ubc_v", "ubc_w", "learned_zmin")]
 
 
 _lib = None
@@ -55,6 +55,14 @@ def library() -> ctypes.CDLL:
     if _lib is None:
         if not LIB_PATH.exists():
             raise RuntimeError(f"{LIB_PATH} is missing: build it with new-repo_amd/build.py")
+        # torch ships its own HIP/HSA runtime under the same sonames as
+        # /opt/rocm's.  Loaded first, it satisfies this library's dependency,
+        # so one runtime serves both and device pointers can be exchanged;
+        # loaded after, a second HSA runtime cannot open the GPU.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = ctypes.CDLL(str(LIB_PATH))
         vp, dp, ip, lp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int), \
             ctypes.POINTER(ctypes.c_long)

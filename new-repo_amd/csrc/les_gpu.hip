@@ -294,7 +294,7 @@ static SgsArgs<T> sgs_args(Les& L) {
 }
 
 template <typename T>
-static void lasd_update(Les& L, int jtt);
+static void lasd_update(Les& L, int jtt, int nlev);
 template <typename T>
 static void learned_update(Les& L, int jtt);
 
@@ -321,9 +321,9 @@ static void sgs_stress(Les& L, int jtt) {
     kt_end(L, KT_POINTWISE, (double)L.np * (L.nz + 1) * (16.0 * sizeof(T) + (a.accumulate ? 3 * 8.0 + 6.0 * sizeof(T) : 0.0)));
     if constexpr (sizeof(T) == 4) {
       if (L.P.precision == 2) learned_update<T>(L, jtt);
-      else lasd_update<T>(L, jtt);
+      else lasd_update<T>(L, jtt, L.nz + 1);
     } else {
-      lasd_update<T>(L, jtt);
+      lasd_update<T>(L, jtt, L.nz + 1);
     }
     a.accumulate = 0;
   } else {
@@ -513,6 +513,13 @@ int les_init(const les_params* p) {
   L->nbp = (long)L->nx2 * L->ny2;
   L->dx = P.lx / (double)P.nx; L->dy = P.ly / (double)P.ny; L->dz = P.lz / (double)P.nz;
   L->delta = std::pow(L->dx * L->dy * L->dz, 1.0 / 3.0);
+  // near-wall band of the learned mode: SGS level k sits at z = dz/2 (k = 1)
+  // or (k-1) dz, the height the surrogate's log(z/delta) feature uses
+  L->nband = 0;
+  for (int k = 1; k <= P.nz + 1; ++k) {
+    const double z = k == 1 ? 0.5 * L->dz : (double)(k - 1) * L->dz;
+    if (z / L->delta < P.learned_zmin) L->nband = k;
+  }
   L->jt_total = 0;
   L->F_coeff_init = P.lagran_init != 0;
   L->mixed = P.precision != 0;

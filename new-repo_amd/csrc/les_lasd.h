@@ -48,6 +48,7 @@ struct LagrArgs {
   T *cs, *beta;
   double dt, csc, dx, dy, dz, opftdelta, lagran_dt;
   int nx, ny, nz;
+  int nlev;  // levels 1..nlev are updated
   long np;
   int interp;  // 0 on the first update (F initialised from MM, NN)
   int halo_mode;
@@ -56,7 +57,7 @@ struct LagrArgs {
 template <typename T>
 __global__ void k_lagr(LagrArgs<T> a) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= a.np * (a.nz + 1)) return;
+  if (t >= a.np * a.nlev) return;
   const int k = 1 + (int)(t / a.np);
   const long q = t % a.np;
   const int i = 1 + (int)(q % a.nx), j = 1 + (int)(q / a.nx);
@@ -136,10 +137,16 @@ __global__ void k_lagr(LagrArgs<T> a) {
 }
 
 // compute_LagScaleDep_coeff on an update step (src/compute_LagScaleDep.f90:237-292)
+// on levels 1..nlev (nlev = nz + 1: the whole column).  The procedure is
+// local in z apart from the Lagrangian gather, which reads F_* one level
+// above: with nlev < nz + 1 the levels above keep their F_* (copied into the
+// new buffer), so level nlev gathers from the last full update.  The physical
+// nz still decides the boundary semantics (halo planes, beta = 1 at nz + 1).
 template <typename T>
-static void lasd_update(Les& L, int jtt) {
+static void lasd_update(Les& L, int jtt, int nlev) {
   (void)jtt;
-  const long npt = L.np * (L.nz + 1);
+  if (nlev > L.nz + 1) nlev = L.nz + 1;
+  const long npt = L.np * nlev;
   // 15 materialised inputs + the 6 S_ij fields
   T* in[21];
   for (int i = 0; i < 9; ++i) in[i] = (T*)L.wk_real[i];
@@ -152,7 +159,7 @@ static void lasd_update(Les& L, int jtt) {
   ia.u = D(L, LES_U); ia.v = D(L, LES_V); ia.w = D(L, LES_W);
   for (int i = 0; i < 6; ++i) ia.s[i] = in[9 + i];
   ia.smod = (T*)L.wk_real[63];
-  ia.np = L.np; ia.nz = L.nz;
+  ia.np = L.np; ia.nz = L.nz; ia.nlev = nlev;
   kt_begin(L, KT_POINTWISE);
   hipLaunchKernelGGL((k_lasd_inputs<T>), dim3(nblk(npt)), dim3(LES_BLOCK), 0, L.stream, ia);
   kt_end(L, KT_POINTWISE, (double)npt * (3 * 8.0 + 22.0 * sizeof(T)));
@@ -168,11 +175,11 @@ static void lasd_update(Les& L, int jtt) {
     const T* src[3];
     Cplx<T>* sp[3];
     for (int t = 0; t < 3; ++t) { src[t] = in[f0 + t]; sp[t] = (Cplx<T>*)L.wk_spec[t]; }
-    rows_r2c<T, T>(L, src, sp, 3, 1, L.nz + 1, (T)1);
+    rows_r2c<T, T>(L, src, sp, 3, 1, nlev, (T)1);
     for (int t = 0; t < 3; ++t) {
       Cplx<T>* o2 = (Cplx<T>*)L.wk_spec[3 + 2 * t];
       Cplx<T>* o4 = (Cplx<T>*)L.wk_spec[4 + 2 * t];
-      cols<T>(L, COP_FILT2, sp[t], nullptr, o2, o4, nullptr, 2, 1, 1, 1, L.nz + 1, g2, g4);
+      cols<T>(L, COP_FILT2, sp[t], nullptr, o2, o4, nullptr, 2, 1, 1, 1, nlev, g2, g4);
     }
     const Cplx<T>* cs[6];
     T* outs[6];
@@ -182,7 +189,7 @@ static void lasd_update(Les& L, int jtt) {
       outs[2 * t] = fl2[f0 + t];
       outs[2 * t + 1] = fl4[f0 + t];
     }
-    rows_c2r<T, T>(L, cs, outs, 6, 1, L.nz + 1);
+    rows_c2r<T, T>(L, cs, outs, 6, 1, nlev);
   }
   T* LM = (T*)L.wk_real[57];
   T* MM = (T*)L.wk_real[58];
@@ -195,7 +202,7 @@ static void lasd_update(Les& L, int jtt) {
     ga.BB = s == 0 ? MM : NN;
     ga.tf2 = (T)(s == 0 ? 4.0 : 16.0);
     ga.cst = (T)(2.0 * (L.delta * L.delta));
-    ga.np = L.np; ga.nz = L.nz;
+    ga.np = L.np; ga.nz = L.nz; ga.nlev = nlev;
     kt_begin(L, KT_POINTWISE);
     hipLaunchKernelGGL((k_germano<T>), dim3(nblk(npt)), dim3(LES_BLOCK), 0, L.stream, ga);
     kt_end(L, KT_POINTWISE, (double)npt * 23.0 * sizeof(T));
@@ -206,6 +213,12 @@ static void lasd_update(Les& L, int jtt) {
     L.f[LES_F_LM + f] = (void*)L.Fold[f];
     L.Fold[f] = (double*)tmp;
   }
+  if (nlev < L.nz + 1) {
+    const size_t off = (size_t)(nlev + 1) * L.np;
+    const size_t cnt = (size_t)(L.nz + 1 - nlev) * L.np;
+    for (int f = 0; f < 4; ++f)
+      hipMemcpyAsync(D(L, LES_F_LM + f) + off, L.Fold[f] + off, 8 * cnt, hipMemcpyDeviceToDevice, L.stream);
+  }
   LagrArgs<T> la;
   la.ulag = F<T>(L, LES_U_LAG); la.vlag = F<T>(L, LES_V_LAG); la.wlag = F<T>(L, LES_W_LAG);
   for (int f = 0; f < 4; ++f) { la.Fo[f] = L.Fold[f]; la.Fn[f] = D(L, LES_F_LM + f); }
@@ -215,7 +228,7 @@ static void lasd_update(Les& L, int jtt) {
   la.dx = L.dx; la.dy = L.dy; la.dz = L.dz;
   la.opftdelta = 1.5 * L.delta;
   la.lagran_dt = L.P.dt * (double)L.P.cs_count;
-  la.nx = L.nx; la.ny = L.ny; la.nz = L.nz; la.np = L.np;
+  la.nx = L.nx; la.ny = L.ny; la.nz = L.nz; la.nlev = nlev; la.np = L.np;
   la.interp = L.F_coeff_init;
   la.halo_mode = L.P.halo_mode;
   kt_begin(L, KT_LAGR);

@@ -10,7 +10,7 @@ struct MlpState {
   float* norm;     // xmean[12] xstd[12]
   float hn[29];    // host: xmean[12] xstd[12] ymean[2] ystd[2] xlim
   short* X;        // step features [npts][16]
-  float* red;      // reduction partials
+  double* red;     // reduction partials
   long redn;
   int step;
   int ready;
@@ -35,7 +35,7 @@ static int mlp_alloc(Les& L) {
   const long npt = L.np * (L.nz + 1);
   M->X = (short*)dalloc(L, 2 * 16 * (size_t)npt);
   M->redn = 2 * (long)nblk(npt) + 64;
-  M->red = (float*)dalloc(L, 4 * (size_t)M->redn);
+  M->red = (double*)dalloc(L, 8 * (size_t)M->redn);
   L.mlp = M;
   if (L.alloc_failed) { les_set_error("out of memory (MLP)"); return 3; }
   return 0;
@@ -123,12 +123,16 @@ static int mlp_forward_backward(Les& L, const short* X, const float* Y, long n, 
   return 0;
 }
 
-// features + inference for the current gradients (S_ij already written);
-// cs/beta targets of elem size eb
+// features + inference for the current gradients (S_ij already written) on
+// levels k0..nz+1; cs/beta targets of elem size eb
 template <typename T>
-static void learned_coeffs(Les& L, void* cs, void* beta, int eb) {
+static void learned_coeffs(Les& L, void* cs, void* beta, int eb, int k0) {
   MlpState* M = L.mlp;
+  if (k0 > L.nz + 1) return;
   const long npt = L.np * (L.nz + 1);
+  const int nlev = L.nz + 2 - k0;
+  const long p0 = (long)(k0 - 1) * L.np;
+  const long nsel = (long)nlev * L.np;
   SgsArgs<T> a = sgs_args<T>(L);
   T* rr[3] = {(T*)L.wk_real[0], (T*)L.wk_real[1], (T*)L.wk_real[2]};
   hipLaunchKernelGGL((k_rij<T>), dim3(nblk(npt)), dim3(LES_BLOCK), 0, L.stream, a, rr[0], rr[1], rr[2]);
@@ -143,10 +147,10 @@ static void learned_coeffs(Les& L, void* cs, void* beta, int eb) {
     const T* src[3];
     Cplx<T>* sp[3];
     for (int t = 0; t < 3; ++t) { src[t] = in9[f0 + t]; sp[t] = (Cplx<T>*)L.wk_spec[t]; }
-    rows_r2c<T, T>(L, src, sp, 3, 1, L.nz + 1, (T)1);
+    rows_r2c<T, T>(L, src, sp, 3, k0, nlev, (T)1);
     for (int t = 0; t < 3; ++t)
       cols<T>(L, COP_FILT2, sp[t], nullptr, (Cplx<T>*)L.wk_spec[3 + 2 * t], (Cplx<T>*)L.wk_spec[4 + 2 * t], nullptr, 2, 1,
-              1, 1, L.nz + 1, g2, g4);
+              1, k0, nlev, g2, g4);
     const Cplx<T>* cs6[6];
     T* outs[6];
     for (int t = 0; t < 3; ++t) {
@@ -155,7 +159,7 @@ static void learned_coeffs(Les& L, void* cs, void* beta, int eb) {
       outs[2 * t] = f2[f0 + t];
       outs[2 * t + 1] = f4[f0 + t];
     }
-    rows_c2r<T, T>(L, cs6, outs, 6, 1, L.nz + 1);
+    rows_c2r<T, T>(L, cs6, outs, 6, k0, nlev);
   }
   FeatArgs fa;
   for (int i = 0; i < 6; ++i) { fa.s[0][i] = (const float*)in9[i]; fa.s[1][i] = (const float*)f2[i]; fa.s[2][i] = (const float*)f4[i]; }
@@ -163,64 +167,76 @@ static void learned_coeffs(Les& L, void* cs, void* beta, int eb) {
   fa.np = L.np; fa.nz = L.nz; fa.dz = (float)L.dz; fa.delta = (float)L.delta;
   fa.xmean = M->norm; fa.xstd = M->norm + 12; fa.xlim = M->hn[28];
   fa.X = M->X; fa.raw = nullptr; fa.tgt = nullptr; fa.cs = nullptr; fa.beta = nullptr; fa.elem_bytes = 4;
-  fa.stride = 1; fa.nsel = npt;
+  fa.stride = 1; fa.nsel = nsel; fa.p0 = p0;
   kt_begin(L, KT_MLP);
-  hipLaunchKernelGGL(k_features, dim3(nblk(npt)), dim3(LES_BLOCK), 0, L.stream, fa);
+  hipLaunchKernelGGL(k_features, dim3(nblk(nsel)), dim3(LES_BLOCK), 0, L.stream, fa);
   if (cs) {
     InferArgs ia;
     ia.X = M->X; ia.Wb = M->wb; ia.bias = M->w + MLP_NMAT;
     ia.ymean0 = M->hn[24]; ia.ymean1 = M->hn[25]; ia.ystd0 = M->hn[26]; ia.ystd1 = M->hn[27];
     ia.n = npt; ia.np = L.np; ia.nz = L.nz;
     ia.cs = cs; ia.beta = beta; ia.elem_bytes = eb;
-    ia.raw_out = nullptr; ia.write_fields = 1;
+    ia.raw_out = nullptr; ia.write_fields = 1; ia.p0 = p0;
     hipLaunchKernelGGL(k_mlp_infer, dim3(2048), dim3(256), 0, L.stream, ia);
   }
-  kt_end(L, KT_MLP, (double)npt * (27.0 * sizeof(T) + 2.0 * eb));
+  kt_end(L, KT_MLP, (double)nsel * (27.0 * sizeof(T) + 2.0 * eb));
 }
 
-static double reduce_rel(Les& L, const void* a, const void* b, int ea, int eb) {
+// relative L2 distance of a from b over levels k0..nz+1 (values above clip
+// compared as clip when clip > 0)
+static double reduce_rel(Les& L, const void* a, const void* b, int ea, int eb, int k0, double clip) {
   MlpState* M = L.mlp;
-  const long npt = L.np * (L.nz + 1);
+  if (k0 > L.nz + 1) return 0.0;
+  const long npt = L.np * (L.nz + 2 - k0);
   DiffArgs d;
-  d.a = a; d.b = b; d.ea = ea; d.eb = eb; d.off = L.np; d.n = npt; d.part = M->red;
+  d.a = a; d.b = b; d.ea = ea; d.eb = eb; d.off = (long)k0 * L.np; d.n = npt; d.clip = clip; d.part = M->red;
   const unsigned nb = nblk(npt);
   hipLaunchKernelGGL(k_diff_norm, dim3(nb), dim3(LES_BLOCK), 0, L.stream, d);
-  std::vector<float> h(2 * nb);
-  hipMemcpyAsync(h.data(), M->red, 8 * (size_t)nb, hipMemcpyDeviceToHost, L.stream);
+  std::vector<double> h(2 * nb);
+  hipMemcpyAsync(h.data(), M->red, 16 * (size_t)nb, hipMemcpyDeviceToHost, L.stream);
   hipStreamSynchronize(L.stream);
   double s1 = 0, s2 = 0;
   for (unsigned i = 0; i < nb; ++i) { s1 += h[2 * i]; s2 += h[2 * i + 1]; }
   return s2 > 0 ? std::sqrt(s1 / s2) : std::sqrt(s1);
 }
 
-// Coefficient update of the learned mode on a dynamic-procedure step:
-// judge steps (and every step until weights are loaded) run the plain
-// procedure; the surrogate is evaluated beside it for the judge statistics.
-// Other update steps use the surrogate; out-of-range points keep their
-// last coefficients.
+// Coefficient update of the learned mode on a dynamic-procedure step.
+// The first update, judge steps and every step until weights are loaded run
+// the plain procedure on the whole column; the surrogate is evaluated beside
+// it for the judge statistics (levels above the near-wall band, beta
+// compared up to 8, the bound of the training targets).  Other update steps
+// run the plain procedure on the near-wall band 1..nband, where z / delta is
+// below the surrogate's training range, and the surrogate above it;
+// out-of-range points keep their last coefficients.
 template <typename T>
 static void learned_update(Les& L, int jtt) {
-  const bool judge = !(L.mlp && L.mlp->ready) || (L.P.judge_every > 0 && jtt % L.P.judge_every == 0);
+  const bool ready = L.mlp && L.mlp->ready;
+  const bool judge = !ready || !L.F_coeff_init || (L.P.judge_every > 0 && jtt % L.P.judge_every == 0);
+  const int k0 = L.nband + 1;
   if (judge) {
-    lasd_update<T>(L, jtt);
-    if (L.mlp && L.mlp->ready) {
+    lasd_update<T>(L, jtt, L.nz + 1);
+    if (ready) {
       void* tcs = L.wk_real[57];
       void* tbe = L.wk_real[58];
       const size_t bytes = sizeof(T) * (size_t)L.nf;
       hipMemcpyAsync(tcs, L.f[LES_CS_OPT2], bytes, hipMemcpyDeviceToDevice, L.stream);
       hipMemcpyAsync(tbe, L.f[LES_BETA], bytes, hipMemcpyDeviceToDevice, L.stream);
-      learned_coeffs<T>(L, tcs, tbe, sizeof(T));
-      L.judge[0] = reduce_rel(L, tcs, L.f[LES_CS_OPT2], sizeof(T), sizeof(T));
-      L.judge[1] = reduce_rel(L, tbe, L.f[LES_BETA], sizeof(T), sizeof(T));
+      learned_coeffs<T>(L, tcs, tbe, sizeof(T), k0);
+      L.judge[0] = reduce_rel(L, tcs, L.f[LES_CS_OPT2], sizeof(T), sizeof(T), k0, 0.0);
+      L.judge[1] = reduce_rel(L, tbe, L.f[LES_BETA], sizeof(T), sizeof(T), k0, 8.0);
+      L.judge[2] = (double)L.nband;
       L.judge[3] = (double)jtt;
     }
     return;
   }
-  learned_coeffs<T>(L, L.f[LES_CS_OPT2], L.f[LES_BETA], sizeof(T));
-  const size_t tb = sizeof(T) * (size_t)L.nf;
-  hipMemsetAsync(L.f[LES_U_LAG], 0, tb, L.stream);
-  hipMemsetAsync(L.f[LES_V_LAG], 0, tb, L.stream);
-  hipMemsetAsync(L.f[LES_W_LAG], 0, tb, L.stream);
+  if (L.nband > 0) lasd_update<T>(L, jtt, L.nband);  // also clears the u_lag sums
+  learned_coeffs<T>(L, L.f[LES_CS_OPT2], L.f[LES_BETA], sizeof(T), k0);
+  if (L.nband == 0) {
+    const size_t tb = sizeof(T) * (size_t)L.nf;
+    hipMemsetAsync(L.f[LES_U_LAG], 0, tb, L.stream);
+    hipMemsetAsync(L.f[LES_V_LAG], 0, tb, L.stream);
+    hipMemsetAsync(L.f[LES_W_LAG], 0, tb, L.stream);
+  }
 }
 
 // compact public weight layout <-> padded device layout
@@ -347,7 +363,7 @@ int les_mlp_forward(const void* d_x, long n, void* d_out) {
   ia.ymean0 = 0; ia.ymean1 = 0; ia.ystd0 = 1; ia.ystd1 = 1;
   ia.n = n; ia.np = L->np; ia.nz = L->nz;
   ia.cs = nullptr; ia.beta = nullptr; ia.elem_bytes = 4;
-  ia.raw_out = (float*)d_out; ia.write_fields = 0;
+  ia.raw_out = (float*)d_out; ia.write_fields = 0; ia.p0 = 0;
   hipLaunchKernelGGL(k_mlp_infer, dim3(256), dim3(256), 0, L->stream, ia);
   LES_CHECK(hipGetLastError());
   LES_CHECK(hipStreamSynchronize(L->stream));
@@ -363,8 +379,11 @@ int les_sgs_samples(void* d_x, void* d_y, long max_samples, long* n_written, int
   if (stride < 1) stride = 1;
   int rc = mlp_alloc(*L);
   if (rc) return rc;
+  // the training range: levels above the near-wall band
+  const int k0 = L->nband + 1;
   const long npt = L->np * (L->nz + 1);
-  long nsel = (npt + stride - 1) / stride;
+  const long p0 = (long)(k0 - 1) * L->np;
+  long nsel = k0 > L->nz + 1 ? 0 : (npt - p0 + stride - 1) / stride;
   if (nsel > max_samples) nsel = max_samples;
   SgsArgs<float> a = sgs_args<float>(*L);
   a.accumulate = 0;
@@ -373,7 +392,7 @@ int les_sgs_samples(void* d_x, void* d_y, long max_samples, long* n_written, int
   // filtered invariants, then raw features and the current coefficients
   MlpState* M = L->mlp;
   float save = M->hn[28];
-  learned_coeffs<float>(*L, nullptr, nullptr, 4);  // fills the filtered fields (feature output unused)
+  learned_coeffs<float>(*L, nullptr, nullptr, 4, k0);  // fills the filtered fields (feature output unused)
   M->hn[28] = save;
   FeatArgs fa;
   const float* rr[3] = {(const float*)L->wk_real[0], (const float*)L->wk_real[1], (const float*)L->wk_real[2]};
@@ -392,8 +411,8 @@ int les_sgs_samples(void* d_x, void* d_y, long max_samples, long* n_written, int
   fa.xmean = M->norm; fa.xstd = M->norm + 12; fa.xlim = 1e30f;
   fa.X = nullptr; fa.raw = (float*)d_x; fa.tgt = (float*)d_y;
   fa.cs = L->f[LES_CS_OPT2]; fa.beta = L->f[LES_BETA]; fa.elem_bytes = L->fbytes[LES_CS_OPT2];
-  fa.stride = stride; fa.nsel = nsel;
-  hipLaunchKernelGGL(k_features, dim3(nblk(nsel)), dim3(LES_BLOCK), 0, L->stream, fa);
+  fa.stride = stride; fa.nsel = nsel; fa.p0 = p0;
+  if (nsel > 0) hipLaunchKernelGGL(k_features, dim3(nblk(nsel)), dim3(LES_BLOCK), 0, L->stream, fa);
   LES_CHECK(hipGetLastError());
   LES_CHECK(hipStreamSynchronize(L->stream));
   if (n_written) *n_written = nsel;

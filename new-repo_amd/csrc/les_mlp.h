@@ -192,6 +192,7 @@ struct InferArgs {
   int elem_bytes;
   float* raw_out;      // optional: normalised outputs [n][2]
   int write_fields;
+  long p0;             // first point (multiple of 16): levels below are left alone
 };
 
 __device__ __forceinline__ void store_coef(const InferArgs& a, long p, double cs, double beta) {
@@ -231,7 +232,7 @@ __global__ void __launch_bounds__(256) k_mlp_infer(InferArgs a) {
     }
   for (int j = 0; j < 4; ++j) bb4[j] = a.bias[MB4 + q + j];
   const long ntiles = (a.n + 15) / 16;
-  for (long tile = wave; tile < ntiles; tile += nwaves) {
+  for (long tile = a.p0 / 16 + wave; tile < ntiles; tile += nwaves) {
     const long p = tile * 16 + r;
     s4v x = {0, 0, 0, 0};
     if (p < a.n)
@@ -321,6 +322,7 @@ struct FeatArgs {
   int elem_bytes;
   long stride;  // sample every stride-th point when raw != null
   long nsel;
+  long p0;      // first point (offset from level 1): levels below are skipped
 };
 
 __device__ __forceinline__ void invariants(const float* s, const float* rr, float* out) {
@@ -350,10 +352,10 @@ __device__ __forceinline__ void invariants(const float* s, const float* rr, floa
 __global__ void k_features(FeatArgs a) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long n = a.np * (a.nz + 1);
-  long p = t;
+  long p = a.p0 + t;
   if (a.raw) {
     if (t >= a.nsel) return;
-    p = t * a.stride;
+    p = a.p0 + t * a.stride;
   }
   if (p >= n) return;
   const long c = a.np + p;
@@ -408,24 +410,26 @@ __global__ void k_prepare(const float* raw, const float* tgt, const float* xmean
   Y[p * 16 + 1] = (tgt[p * 2 + 1] - ym1) / ys1;
 }
 
-// sum of squared differences and of squares over levels 1..nz+1, per block
+// sum of squared differences and of squares over a range of points, per block
 struct DiffArgs {
   const void* a;
   const void* b;
   int ea, eb;
   long off, n;
-  float* part;  // [2 * nblocks]
+  double clip;   // values above clip are compared as clip (0: no clipping)
+  double* part;  // [2 * nblocks]
 };
 
 __global__ void k_diff_norm(DiffArgs d) {
-  __shared__ float r1[256], r2[256];
+  __shared__ double r1[256], r2[256];
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  float x = 0.f, y = 0.f;
+  double x = 0.0, y = 0.0;
   if (t < d.n) {
-    const double va = d.ea == 8 ? ((const double*)d.a)[d.off + t] : (double)((const float*)d.a)[d.off + t];
-    const double vb = d.eb == 8 ? ((const double*)d.b)[d.off + t] : (double)((const float*)d.b)[d.off + t];
-    x = (float)((va - vb) * (va - vb));
-    y = (float)(vb * vb);
+    double va = d.ea == 8 ? ((const double*)d.a)[d.off + t] : (double)((const float*)d.a)[d.off + t];
+    double vb = d.eb == 8 ? ((const double*)d.b)[d.off + t] : (double)((const float*)d.b)[d.off + t];
+    if (d.clip > 0.0) { va = fmin(va, d.clip); vb = fmin(vb, d.clip); }
+    x = (va - vb) * (va - vb);
+    y = vb * vb;
   }
   r1[threadIdx.x] = x;
   r2[threadIdx.x] = y;

@@ -248,12 +248,13 @@ struct LasdInArgs {
   const T* smod;
   long np;
   int nz;
+  int nlev;  // levels 1..nlev
 };
 
 template <typename T>
 __global__ void k_lasd_inputs(LasdInArgs<T> a) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= a.np * (a.nz + 1)) return;
+  if (t >= a.np * a.nlev) return;
   const int k = 1 + (int)(t / a.np);
   const long q = t % a.np;
   const long c = (long)k * a.np + q;
@@ -283,12 +284,13 @@ struct GermanoArgs {
   T tf2, cst;
   long np;
   int nz;
+  int nlev;  // levels 1..nlev
 };
 
 template <typename T>
 __global__ void k_germano(GermanoArgs<T> a) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= a.np * (a.nz + 1)) return;
+  if (t >= a.np * a.nlev) return;
   const long c = a.np + t;
   const T uf = a.fl[0][c], vf = a.fl[1][c], wf = a.fl[2][c];
   const T A11 = a.fl[3][c] - uf * uf, A12 = a.fl[4][c] - uf * vf, A13 = a.fl[5][c] - uf * wf;

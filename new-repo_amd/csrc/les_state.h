@@ -60,6 +60,7 @@ struct Les {
   // learned SGS surrogate (allocated when weights or samples are first used)
   struct MlpState* mlp;
   double judge[4];        // rel. L2 of Cs^2 and beta vs the plain procedure, -, step
+  int nband;              // levels 1..nband (z / delta < learned_zmin) keep the plain procedure
 
   // timing
   int timing;

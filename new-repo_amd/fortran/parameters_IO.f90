@@ -4,7 +4,7 @@
 !! Reads the reference's parameters.nlst (same six namelist groups and
 !! variable names, src/parameters_IO.f90:151-166) and nondimensionalises it
 !! the same way (src/parameters_IO.f90:240-298).  An optional seventh group
-!! &les_gpu_param (precision, halo_mode, judge_every) selects the device
+!! &les_gpu_param (precision, halo_mode, judge_every, learned_zmin) selects the device
 !! variant.  read_parameters fills les_params_c for les_init.
 !------------------------------------------------------------------------------
 module parameters_IO
@@ -45,6 +45,7 @@ module parameters_IO
   real(rprec), public :: dudz0 = 0._rprec, z_turb = 0.5_rprec, vnf = 1._rprec
   ! les_gpu_param (device variant)
   integer, public :: precision = 0, halo_mode = 0, judge_every = 50
+  real(rprec), public :: learned_zmin = 4._rprec
   ! derived
   real(rprec), public :: dx, dy, dz
   logical, public :: Lagran_init = .false.
@@ -65,7 +66,7 @@ contains
     namelist / sgs_param / Cs0, sgs_model, cs_count, dyn_init
     namelist / momentum_bc / lbc_model, lbc_special, ubc_model, ubc_special, wall_model, surface_files
     namelist / momentum_ic / ic_type, u0, v0, w0, u_star, dudz0, z_turb, vnf
-    namelist / les_gpu_param / precision, halo_mode, judge_every
+    namelist / les_gpu_param / precision, halo_mode, judge_every, learned_zmin
 
     open(88, file='parameters.nlst', status='OLD', action='READ', delim='APOSTROPHE', iostat=ios)
     if (ios /= 0) call abort22('Could not open parameters.nlst for reading')
@@ -110,6 +111,7 @@ contains
     end if
     cparams%ubc_u_neu = 1; cparams%ubc_v_neu = 1
     cparams%halo_mode = halo_mode; cparams%precision = precision; cparams%judge_every = judge_every
+    cparams%learned_zmin = learned_zmin
     cparams%coriolis = 0; cparams%coriol = 0._rprec; cparams%ug = 0._rprec; cparams%vg = 0._rprec
     if (coriolis_flag) then
        cparams%coriolis = 1

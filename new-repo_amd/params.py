@@ -56,6 +56,7 @@ class LesParams:
     vg: float = 0.0
     precision: int = PREC_FP64
     judge_every: int = 50       # learned SGS: plain LASD judge cadence
+    learned_zmin: float = 4.0   # learned SGS: plain LASD on levels with z / delta below this
 
     # ---- vectors shared with the CPU oracle (orc_init argument order)
     def int_vector(self) -> List[int]:

@@ -117,3 +117,47 @@ def test_learned_step_with_judge_beats_constant_coefficient():
         beta = g.download("beta")[1:prm.nz + 2]
         assert np.array_equal(cs.astype(np.float32).astype(np.float64), cs) and cs.min() >= np.float32(1e-24)
         assert beta.min() >= 0.125 and np.all(beta[-1] == 1.0)
+
+
+def _run_cs(prec, steps, weights=None, **kw):
+    pkg, P, ic = load_pkg(), load_params(), load_ic()
+    prm = P.LesParams(nx=32, ny=32, nz=16, precision=prec, judge_every=50, **kw)
+    u, v, w = ic.log_law_ic(prm.nx, prm.ny, prm.nz, prm.lz, prm.z0, 0.1, seed=11)
+    with pkg.LesGpu(prm) as g:
+        g.set_ic(u, v, w)
+        if weights is not None:
+            g.mlp_set_weights(weights)
+        g.step(steps)
+        return g.download("cs_opt2"), g.download("u"), g.judge_stats() if weights is not None else None
+
+
+def _some_weights():
+    return np.concatenate([M.init_params(7).astype(np.float32), _identity_norm()])
+
+
+def test_band_covering_every_level_is_the_plain_procedure():
+    # learned_zmin above the domain puts every level in the near-wall band:
+    # the learned mode must then reproduce the mixed mode bit for bit
+    P = load_params()
+    print("learned mode, band = whole column, vs mixed mode", flush=True)
+    cs_m, u_m, _ = _run_cs(P.PREC_MIXED, 20)
+    cs_l, u_l, js = _run_cs(P.PREC_LEARNED, 20, _some_weights(), learned_zmin=1e9)
+    assert np.array_equal(cs_m[1:], cs_l[1:]) and np.array_equal(u_m, u_l)
+
+
+def test_band_levels_follow_the_plain_procedure():
+    # the first update (step 5) is a full plain update in both modes, so the
+    # states agree up to the coefficient update of step 10; that band update
+    # must give the plain coefficients on levels 1..nband, the surrogate's
+    # above (the velocity after step 10 already differs through them)
+    P = load_params()
+    prm = P.LesParams(nx=32, ny=32, nz=16)
+    delta = (prm.lx / 32 * prm.ly / 32 * prm.lz / 16) ** (1 / 3)
+    z = np.array([0.5 * prm.lz / 16] + [(k - 1) * prm.lz / 16 for k in range(2, 18)])
+    nband = int(np.sum(z / delta < 4.0))
+    assert 0 < nband < 17
+    print(f"band of {nband} levels: learned vs mixed at step 10", flush=True)
+    cs_m, u_m, _ = _run_cs(P.PREC_MIXED, 10)
+    cs_l, u_l, _ = _run_cs(P.PREC_LEARNED, 10, _some_weights(), learned_zmin=4.0)
+    assert np.array_equal(cs_m[1:nband + 1], cs_l[1:nband + 1])
+    assert not np.array_equal(cs_m[nband + 1:18], cs_l[nband + 1:18])
