@@ -55,10 +55,40 @@ def load_pkg():
 
 
 WEIGHTS = ROOT / "new-repo_amd" / "weights" / "sgs_mlp.npy"
+# HBM traffic per launch of a kernel family, from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE passes scaled by a copy kernel of known size (written by
+# new-repo_amd/profile_summary.py)
+TRAFFIC = ROOT / "profiles" / "pmc_traffic.json"
 
 
 def load_weights():
     return np.load(WEIGHTS, allow_pickle=False) if WEIGHTS.exists() else None
+
+
+def copy_calibration(mib):
+    """A plain device copy of known size (y = x * 1 over `mib` MiB of float32,
+    three launches) that a counter pass sees beside the LES kernels: its
+    FETCH_SIZE / WRITE_SIZE per dispatch calibrate the counters."""
+    import torch
+    n = mib * (1 << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    print(f"# copy calibration: 3 x {mib} MiB read + {mib} MiB written", file=sys.stderr, flush=True)
+
+
+def load_traffic(variant, family, grid):
+    if not TRAFFIC.exists():
+        return None
+    try:
+        t = json.loads(TRAFFIC.read_text())
+    except ValueError:
+        return None
+    if t.get("variant") == variant and t.get("kernel_family") == family and t.get("grid") == list(grid):
+        return t
+    return None
 
 
 def run_variant(pkg, P, ic, precision, steps, warmup, nx, ny, nz, kernel_stats=True, seed=1234):
@@ -187,6 +217,9 @@ def main(argv=None):
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--profile-steps", type=int, default=500,
                     help="steps of the matched-error profile comparison (0 to skip)")
+    ap.add_argument("--no-kernel-stats", action="store_true", help="skip the per-launch HIP-event pass")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first launch a copy kernel of MIB MiB (counter calibration)")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): the whole domain lives on a single MI355X "
@@ -203,10 +236,13 @@ def main(argv=None):
             wanted.append("learned")
     else:
         wanted = args.variants.split(",")
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration)
     results = {}
     for name in wanted:
         try:
-            results[name] = run_variant(pkg, P, ic, order[name], args.steps, args.warmup, nx, ny, nz)
+            results[name] = run_variant(pkg, P, ic, order[name], args.steps, args.warmup, nx, ny, nz,
+                                        kernel_stats=not args.no_kernel_stats)
         except Exception as e:  # a variant that fails is reported, not hidden
             results[name] = {"error": str(e)}
         print(f"# {name}: {results[name].get('mpts', results[name].get('error'))}", file=sys.stderr, flush=True)
@@ -222,9 +258,12 @@ def main(argv=None):
     if dom:
         d = fam[dom]
         ach = d["bytes"] / d["calls"] / (d["avg_ms"] * 1e-3)
+        tr = load_traffic(head, dom, (nx, ny, nz))
         roof = {"bound": "hbm", "achieved": ach / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
-                "frac": ach / PEAK_HBM, "traffic": None, "kernel_family": dom,
-                "avg_launch_ms": d["avg_ms"], "bytes_per_launch": d["bytes"] / d["calls"]}
+                "frac": ach / PEAK_HBM, "traffic": tr["traffic_bytes_per_launch"] if tr else None,
+                "kernel_family": dom, "avg_launch_ms": d["avg_ms"], "bytes_per_launch": d["bytes"] / d["calls"]}
+        if tr:
+            roof["traffic_source"] = "profiles/pmc_traffic.json (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)"
     step_key = "fp64" if head == "fp64" else "mixed"
     step_bytes = STEP_BYTES[step_key] * nx * ny * nz
     out = {
