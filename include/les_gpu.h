@@ -12,6 +12,9 @@
  * buffers are synchronised only by download/status calls.  Not reentrant,
  * like the reference's module state.
  *
+ * Sizes: nx, ny multiples of 4 of the form 2^a 3^b with 3 nx / 2, 3 ny / 2 <= 1024
+ * (every transform is held by one wavefront); nz >= 3.
+ *
  * Layout of every 3D field: (nx, ny, nz+2) float64 on the host, Fortran
  * order (x fastest), z levels 0..nz+1 -- the reference's
  * dimension(nx, ny, 0:nz+1) arrays with their ghost planes.  Fields the

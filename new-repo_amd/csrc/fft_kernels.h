@@ -25,31 +25,26 @@ struct RowR2CArgs {
   Cplx<T>* dst[LES_MAX_FIELDS];
   int nx, ny, lhx;
   int k0, nplanes;  // planes [k0, k0+nplanes)
-  int lpb;          // row pairs per block
   int mode;
   T scale, div;
 };
 
+// One wavefront per row pair: load, transform and unpack without any
+// block-level barrier.  LDS: one line of nx points per wavefront.
 template <typename TI, typename T>
 __global__ void __launch_bounds__(LES_BLOCK) k_rows_r2c(RowR2CArgs<TI, T> a, FftPlan pl, const Cplx<T>* __restrict__ tw) {
   extern __shared__ unsigned char smem_raw[];
-  Cplx<T>* bufA = reinterpret_cast<Cplx<T>*>(smem_raw);
   const int nx = a.nx, lhx = a.lhx;
-  Cplx<T>* bufB = bufA + a.lpb * nx;
-  const int f = blockIdx.y;
-  const TI* __restrict__ src = a.src[f];
-  Cplx<T>* __restrict__ dst = a.dst[f];
+  const int lane = threadIdx.x & (LES_WAVE - 1), wave = threadIdx.x / LES_WAVE;
+  Cplx<T>* z = reinterpret_cast<Cplx<T>*>(smem_raw) + wave * nx;
   const int npairs = a.ny / 2;
-  const long job0 = (long)blockIdx.x * a.lpb;
-  const long njobs = (long)a.nplanes * npairs;
-  int nl = a.lpb;
-  if (job0 + nl > njobs) nl = (int)(njobs - job0);
-  for (int idx = threadIdx.x; idx < nl * nx; idx += blockDim.x) {
-    const int l = idx / nx, n = idx - l * nx;
-    const long job = job0 + l;
-    const int k = a.k0 + (int)(job / npairs), jp = (int)(job % npairs);
-    const long base = ((long)k * a.ny + 2 * jp) * nx + n;
-    T v0 = (T)src[base], v1 = (T)src[base + nx];
+  const long job = (long)blockIdx.x * (blockDim.x / LES_WAVE) + wave;
+  if (job >= (long)a.nplanes * npairs) return;
+  const int f = blockIdx.y;
+  const int k = a.k0 + (int)(job / npairs), jp = (int)(job % npairs);
+  const TI* __restrict__ src = a.src[f] + ((long)k * a.ny + 2 * jp) * nx;
+  for (int n = lane; n < nx; n += LES_WAVE) {
+    T v0 = (T)src[n], v1 = (T)src[n + nx];
     if (a.mode == LOAD_SCALE_DIV) {
       v0 = a.scale * (v0 / a.div);
       v1 = a.scale * (v1 / a.div);
@@ -57,20 +52,17 @@ __global__ void __launch_bounds__(LES_BLOCK) k_rows_r2c(RowR2CArgs<TI, T> a, Fft
       v0 = a.scale * v0;
       v1 = a.scale * v1;
     }
-    bufA[l * nx + n] = {v0, v1};
+    z[n] = {v0, v1};
   }
-  __syncthreads();
-  Cplx<T>* Z = lds_fft(bufA, bufB, nl, nx, pl, tw, false);
-  for (int idx = threadIdx.x; idx < nl * lhx; idx += blockDim.x) {
-    const int l = idx / lhx, kk = idx - l * lhx;
-    const long job = job0 + l;
-    const int k = a.k0 + (int)(job / npairs), jp = (int)(job % npairs);
-    const Cplx<T> zk = Z[l * nx + kk], zm = Z[l * nx + (kk == 0 ? 0 : nx - kk)];
+  wave_sync();
+  wave_fft(z, pl, tw, false, lane);
+  Cplx<T>* __restrict__ dst = a.dst[f] + ((long)k * a.ny + 2 * jp) * lhx;
+  for (int kk = lane; kk < lhx; kk += LES_WAVE) {
+    const Cplx<T> zk = z[kk], zm = z[kk == 0 ? 0 : nx - kk];
     const Cplx<T> A = {(T)0.5 * (zk.x + zm.x), (T)0.5 * (zk.y - zm.y)};
     const Cplx<T> B = {(T)0.5 * (zk.y + zm.y), (T)0.5 * (zm.x - zk.x)};
-    const long ob = ((long)k * a.ny + 2 * jp) * lhx + kk;
-    dst[ob] = A;
-    dst[ob + lhx] = B;
+    dst[kk] = A;
+    dst[kk + lhx] = B;
   }
 }
 
@@ -81,46 +73,36 @@ struct RowC2RArgs {
   TO* dst[LES_MAX_FIELDS];
   int nx, ny, lhx;
   int k0, nplanes;
-  int lpb;
 };
 
 template <typename T, typename TO>
 __global__ void __launch_bounds__(LES_BLOCK) k_rows_c2r(RowC2RArgs<T, TO> a, FftPlan pl, const Cplx<T>* __restrict__ tw) {
   extern __shared__ unsigned char smem_raw[];
-  Cplx<T>* bufA = reinterpret_cast<Cplx<T>*>(smem_raw);
   const int nx = a.nx, lhx = a.lhx, h = nx / 2;
-  Cplx<T>* bufB = bufA + a.lpb * nx;
-  const int f = blockIdx.y;
-  const Cplx<T>* __restrict__ src = a.src[f];
-  TO* __restrict__ dst = a.dst[f];
+  const int lane = threadIdx.x & (LES_WAVE - 1), wave = threadIdx.x / LES_WAVE;
+  Cplx<T>* z = reinterpret_cast<Cplx<T>*>(smem_raw) + wave * nx;
   const int npairs = a.ny / 2;
-  const long job0 = (long)blockIdx.x * a.lpb;
-  const long njobs = (long)a.nplanes * npairs;
-  int nl = a.lpb;
-  if (job0 + nl > njobs) nl = (int)(njobs - job0);
-  for (int idx = threadIdx.x; idx < nl * lhx; idx += blockDim.x) {
-    const int l = idx / lhx, kk = idx - l * lhx;
-    const long job = job0 + l;
-    const int k = a.k0 + (int)(job / npairs), jp = (int)(job % npairs);
-    const long ib = ((long)k * a.ny + 2 * jp) * lhx + kk;
-    Cplx<T> A = src[ib], B = src[ib + lhx];
+  const long job = (long)blockIdx.x * (blockDim.x / LES_WAVE) + wave;
+  if (job >= (long)a.nplanes * npairs) return;
+  const int f = blockIdx.y;
+  const int k = a.k0 + (int)(job / npairs), jp = (int)(job % npairs);
+  const Cplx<T>* __restrict__ src = a.src[f] + ((long)k * a.ny + 2 * jp) * lhx;
+  for (int kk = lane; kk < lhx; kk += LES_WAVE) {
+    Cplx<T> A = src[kk], B = src[kk + lhx];
     if (kk == 0 || kk == h) {
       A.y = (T)0;
       B.y = (T)0;
     }
-    bufA[l * nx + kk] = {A.x - B.y, A.y + B.x};
-    if (kk != 0 && kk != h) bufA[l * nx + nx - kk] = {A.x + B.y, B.x - A.y};
+    z[kk] = {A.x - B.y, A.y + B.x};
+    if (kk != 0 && kk != h) z[nx - kk] = {A.x + B.y, B.x - A.y};
   }
-  __syncthreads();
-  Cplx<T>* Z = lds_fft(bufA, bufB, nl, nx, pl, tw, true);
-  for (int idx = threadIdx.x; idx < nl * nx; idx += blockDim.x) {
-    const int l = idx / nx, n = idx - l * nx;
-    const long job = job0 + l;
-    const int k = a.k0 + (int)(job / npairs), jp = (int)(job % npairs);
-    const long ob = ((long)k * a.ny + 2 * jp) * nx + n;
-    const Cplx<T> z = Z[l * nx + n];
-    dst[ob] = (TO)z.x;
-    dst[ob + nx] = (TO)z.y;
+  wave_sync();
+  wave_fft(z, pl, tw, true, lane);
+  TO* __restrict__ dst = a.dst[f] + ((long)k * a.ny + 2 * jp) * nx;
+  for (int n = lane; n < nx; n += LES_WAVE) {
+    const Cplx<T> v = z[n];
+    dst[n] = (TO)v.x;
+    dst[n + nx] = (TO)v.y;
   }
 }
 
@@ -138,7 +120,7 @@ struct ColArgs {
   const T* g1;
   int lhx, ny;
   int k0, nplanes;
-  int cb;        // columns per block
+  int ls;        // LDS line stride (ny + 1: keeps the transposed stores off one bank)
   int ncb;       // column blocks per plane
   int nin, nout;
   int op;
@@ -174,60 +156,52 @@ __device__ __forceinline__ Cplx<T> col_op(const ColArgs<T>& a, int o, int kx, in
   }
 }
 
-template <typename T>
+// A tile of CB adjacent kx columns of one plane: loaded row by row (CB
+// contiguous complex values per row), transposed into LDS so every column is
+// a contiguous line, transformed along y by one wavefront per column,
+// combined by the spectral op and transformed back, then stored row by row.
+template <typename T, int CB>
 __global__ void __launch_bounds__(LES_BLOCK) k_cols(ColArgs<T> a, FftPlan pl, const Cplx<T>* __restrict__ tw) {
   extern __shared__ unsigned char smem_raw[];
-  const int ny = a.ny, cb = a.cb;
-  const int L = cb * ny;
+  const int ny = a.ny, LS = a.ls;
   Cplx<T>* X0 = reinterpret_cast<Cplx<T>*>(smem_raw);
-  Cplx<T>* X1 = X0 + L;
-  Cplx<T>* Y = X0 + 2 * L;
-  Cplx<T>* W = X0 + 3 * L;
+  Cplx<T>* X1 = X0 + CB * LS;
+  Cplx<T>* Y = X0 + (a.nin > 1 ? 2 : 1) * CB * LS;
   const int k = a.k0 + blockIdx.x / a.ncb;
-  const int c0 = (blockIdx.x % a.ncb) * cb;
+  const int c0 = (blockIdx.x % a.ncb) * CB;
   int nc = a.lhx - c0;
-  if (nc > cb) nc = cb;
-  const long pbase = (long)k * ny * a.lhx;
-  // load inputs transposed: X[c * ny + j]
+  if (nc > CB) nc = CB;
+  const long pbase = (long)k * ny * a.lhx + c0;
   for (int t = 0; t < a.nin; ++t) {
     const Cplx<T>* __restrict__ src = a.in[t] + pbase;
     Cplx<T>* X = t == 0 ? X0 : X1;
-    for (int idx = threadIdx.x; idx < nc * ny; idx += blockDim.x) {
-      const int j = idx / nc, c = idx - j * nc;
-      X[c * ny + j] = src[(long)j * a.lhx + c0 + c];
+    for (int idx = threadIdx.x; idx < CB * ny; idx += blockDim.x) {
+      const int j = idx / CB, c = idx % CB;
+      if (c < nc) X[c * LS + j] = src[(long)j * a.lhx + c];
     }
   }
   __syncthreads();
-  Cplx<T>* R0 = X0;
-  Cplx<T>* R1 = X1;
+  const int lane = threadIdx.x & (LES_WAVE - 1), wave = threadIdx.x / LES_WAVE, nw = blockDim.x / LES_WAVE;
   if (a.fwd) {
-    // forward transforms of the inputs; Y and W serve as scratch
-    R0 = lds_fft(X0, Y, nc, ny, pl, tw, false);
-    if (R0 == Y) {  // keep inputs out of the output buffers
-      for (int idx = threadIdx.x; idx < nc * ny; idx += blockDim.x) X0[idx] = Y[idx];
-      __syncthreads();
-      R0 = X0;
-    }
-    if (a.nin > 1) {
-      R1 = lds_fft(X1, W, nc, ny, pl, tw, false);
-      if (R1 == W) {
-        for (int idx = threadIdx.x; idx < nc * ny; idx += blockDim.x) X1[idx] = W[idx];
-        __syncthreads();
-        R1 = X1;
-      }
+    for (int c = wave; c < nc; c += nw) {
+      wave_fft(X0 + c * LS, pl, tw, false, lane);
+      if (a.nin > 1) wave_fft(X1 + c * LS, pl, tw, false, lane);
     }
   }
   for (int o = 0; o < a.nout; ++o) {
-    for (int idx = threadIdx.x; idx < nc * ny; idx += blockDim.x) {
-      const int c = idx / ny, j = idx - c * ny;
-      Y[idx] = col_op(a, o, c0 + c, j, R0[idx], a.nin > 1 ? R1[idx] : R0[idx]);
+    for (int c = wave; c < nc; c += nw) {
+      const Cplx<T>* x0 = X0 + c * LS;
+      const Cplx<T>* x1 = (a.nin > 1 ? X1 : X0) + c * LS;
+      Cplx<T>* y = Y + c * LS;
+      for (int j = lane; j < ny; j += LES_WAVE) y[j] = col_op(a, o, c0 + c, j, x0[j], x1[j]);
+      wave_sync();
+      if (a.inv) wave_fft(y, pl, tw, true, lane);
     }
     __syncthreads();
-    Cplx<T>* Rs = a.inv ? lds_fft(Y, W, nc, ny, pl, tw, true) : Y;
     Cplx<T>* __restrict__ dst = a.out[o] + pbase;
-    for (int idx = threadIdx.x; idx < nc * ny; idx += blockDim.x) {
-      const int j = idx / nc, c = idx - j * nc;
-      dst[(long)j * a.lhx + c0 + c] = Rs[c * ny + j];
+    for (int idx = threadIdx.x; idx < CB * ny; idx += blockDim.x) {
+      const int j = idx / CB, c = idx % CB;
+      if (c < nc) dst[(long)j * a.lhx + c] = Y[c * LS + j];
     }
     __syncthreads();
   }
@@ -237,18 +211,6 @@ __global__ void __launch_bounds__(LES_BLOCK) k_cols(ColArgs<T> a, FftPlan pl, co
 // Padding keeps modes |k| < n/2 (Nyquist dropped) of a length-n1 line in a
 // length-n2 spectrum; truncation is the mirror image
 // (src/compute_convective_padd.f90:288-319, 352-382).
-template <typename T>
-__device__ __forceinline__ void remap_modes(const Cplx<T>* z, int n1, Cplx<T>* zb, int n2, int nlines, int s1, int s2) {
-  const int h = (n1 < n2 ? n1 : n2) / 2;
-  for (int idx = threadIdx.x; idx < nlines * n2; idx += blockDim.x) {
-    const int l = idx / n2, kk = idx - l * n2;
-    Cplx<T> v = {(T)0, (T)0};
-    if (kk < h) v = z[l * s1 + kk];
-    else if (kk > n2 - h) v = z[l * s1 + n1 - (n2 - kk)];
-    zb[l * s2 + kk] = v;
-  }
-}
-
 // Convection source planes padded in x: which = 0..5 -> -c u, -c v, -c w,
 // c (dw/dy - dv/dz), c (du/dz - dw/dx), c (dv/dx - du/dy), with the wall-law
 // first level of the vorticities (src/compute_convective_padd.f90:133-169).
@@ -275,9 +237,10 @@ __device__ __forceinline__ T conv_src(const ConvSrc<TI>& s, int which, int k, lo
   }
 }
 
-// Row resampling: real rows of length n1 -> real rows of length n2.
-// src_mode 0: plain real array (nx = n1) src[f]; src_mode 1: convection
-// sources (field = blockIdx.y) computed from ConvSrc.
+// Row resampling: real rows of length n1 -> real rows of length n2, one
+// wavefront per row pair (transform, pad or truncate in place, transform
+// back).  src_mode 0: plain real array (nx = n1) src[f]; src_mode 1:
+// convection sources (field = blockIdx.y) computed from ConvSrc.
 template <typename T, typename TI, typename TO>
 struct RowResArgs {
   const T* src[LES_MAX_FIELDS];
@@ -288,7 +251,6 @@ struct RowResArgs {
   int n1, n2, ny;
   int k0, nplanes;
   long src_plane, dst_plane;  // plane strides (elements)
-  int lpb;
 };
 
 template <typename T, typename TI, typename TO>
@@ -297,49 +259,39 @@ __global__ void __launch_bounds__(LES_BLOCK) k_rows_resample(RowResArgs<T, TI, T
   extern __shared__ unsigned char smem_raw[];
   const int n1 = a.n1, n2 = a.n2;
   const int nm = n1 > n2 ? n1 : n2;
-  Cplx<T>* bufA = reinterpret_cast<Cplx<T>*>(smem_raw);
-  Cplx<T>* bufB = bufA + a.lpb * nm;
-  const int f = blockIdx.y;
+  const int lane = threadIdx.x & (LES_WAVE - 1), wave = threadIdx.x / LES_WAVE;
+  Cplx<T>* z = reinterpret_cast<Cplx<T>*>(smem_raw) + wave * nm;
   const int npairs = a.ny / 2;
-  const long job0 = (long)blockIdx.x * a.lpb;
-  const long njobs = (long)a.nplanes * npairs;
-  int nl = a.lpb;
-  if (job0 + nl > njobs) nl = (int)(njobs - job0);
+  const long job = (long)blockIdx.x * (blockDim.x / LES_WAVE) + wave;
+  if (job >= (long)a.nplanes * npairs) return;
+  const int f = blockIdx.y;
+  const int k = a.k0 + (int)(job / npairs), jp = (int)(job % npairs);
   const long np = (long)n1 * a.ny;
-  for (int idx = threadIdx.x; idx < nl * n1; idx += blockDim.x) {
-    const int l = idx / n1, n = idx - l * n1;
-    const long job = job0 + l;
-    const int k = a.k0 + (int)(job / npairs), jp = (int)(job % npairs);
-    T v0, v1;
-    if (a.src_mode == 1) {
+  if (a.src_mode == 1) {
+    for (int n = lane; n < n1; n += LES_WAVE) {
       const long q = (long)(2 * jp) * n1 + n;
-      v0 = conv_src<T, TI>(a.cs, f, k, q, np, a.cconv);
-      v1 = conv_src<T, TI>(a.cs, f, k, q + n1, np, a.cconv);
-    } else {
-      const long b = (long)k * a.src_plane + (long)(2 * jp) * n1 + n;
-      v0 = a.src[f][b];
-      v1 = a.src[f][b + n1];
+      z[n] = {conv_src<T, TI>(a.cs, f, k, q, np, a.cconv), conv_src<T, TI>(a.cs, f, k, q + n1, np, a.cconv)};
     }
-    bufA[l * nm + n] = {v0, v1};
+  } else {
+    const T* __restrict__ src = a.src[f] + (long)k * a.src_plane + (long)(2 * jp) * n1;
+    for (int n = lane; n < n1; n += LES_WAVE) z[n] = {src[n], src[n + n1]};
   }
-  __syncthreads();
-  Cplx<T>* Z = lds_fft(bufA, bufB, nl, nm, p1, tw, false);
-  Cplx<T>* Zb = (Z == bufA) ? bufB : bufA;
-  remap_modes(Z, n1, Zb, n2, nl, nm, nm);
-  __syncthreads();
-  Cplx<T>* R = lds_fft(Zb, Z, nl, nm, p2, tw, true);
-  for (int idx = threadIdx.x; idx < nl * n2; idx += blockDim.x) {
-    const int l = idx / n2, n = idx - l * n2;
-    const long job = job0 + l;
-    const int k = a.k0 + (int)(job / npairs), jp = (int)(job % npairs);
-    const long ob = (long)k * a.dst_plane + (long)(2 * jp) * n2 + n;
-    const Cplx<T> z = R[l * nm + n];
-    a.dst[f][ob] = (TO)z.x;
-    a.dst[f][ob + n2] = (TO)z.y;
+  wave_sync();
+  wave_fft(z, p1, tw, false, lane);
+  wave_remap(z, n1, n2, lane);
+  wave_fft(z, p2, tw, true, lane);
+  TO* __restrict__ dst = a.dst[f] + (long)k * a.dst_plane + (long)(2 * jp) * n2;
+  for (int n = lane; n < n2; n += LES_WAVE) {
+    const Cplx<T> v = z[n];
+    dst[n] = (TO)v.x;
+    dst[n + n2] = (TO)v.y;
   }
 }
 
-// Column resampling: columns (i, i+1) of a real (W, h1) plane -> (W, h2).
+// Column resampling: column pairs (i, i+1) of a real (W, h1) plane -> (W, h2).
+// A tile of CP pairs is loaded row by row (2 CP contiguous reals), held as
+// CP complex lines in LDS, resampled by one wavefront per line, stored row by
+// row.
 template <typename T>
 struct ColResArgs {
   const T* src[LES_MAX_FIELDS];
@@ -347,42 +299,47 @@ struct ColResArgs {
   int W, h1, h2;
   int k0, nplanes;
   long src_plane, dst_plane;
-  int cp;   // column pairs per block
-  int ncp;  // column-pair blocks per plane
+  int ls;   // LDS line stride (max(h1, h2) + 1)
+  int ncp;  // column-pair tiles per plane
 };
 
-template <typename T>
+template <typename T, int CP>
 __global__ void __launch_bounds__(LES_BLOCK) k_cols_resample(ColResArgs<T> a, FftPlan p1, FftPlan p2,
                                                              const Cplx<T>* __restrict__ tw) {
   extern __shared__ unsigned char smem_raw[];
-  const int h1 = a.h1, h2 = a.h2;
-  const int nm = h1 > h2 ? h1 : h2;
-  Cplx<T>* bufA = reinterpret_cast<Cplx<T>*>(smem_raw);
-  Cplx<T>* bufB = bufA + a.cp * nm;
+  const int h1 = a.h1, h2 = a.h2, LS = a.ls;
+  Cplx<T>* buf = reinterpret_cast<Cplx<T>*>(smem_raw);
   const int f = blockIdx.y;
   const int k = a.k0 + blockIdx.x / a.ncp;
-  const int p0 = (blockIdx.x % a.ncp) * a.cp;  // first column pair
+  const int p0 = (blockIdx.x % a.ncp) * CP;  // first column pair
   int nl = a.W / 2 - p0;
-  if (nl > a.cp) nl = a.cp;
-  const T* __restrict__ src = a.src[f] + (long)k * a.src_plane;
-  for (int idx = threadIdx.x; idx < nl * 2 * h1; idx += blockDim.x) {
-    const int j = idx / (2 * nl), r = idx - j * (2 * nl);
-    const int l = r >> 1, comp = r & 1;
-    const T v = src[(long)j * a.W + 2 * (p0 + l) + comp];
-    if (comp == 0) bufA[l * nm + j].x = v;
-    else bufA[l * nm + j].y = v;
+  if (nl > CP) nl = CP;
+  const T* __restrict__ src = a.src[f] + (long)k * a.src_plane + 2 * p0;
+  for (int idx = threadIdx.x; idx < 2 * CP * h1; idx += blockDim.x) {
+    const int j = idx / (2 * CP), r = idx % (2 * CP);
+    const int l = r >> 1;
+    if (l < nl) {
+      const T v = src[(long)j * a.W + r];
+      if (r & 1) buf[l * LS + j].y = v;
+      else buf[l * LS + j].x = v;
+    }
   }
   __syncthreads();
-  Cplx<T>* Z = lds_fft(bufA, bufB, nl, nm, p1, tw, false);
-  Cplx<T>* Zb = (Z == bufA) ? bufB : bufA;
-  remap_modes(Z, h1, Zb, h2, nl, nm, nm);
+  const int lane = threadIdx.x & (LES_WAVE - 1), wave = threadIdx.x / LES_WAVE, nw = blockDim.x / LES_WAVE;
+  for (int l = wave; l < nl; l += nw) {
+    Cplx<T>* z = buf + l * LS;
+    wave_fft(z, p1, tw, false, lane);
+    wave_remap(z, h1, h2, lane);
+    wave_fft(z, p2, tw, true, lane);
+  }
   __syncthreads();
-  Cplx<T>* R = lds_fft(Zb, Z, nl, nm, p2, tw, true);
-  T* __restrict__ dst = a.dst[f] + (long)k * a.dst_plane;
-  for (int idx = threadIdx.x; idx < nl * 2 * h2; idx += blockDim.x) {
-    const int j = idx / (2 * nl), r = idx - j * (2 * nl);
-    const int l = r >> 1, comp = r & 1;
-    const Cplx<T> z = R[l * nm + j];
-    dst[(long)j * a.W + 2 * (p0 + l) + comp] = comp == 0 ? z.x : z.y;
+  T* __restrict__ dst = a.dst[f] + (long)k * a.dst_plane + 2 * p0;
+  for (int idx = threadIdx.x; idx < 2 * CP * h2; idx += blockDim.x) {
+    const int j = idx / (2 * CP), r = idx % (2 * CP);
+    const int l = r >> 1;
+    if (l < nl) {
+      const Cplx<T> z = buf[l * LS + j];
+      dst[(long)j * a.W + r] = (r & 1) ? z.y : z.x;
+    }
   }
 }

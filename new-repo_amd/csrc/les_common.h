@@ -1,5 +1,5 @@
 // Shared device-side building blocks of the LES library: complex arithmetic,
-// the LDS Stockham FFT used by every horizontal transform, launch helpers.
+// the wave-level LDS Stockham FFT used by every horizontal transform.
 //
 // FFT algorithm (fixed so results are reproducible across devices and against
 // the CPU restatement used in testing): complex length-n transforms with
@@ -67,40 +67,81 @@ __device__ __forceinline__ void bfly(Cplx<T>* x, int R, int sgn) {
   }
 }
 
-// Transform `nlines` lines held in LDS at a + l*stride (l < nlines) with the
-// block's threads; b is scratch of the same shape.  Returns the buffer that
-// holds the result (a or b).  Ends with a barrier.
+// Wave-level transform of one line of n <= LES_MAX_N points held in LDS,
+// in place.  The 64 lanes of a wavefront own the line: every stage first
+// loads the inputs of all of the lane's butterflies into registers, then
+// writes the outputs, so no other wave and no block barrier is involved.
+// Within a wavefront LDS instructions complete in issue order; wave_sync()
+// keeps the compiler from moving LDS accesses across the phase boundaries.
+#define LES_WAVE 64
+#define LES_MAX_N 1024
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename T, int R>
+__device__ __forceinline__ void wave_stage(Cplx<T>* z, int n, int p, const Cplx<T>* __restrict__ tw, int sgn, int lane) {
+  constexpr int MAXB = (LES_MAX_N / R + LES_WAVE - 1) / LES_WAVE;
+  const int m = n / R;
+  const int pm = p - 1;
+  const bool pow2 = (p & pm) == 0;
+  Cplx<T> x[MAXB][4];
+#pragma unroll
+  for (int t = 0; t < MAXB; ++t) {
+    const int i = lane + LES_WAVE * t;
+    if (i < m) {
+      const int k = pow2 ? (i & pm) : (i % p);
+      x[t][0] = z[i];
+#pragma unroll
+      for (int r = 1; r < R; ++r) x[t][r] = c_mul(z[i + r * m], tw[(r - 1) * p + k]);
+    }
+  }
+  wave_sync();
+#pragma unroll
+  for (int t = 0; t < MAXB; ++t) {
+    const int i = lane + LES_WAVE * t;
+    if (i < m) {
+      const int k = pow2 ? (i & pm) : (i % p);
+      bfly(x[t], R, sgn);
+      const int j = (i - k) * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) z[j + r * p] = x[t][r];
+    }
+  }
+  wave_sync();
+}
+
 template <typename T>
-__device__ Cplx<T>* lds_fft(Cplx<T>* a, Cplx<T>* b, int nlines, int stride, const FftPlan& pl,
-                            const Cplx<T>* __restrict__ twpool, bool inverse) {
-  const int n = pl.n;
+__device__ __forceinline__ void wave_fft(Cplx<T>* z, const FftPlan& pl, const Cplx<T>* __restrict__ twpool, bool inverse,
+                                         int lane) {
   const int sgn = inverse ? 1 : -1;
   const Cplx<T>* tw0 = twpool + (inverse ? pl.twi : pl.twf);
-  Cplx<T>* in = a;
-  Cplx<T>* out = b;
   for (int s = 0; s < pl.nst; ++s) {
-    const int R = pl.R[s], p = pl.P[s], m = n / R;
+    const int R = pl.R[s], p = pl.P[s];
     const Cplx<T>* tw = tw0 + pl.off[s];
-    const int total = nlines * m;
-    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-      const int l = idx / m;
-      const int i = idx - l * m;
-      const int k = i % p;
-      const Cplx<T>* src = in + l * stride;
-      Cplx<T> x[4];
-      x[0] = src[i];
-      for (int r = 1; r < R; ++r) x[r] = c_mul(src[i + r * m], tw[(r - 1) * p + k]);
-      bfly(x, R, sgn);
-      Cplx<T>* dst = out + l * stride;
-      const int j = (i - k) * R + k;
-      for (int r = 0; r < R; ++r) dst[j + r * p] = x[r];
-    }
-    __syncthreads();
-    Cplx<T>* t = in;
-    in = out;
-    out = t;
+    if (R == 4) wave_stage<T, 4>(z, pl.n, p, tw, sgn, lane);
+    else if (R == 2) wave_stage<T, 2>(z, pl.n, p, tw, sgn, lane);
+    else wave_stage<T, 3>(z, pl.n, p, tw, sgn, lane);
   }
-  return in;
+}
+
+// 3/2 padding / truncation of one transformed line in place (buffer of at
+// least max(n1, n2) points): modes |k| < min(n1, n2)/2 are kept, the
+// Nyquist mode of the shorter length is dropped, the rest is zero
+// (src/compute_convective_padd.f90:288-319, 352-382).  For n2 = 3 n1 / 2 and
+// n1 = 3 n2 / 2 the moved high modes [n1-h+1, n1) and their targets
+// [n2-h+1, n2) never overlap, so they are copied directly; the zeroed band
+// [h, n2-h] overlaps the sources and is written after the copies.
+template <typename T>
+__device__ __forceinline__ void wave_remap(Cplx<T>* z, int n1, int n2, int lane) {
+  const int h = (n1 < n2 ? n1 : n2) / 2;
+  for (int q = lane; q < h - 1; q += LES_WAVE) z[n2 - h + 1 + q] = z[n1 - h + 1 + q];
+  wave_sync();
+  for (int kk = h + lane; kk <= n2 - h; kk += LES_WAVE) z[kk] = Cplx<T>{(T)0, (T)0};
+  wave_sync();
 }
 
 // Twiddle pool of one precision, plus plans of the four lengths in use.

@@ -47,7 +47,7 @@ static bool build_plan(int n, HostPlan& hp, int pool_off, int* used) {
   int a = 0, b = 0, m = n;
   while (m % 2 == 0) { m /= 2; ++a; }
   while (m % 3 == 0) { m /= 3; ++b; }
-  if (m != 1 || n < 2) return false;
+  if (m != 1 || n < 2 || n > LES_MAX_N) return false;  // wave-level transforms hold n <= LES_MAX_N points
   FftPlan& p = hp.p;
   p.n = n;
   p.nst = 0;
@@ -119,11 +119,8 @@ static void kt_end(Les& L, int fam, double bytes = 0.0) {
 }
 
 // ------------------------------------------------------------------ launch helpers
-template <typename T>
-static int lines_per_block(int n) {
-  int l = (sizeof(T) == 8 ? 1024 : 2048) / n;
-  return l < 1 ? 1 : l;
-}
+// Row kernels: one wavefront per row pair, LES_BLOCK / 64 of them per block.
+static const int kWavesPerBlock = LES_BLOCK / LES_WAVE;
 
 template <typename TI, typename T>
 static void rows_r2c(Les& L, const TI* const* src, Cplx<T>* const* dst, int nf, int k0, int npl, T scale, int mode = LOAD_SCALE,
@@ -131,11 +128,10 @@ static void rows_r2c(Les& L, const TI* const* src, Cplx<T>* const* dst, int nf, 
   RowR2CArgs<TI, T> a;
   for (int f = 0; f < nf; ++f) { a.src[f] = src[f]; a.dst[f] = dst[f]; }
   a.nx = L.nx; a.ny = ny_override ? ny_override : L.ny; a.lhx = L.lhx; a.k0 = k0; a.nplanes = npl;
-  a.lpb = lines_per_block<T>(L.nx);
   a.mode = mode; a.scale = scale; a.div = div;
   long jobs = (long)npl * (a.ny / 2);
-  dim3 grid((unsigned)((jobs + a.lpb - 1) / a.lpb), nf);
-  size_t sh = 2 * (size_t)a.lpb * L.nx * sizeof(Cplx<T>);
+  dim3 grid((unsigned)((jobs + kWavesPerBlock - 1) / kWavesPerBlock), nf);
+  size_t sh = (size_t)kWavesPerBlock * L.nx * sizeof(Cplx<T>);
   kt_begin(L, KT_ROWS_R2C);
   hipLaunchKernelGGL((k_rows_r2c<TI, T>), grid, dim3(LES_BLOCK), sh, L.stream, a, L.px, twpool<T>(L));
   kt_end(L, KT_ROWS_R2C, (double)nf * npl * a.ny * ((double)L.nx * sizeof(TI) + (double)L.lhx * sizeof(Cplx<T>)));
@@ -146,19 +142,26 @@ static void rows_c2r(Les& L, const Cplx<T>* const* src, TO* const* dst, int nf, 
   RowC2RArgs<T, TO> a;
   for (int f = 0; f < nf; ++f) { a.src[f] = src[f]; a.dst[f] = dst[f]; }
   a.nx = L.nx; a.ny = L.ny; a.lhx = L.lhx; a.k0 = k0; a.nplanes = npl;
-  a.lpb = lines_per_block<T>(L.nx);
   long jobs = (long)npl * (L.ny / 2);
-  dim3 grid((unsigned)((jobs + a.lpb - 1) / a.lpb), nf);
-  size_t sh = 2 * (size_t)a.lpb * L.nx * sizeof(Cplx<T>);
+  dim3 grid((unsigned)((jobs + kWavesPerBlock - 1) / kWavesPerBlock), nf);
+  size_t sh = (size_t)kWavesPerBlock * L.nx * sizeof(Cplx<T>);
   kt_begin(L, KT_ROWS_C2R);
   hipLaunchKernelGGL((k_rows_c2r<T, TO>), grid, dim3(LES_BLOCK), sh, L.stream, a, L.px, twpool<T>(L));
   kt_end(L, KT_ROWS_C2R, (double)nf * npl * L.ny * ((double)L.lhx * sizeof(Cplx<T>) + (double)L.nx * sizeof(TO)));
 }
 
+// Column tiles: 128 contiguous bytes per row for single-input ops, 64 for
+// two-input ops (three LDS tiles instead of two).
 template <typename T>
-static int cols_per_block(int ny) {
-  int c = (sizeof(T) == 8 ? 1024 : 2048) / ny;
-  return c < 1 ? 1 : c;
+static int col_tile(int nin) {
+  const int cb = (nin > 1 ? 64 : 128) / (int)sizeof(Cplx<T>);
+  return cb;
+}
+
+template <typename T, int CB>
+static void launch_cols(Les& L, const ColArgs<T>& a, int npl) {
+  const size_t sh = (size_t)(a.nin + 1) * CB * a.ls * sizeof(Cplx<T>);
+  hipLaunchKernelGGL((k_cols<T, CB>), dim3((unsigned)(a.ncb * npl)), dim3(LES_BLOCK), sh, L.stream, a, L.py, twpool<T>(L));
 }
 
 template <typename T>
@@ -171,13 +174,15 @@ static void cols(Les& L, int op, const Cplx<T>* in0, const Cplx<T>* in1, Cplx<T>
   a.kky = sizeof(T) == 8 ? (const T*)L.kky64 : (const T*)L.kky32;
   a.g0 = g0; a.g1 = g1;
   a.lhx = L.lhx; a.ny = L.ny; a.k0 = k0; a.nplanes = npl;
-  a.cb = cols_per_block<T>(L.ny);
-  a.ncb = (L.lhx + a.cb - 1) / a.cb;
+  a.ls = L.ny + 1;
   a.nin = in1 ? 2 : 1;
   a.nout = nout; a.op = op; a.fwd = fwd; a.inv = inv;
-  size_t sh = 4 * (size_t)a.cb * L.ny * sizeof(Cplx<T>);
+  const int cb = col_tile<T>(a.nin);
+  a.ncb = (L.lhx + cb - 1) / cb;
   kt_begin(L, KT_COLS);
-  hipLaunchKernelGGL((k_cols<T>), dim3((unsigned)(a.ncb * npl)), dim3(LES_BLOCK), sh, L.stream, a, L.py, twpool<T>(L));
+  if (cb == 4) launch_cols<T, 4>(L, a, npl);
+  else if (cb == 8) launch_cols<T, 8>(L, a, npl);
+  else launch_cols<T, 16>(L, a, npl);
   kt_end(L, KT_COLS, (double)npl * L.ns * sizeof(Cplx<T>) * (a.nin + nout));
 }
 
@@ -460,14 +465,16 @@ static int setup_kernel_attrs() {
   hipFuncSetAttribute((const void*)k_rows_c2r<double, double>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
   hipFuncSetAttribute((const void*)k_rows_c2r<float, float>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
   hipFuncSetAttribute((const void*)k_rows_c2r<float, double>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
-  hipFuncSetAttribute((const void*)k_cols<double>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
-  hipFuncSetAttribute((const void*)k_cols<float>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+  hipFuncSetAttribute((const void*)k_cols<double, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+  hipFuncSetAttribute((const void*)k_cols<double, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+  hipFuncSetAttribute((const void*)k_cols<float, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+  hipFuncSetAttribute((const void*)k_cols<float, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
   hipFuncSetAttribute((const void*)k_rows_resample<double, double, double>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
   hipFuncSetAttribute((const void*)k_rows_resample<float, double, float>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
   hipFuncSetAttribute((const void*)k_rows_resample<float, float, float>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
   hipFuncSetAttribute((const void*)k_rows_resample<float, double, double>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
-  hipFuncSetAttribute((const void*)k_cols_resample<double>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
-  hipFuncSetAttribute((const void*)k_cols_resample<float>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+  hipFuncSetAttribute((const void*)k_cols_resample<double, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+  hipFuncSetAttribute((const void*)k_cols_resample<float, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
   return 0;
 }
 

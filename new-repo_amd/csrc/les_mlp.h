@@ -42,14 +42,19 @@ __device__ __forceinline__ f4v mfma16(s4v a, s4v b, f4v c) {
 }
 
 // GELU, tanh form, and its derivative
-__device__ __forceinline__ float gelu(float x) {
+// GELU, tanh form: 0.5 x (1 + tanh(y)), y = c (x + 0.044715 x^3), evaluated
+// as x s with s = 0.5 (1 + tanh(y)) = 1 / (1 + exp(-2y)): one hardware exp
+// and one reciprocal instead of tanhf.  Derivative: s + 2 x s (1 - s) y'/x'.
+__device__ __forceinline__ float gelu_s(float x) {
   const float c = 0.7978845608028654f;
-  return 0.5f * x * (1.0f + tanhf(c * (x + 0.044715f * x * x * x)));
+  const float y = c * (x + 0.044715f * x * x * x);
+  return __builtin_amdgcn_rcpf(1.0f + __expf(-2.0f * y));
 }
+__device__ __forceinline__ float gelu(float x) { return x * gelu_s(x); }
 __device__ __forceinline__ float gelu_d(float x) {
   const float c = 0.7978845608028654f;
-  const float t = tanhf(c * (x + 0.044715f * x * x * x));
-  return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * c * (1.0f + 3.0f * 0.044715f * x * x);
+  const float s = gelu_s(x);
+  return s + 2.0f * x * s * (1.0f - s) * c * (1.0f + 3.0f * 0.044715f * x * x);
 }
 
 // ------------------------------------------------------------------ GEMM

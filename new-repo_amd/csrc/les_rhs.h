@@ -52,6 +52,18 @@ __global__ void k_set_planes(SetPlaneArgs s) {
 }
 
 // compute_dealias_convec (src/compute_convective_padd.f90:115-262)
+// Column-pair tiles of 128 contiguous bytes per row.
+template <typename T>
+static void launch_cols_resample(Les& L, ColResArgs<T>& a, int nf, const FftPlan& p1, const FftPlan& p2) {
+  constexpr int CP = 64 / (int)sizeof(T);
+  const int nm = a.h1 > a.h2 ? a.h1 : a.h2;
+  a.ls = nm + 1;
+  a.ncp = (a.W / 2 + CP - 1) / CP;
+  const size_t sh = (size_t)CP * a.ls * sizeof(Cplx<T>);
+  hipLaunchKernelGGL((k_cols_resample<T, CP>), dim3((unsigned)(a.ncp * a.nplanes), nf), dim3(LES_BLOCK), sh, L.stream, a,
+                     p1, p2, twpool<T>(L));
+}
+
 template <typename T>
 static void convection(Les& L, double* cx, double* cy, double* cz) {
   const int nx = L.nx, ny = L.ny, nz = L.nz, nx2 = L.nx2, ny2 = L.ny2;
@@ -69,12 +81,11 @@ static void convection(Les& L, double* cx, double* cy, double* cz) {
     a.n1 = nx; a.n2 = nx2; a.ny = ny; a.k0 = 1; a.nplanes = nz + 1;
     a.src_plane = (long)nx * ny; a.dst_plane = (long)nx2 * ny;
     const int nm = nx2;
-    a.lpb = lines_per_block<T>(nm);
     long jobs = (long)a.nplanes * (ny / 2);
-    size_t sh = 2 * (size_t)a.lpb * nm * sizeof(Cplx<T>);
+    size_t sh = (size_t)kWavesPerBlock * nm * sizeof(Cplx<T>);
     kt_begin(L, KT_ROWS_RES);
-    hipLaunchKernelGGL((k_rows_resample<T, T, T>), dim3((unsigned)((jobs + a.lpb - 1) / a.lpb), 6), dim3(LES_BLOCK), sh,
-                       L.stream, a, L.px, L.px2, twpool<T>(L));
+    hipLaunchKernelGGL((k_rows_resample<T, T, T>), dim3((unsigned)((jobs + kWavesPerBlock - 1) / kWavesPerBlock), 6),
+                       dim3(LES_BLOCK), sh, L.stream, a, L.px, L.px2, twpool<T>(L));
     kt_end(L, KT_ROWS_RES, (double)a.nplanes * L.np * (3.0 * 8 + 6.0 * sizeof(T)) + 6.0 * a.nplanes * nx2 * ny * sizeof(T));
   }
   // pad in y
@@ -83,12 +94,8 @@ static void convection(Les& L, double* cx, double* cy, double* cz) {
     for (int f = 0; f < 6; ++f) { a.src[f] = (const T*)L.wk_hyb[f]; a.dst[f] = (T*)L.wk_big[f]; }
     a.W = nx2; a.h1 = ny; a.h2 = ny2; a.k0 = 1; a.nplanes = nz + 1;
     a.src_plane = (long)nx2 * ny; a.dst_plane = L.nbp;
-    a.cp = lines_per_block<T>(ny2);
-    a.ncp = (nx2 / 2 + a.cp - 1) / a.cp;
-    size_t sh = 2 * (size_t)a.cp * ny2 * sizeof(Cplx<T>);
     kt_begin(L, KT_COLS_RES);
-    hipLaunchKernelGGL((k_cols_resample<T>), dim3((unsigned)(a.ncp * a.nplanes), 6), dim3(LES_BLOCK), sh, L.stream, a,
-                       L.py, L.py2, twpool<T>(L));
+    launch_cols_resample<T>(L, a, 6, L.py, L.py2);
     kt_end(L, KT_COLS_RES, 6.0 * a.nplanes * nx2 * (double)(ny + ny2) * sizeof(T));
   }
   // products
@@ -109,12 +116,8 @@ static void convection(Les& L, double* cx, double* cy, double* cz) {
     for (int f = 0; f < 3; ++f) { a.src[f] = (const T*)L.wk_prod[f]; a.dst[f] = (T*)L.wk_hyb3[f]; }
     a.W = nx2; a.h1 = ny2; a.h2 = ny; a.k0 = 1; a.nplanes = nz;
     a.src_plane = L.nbp; a.dst_plane = (long)nx2 * ny;
-    a.cp = lines_per_block<T>(ny2);
-    a.ncp = (nx2 / 2 + a.cp - 1) / a.cp;
-    size_t sh = 2 * (size_t)a.cp * ny2 * sizeof(Cplx<T>);
     kt_begin(L, KT_COLS_RES);
-    hipLaunchKernelGGL((k_cols_resample<T>), dim3((unsigned)(a.ncp * a.nplanes), 3), dim3(LES_BLOCK), sh, L.stream, a,
-                       L.py2, L.py, twpool<T>(L));
+    launch_cols_resample<T>(L, a, 3, L.py2, L.py);
     kt_end(L, KT_COLS_RES, 3.0 * a.nplanes * nx2 * (double)(ny + ny2) * sizeof(T));
   }
   // truncate in x, straight into rhs
@@ -126,12 +129,11 @@ static void convection(Les& L, double* cx, double* cy, double* cz) {
     a.src_mode = 0;
     a.n1 = nx2; a.n2 = nx; a.ny = ny; a.k0 = 1; a.nplanes = nz;
     a.src_plane = (long)nx2 * ny; a.dst_plane = L.np;
-    a.lpb = lines_per_block<T>(nx2);
     long jobs = (long)nz * (ny / 2);
-    size_t sh = 2 * (size_t)a.lpb * nx2 * sizeof(Cplx<T>);
+    size_t sh = (size_t)kWavesPerBlock * nx2 * sizeof(Cplx<T>);
     kt_begin(L, KT_ROWS_RES);
-    hipLaunchKernelGGL((k_rows_resample<T, T, double>), dim3((unsigned)((jobs + a.lpb - 1) / a.lpb), 3), dim3(LES_BLOCK),
-                       sh, L.stream, a, L.px2, L.px, twpool<T>(L));
+    hipLaunchKernelGGL((k_rows_resample<T, T, double>), dim3((unsigned)((jobs + kWavesPerBlock - 1) / kWavesPerBlock), 3),
+                       dim3(LES_BLOCK), sh, L.stream, a, L.px2, L.px, twpool<T>(L));
     kt_end(L, KT_ROWS_RES, 3.0 * nz * ny * ((double)nx2 * sizeof(T) + (double)nx * 8));
   }
   SetPlaneArgs sp;

@@ -80,6 +80,8 @@ class LesParams:
                 m //= 3
             if m != 1 or n % 4:
                 raise ValueError(f"horizontal size {n} must be 2^a 3^b and a multiple of 4")
+            if 3 * n // 2 > 1024:
+                raise ValueError(f"horizontal size {n}: the 3/2-padded transform length is limited to 1024")
         if self.nz < 3:
             raise ValueError("nz must be >= 3")
         if self.sgs_model not in (SGS_CST_SMAG, SGS_LASD):

@@ -72,7 +72,9 @@ def test_fft2d_matches_oracle_bitwise():
             np.testing.assert_allclose(back.cpu().numpy() / (nx * ny), a * 0.25, atol=1e-13)
 
 
-@pytest.mark.parametrize("grid", [(16, 16, 8), (32, 32, 16)])
+# 256x256: the production transform lengths (256 and the 3/2-padded 384);
+# 96x48: mixed radices, padded lengths 144 and 72 with two radix-3 stages
+@pytest.mark.parametrize("grid", [(16, 16, 8), (32, 32, 16), (96, 48, 6), (256, 256, 4)])
 def test_step_matches_oracle(grid):
     pkg, O = load_pkg(), load_oracle()
     nx, ny, nz = grid
