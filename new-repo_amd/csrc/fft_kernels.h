@@ -167,8 +167,9 @@ __global__ void __launch_bounds__(LES_BLOCK) k_cols(ColArgs<T> a, FftPlan pl, co
   Cplx<T>* X0 = reinterpret_cast<Cplx<T>*>(smem_raw);
   Cplx<T>* X1 = X0 + CB * LS;
   Cplx<T>* Y = X0 + (a.nin > 1 ? 2 : 1) * CB * LS;
-  const int k = a.k0 + blockIdx.x / a.ncb;
-  const int c0 = (blockIdx.x % a.ncb) * CB;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int k = a.k0 + tile / a.ncb;
+  const int c0 = (tile % a.ncb) * CB;
   int nc = a.lhx - c0;
   if (nc > CB) nc = CB;
   const long pbase = (long)k * ny * a.lhx + c0;
@@ -310,8 +311,9 @@ __global__ void __launch_bounds__(LES_BLOCK) k_cols_resample(ColResArgs<T> a, Ff
   const int h1 = a.h1, h2 = a.h2, LS = a.ls;
   Cplx<T>* buf = reinterpret_cast<Cplx<T>*>(smem_raw);
   const int f = blockIdx.y;
-  const int k = a.k0 + blockIdx.x / a.ncp;
-  const int p0 = (blockIdx.x % a.ncp) * CP;  // first column pair
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int k = a.k0 + tile / a.ncp;
+  const int p0 = (tile % a.ncp) * CP;  // first column pair
   int nl = a.W / 2 - p0;
   if (nl > CP) nl = CP;
   const T* __restrict__ src = a.src[f] + (long)k * a.src_plane + 2 * p0;

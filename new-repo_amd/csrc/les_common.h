@@ -128,6 +128,20 @@ __device__ __forceinline__ void wave_fft(Cplx<T>* z, const FftPlan& pl, const Cp
   }
 }
 
+// XCD-aware tile order.  Workgroups are dispatched round-robin over the 8
+// XCDs (block b on XCD b % 8), each with its own L2.  Tiles that share
+// 128-byte lines (adjacent column tiles of narrower rows) must run on one XCD
+// at nearly the same time, so physical block b = 8 m + x runs logical tile
+// s_x + m, where XCD x owns the contiguous range [s_x, s_x + size_x) of the
+// n tiles (sizes q + 1 for x < n % 8, q = n / 8 otherwise).  A bijection on
+// [0, n).
+#define LES_NXCD 8
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+  const int q = n / LES_NXCD, r = n % LES_NXCD;
+  const int x = b % LES_NXCD, m = b / LES_NXCD;
+  return x * q + (x < r ? x : r) + m;
+}
+
 // 3/2 padding / truncation of one transformed line in place (buffer of at
 // least max(n1, n2) points): modes |k| < min(n1, n2)/2 are kept, the
 // Nyquist mode of the shorter length is dropped, the rest is zero

@@ -150,12 +150,19 @@ static void rows_c2r(Les& L, const Cplx<T>* const* src, TO* const* dst, int nf, 
   kt_end(L, KT_ROWS_C2R, (double)nf * npl * L.ny * ((double)L.lhx * sizeof(Cplx<T>) + (double)L.nx * sizeof(TO)));
 }
 
-// Column tiles: 128 contiguous bytes per row for single-input ops, 64 for
-// two-input ops (three LDS tiles instead of two).
+// Column tiles: LES_COL_TILE_BYTES contiguous bytes per row for
+// single-input ops, half that for two-input ops (three LDS tiles instead of
+// two), at least 4 columns.  With the XCD-aware tile order (xcd_tile) the
+// narrow rows are not over-fetched; 32-byte rows (4 FP32 / 4 FP64 columns)
+// measured fastest at 256^2 because the small tiles leave room for more
+// blocks per CU.
+#ifndef LES_COL_TILE_BYTES
+#define LES_COL_TILE_BYTES 32
+#endif
 template <typename T>
 static int col_tile(int nin) {
-  const int cb = (nin > 1 ? 64 : 128) / (int)sizeof(Cplx<T>);
-  return cb;
+  int cb = (nin > 1 ? LES_COL_TILE_BYTES / 2 : LES_COL_TILE_BYTES) / (int)sizeof(Cplx<T>);
+  return cb < 4 ? 4 : cb;
 }
 
 template <typename T, int CB>
@@ -352,50 +359,61 @@ static void sgs_stress(Les& L, int jtt) {
 // formed directly in FP64 by removing the projection on the kx = nx/2 and
 // ky = ny/2 modes: f -= (-1)^i A_j + (-1)^j B_i - (-1)^(i+j) C, with A, B, C
 // the normalised alternating sums along x, y and both.
-__global__ void k_nyq_rowsum(const double* f, double* A, int nx, long nrows) {
-  __shared__ double red[LES_BLOCK];
-  const long row = blockIdx.x;
-  if (row >= nrows) return;
+// Pass 1: one block per band of 4 rows of a plane (one row per wavefront):
+// A_j = sum_i (-1)^i f_ij / nx by wavefront reduction, and the band's
+// alternating column partial sums P_i = sum_{j in band} (-1)^j f_ij.  Every
+// sum has a fixed order, so results are reproducible.
+__global__ void __launch_bounds__(LES_BLOCK) k_nyq_reduce(const double* __restrict__ f, double* __restrict__ A,
+                                                          double* __restrict__ P, int nx, int ny) {
+  __shared__ double part[LES_BLOCK / LES_WAVE][LES_MAX_N];
+  const int lane = threadIdx.x & (LES_WAVE - 1), wave = threadIdx.x / LES_WAVE;
+  const long row = (long)blockIdx.x * (LES_BLOCK / LES_WAVE) + wave;  // global row (k * ny + j)
+  const int j = (int)(row % ny);
+  const double* __restrict__ fr = f + row * nx;
+  const double sj = (j & 1) ? -1.0 : 1.0;
   double s = 0.0;
-  for (int i = threadIdx.x; i < nx; i += blockDim.x) s += (i & 1) ? -f[row * nx + i] : f[row * nx + i];
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-    __syncthreads();
+  for (int i = lane; i < nx; i += LES_WAVE) {
+    const double v = fr[i];
+    s += (i & 1) ? -v : v;
+    part[wave][i] = sj * v;
   }
-  if (threadIdx.x == 0) A[row] = red[0] / (double)nx;
+  for (int o = LES_WAVE / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, LES_WAVE);
+  if (lane == 0) A[row] = s / (double)nx;
+  __syncthreads();
+  for (int i = threadIdx.x; i < nx; i += blockDim.x)
+    P[(long)blockIdx.x * nx + i] = ((part[0][i] + part[1][i]) + part[2][i]) + part[3][i];
 }
 
-__global__ void k_nyq_colsum(const double* f, double* B, int nx, int ny, int nplanes) {
+// Pass 2: B_i = sum of the ny/4 band partials / ny per column of each plane;
+// C = sum_j (-1)^j A_j / ny per plane (thread i = 0).
+__global__ void k_nyq_bc(const double* __restrict__ A, const double* __restrict__ P, double* __restrict__ B,
+                         double* __restrict__ C, int nx, int ny, int nplanes) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)nx * nplanes) return;
   const int i = (int)(t % nx), k = (int)(t / nx);
+  const int nb = ny / 4;
   double s = 0.0;
-  for (int j = 0; j < ny; ++j) {
-    const double v = f[((long)k * ny + j) * nx + i];
-    s += (j & 1) ? -v : v;
-  }
+  for (int b = 0; b < nb; ++b) s += P[((long)k * nb + b) * nx + i];
   B[t] = s / (double)ny;
+  if (i == 0) {
+    double c = 0.0;
+    for (int j = 0; j < ny; ++j) c += (j & 1) ? -A[(long)k * ny + j] : A[(long)k * ny + j];
+    C[k] = c / (double)ny;
+  }
 }
 
-__global__ void k_nyq_csum(const double* A, double* C, int ny, int nplanes) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nplanes) return;
-  double s = 0.0;
-  for (int j = 0; j < ny; ++j) s += (j & 1) ? -A[(long)k * ny + j] : A[(long)k * ny + j];
-  C[k] = s / (double)ny;
-}
-
-__global__ void k_nyq_apply(double* f, const double* A, const double* B, const double* C, int nx, int ny, long n) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const int i = (int)(t % nx);
-  const long r = t / nx;
+// Pass 3: f -= (-1)^i A_j + (-1)^j B_i - (-1)^(i+j) C, one block per field row.
+__global__ void k_nyq_apply(double* __restrict__ f, const double* __restrict__ A, const double* __restrict__ B,
+                            const double* __restrict__ C, int nx, int ny) {
+  const long r = blockIdx.x;
   const int j = (int)(r % ny);
   const long k = r / ny;
-  const double si = (i & 1) ? -1.0 : 1.0, sj = (j & 1) ? -1.0 : 1.0;
-  f[t] -= si * A[r] + sj * B[k * nx + i] - si * sj * C[k];
+  const double sj = (j & 1) ? -1.0 : 1.0;
+  const double a = A[r], c = C[k];
+  for (int i = threadIdx.x; i < nx; i += blockDim.x) {
+    const double si = (i & 1) ? -1.0 : 1.0;
+    f[r * nx + i] -= si * a + sj * B[k * nx + i] - si * sj * c;
+  }
 }
 
 template <typename T>
@@ -404,12 +422,12 @@ static void nyquist_remove_fp64(Les& L, int fid) {
   double* A = L.nyq_rows;
   double* B = A + (long)L.ny * L.nzg;
   double* C = B + (long)L.nx * L.nzg;
+  double* P = (double*)L.wk_real[62];  // ny/4 band partials per column and plane: nf/4 doubles
   const long nrows = (long)L.ny * L.nzg;
   kt_begin(L, KT_POINTWISE);
-  hipLaunchKernelGGL(k_nyq_rowsum, dim3((unsigned)nrows), dim3(LES_BLOCK), 0, L.stream, f, A, L.nx, nrows);
-  hipLaunchKernelGGL(k_nyq_colsum, dim3(nblk((long)L.nx * L.nzg)), dim3(LES_BLOCK), 0, L.stream, f, B, L.nx, L.ny, L.nzg);
-  hipLaunchKernelGGL(k_nyq_csum, dim3(nblk(L.nzg)), dim3(LES_BLOCK), 0, L.stream, A, C, L.ny, L.nzg);
-  hipLaunchKernelGGL(k_nyq_apply, dim3(nblk(L.nf)), dim3(LES_BLOCK), 0, L.stream, f, A, B, C, L.nx, L.ny, L.nf);
+  hipLaunchKernelGGL(k_nyq_reduce, dim3((unsigned)(nrows / 4)), dim3(LES_BLOCK), 0, L.stream, f, A, P, L.nx, L.ny);
+  hipLaunchKernelGGL(k_nyq_bc, dim3(nblk((long)L.nx * L.nzg)), dim3(LES_BLOCK), 0, L.stream, A, P, B, C, L.nx, L.ny, L.nzg);
+  hipLaunchKernelGGL(k_nyq_apply, dim3((unsigned)nrows), dim3(LES_BLOCK), 0, L.stream, f, A, B, C, L.nx, L.ny);
   kt_end(L, KT_POINTWISE, 24.0 * L.nf);
 }
 
@@ -473,8 +491,10 @@ static int setup_kernel_attrs() {
   hipFuncSetAttribute((const void*)k_rows_resample<float, double, float>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
   hipFuncSetAttribute((const void*)k_rows_resample<float, float, float>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
   hipFuncSetAttribute((const void*)k_rows_resample<float, double, double>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
-  hipFuncSetAttribute((const void*)k_cols_resample<double, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
-  hipFuncSetAttribute((const void*)k_cols_resample<float, 16>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+  hipFuncSetAttribute((const void*)k_cols_resample<double, LES_COLRES_TILE_BYTES / 16>,
+                      hipFuncAttributeMaxDynamicSharedMemorySize, big);
+  hipFuncSetAttribute((const void*)k_cols_resample<float, LES_COLRES_TILE_BYTES / 8>,
+                      hipFuncAttributeMaxDynamicSharedMemorySize, big);
   return 0;
 }
 

@@ -216,50 +216,60 @@ __global__ void __launch_bounds__(256) k_mlp_infer(InferArgs a) {
   const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const long nwaves = ((long)gridDim.x * blockDim.x) >> 6;
   const int r = l & 15, q = (l >> 4) * 4;
-  // weights as A operands: W[16 rb + r][16 cb + q + j]
-  s4v w1[4], w2[16], w3[16], w4[4];
-  for (int rb = 0; rb < 4; ++rb)
-    for (int j = 0; j < 4; ++j) w1[rb][j] = a.Wb[MW1 + (16 * rb + r) * 16 + q + j];
-  for (int rb = 0; rb < 4; ++rb)
-    for (int cb = 0; cb < 4; ++cb)
-      for (int j = 0; j < 4; ++j) {
-        w2[rb * 4 + cb][j] = a.Wb[MW2 + (16 * rb + r) * 64 + 16 * cb + q + j];
-        w3[rb * 4 + cb][j] = a.Wb[MW3 + (16 * rb + r) * 64 + 16 * cb + q + j];
-      }
-  for (int cb = 0; cb < 4; ++cb)
-    for (int j = 0; j < 4; ++j) w4[cb][j] = a.Wb[MW4 + r * 64 + 16 * cb + q + j];
-  float bb1[4][4], bb2[4][4], bb3[4][4], bb4[4];
-  for (int rb = 0; rb < 4; ++rb)
-    for (int j = 0; j < 4; ++j) {
-      bb1[rb][j] = a.bias[MB1 + 16 * rb + q + j];
-      bb2[rb][j] = a.bias[MB2 + 16 * rb + q + j];
-      bb3[rb][j] = a.bias[MB3 + 16 * rb + q + j];
-    }
-  for (int j = 0; j < 4; ++j) bb4[j] = a.bias[MB4 + q + j];
+  // Weights as A-operand fragments W[16 rb + r][16 cb + q + j], j = 0..3, in
+  // LDS in lane order (one 8-byte read per fragment): blocks 0-3 W1 (rb),
+  // 4-19 W2 (rb, cb), 20-35 W3, 36-39 W4 (cb).  Biases in LDS too: read per
+  // tile instead of occupying registers, which doubles the waves per SIMD.
+  __shared__ __align__(16) s4v sw[40 * 64];
+  __shared__ __align__(16) float sb[MLP_NBIAS];
+  for (int idx = threadIdx.x; idx < 40 * 64; idx += blockDim.x) {
+    const int blk = idx >> 6, ln = idx & 63;
+    const int rr = ln & 15, qq = (ln >> 4) * 4;
+    int base;
+    if (blk < 4) base = MW1 + (16 * blk + rr) * 16 + qq;
+    else if (blk < 20) base = MW2 + (16 * ((blk - 4) >> 2) + rr) * 64 + 16 * ((blk - 4) & 3) + qq;
+    else if (blk < 36) base = MW3 + (16 * ((blk - 20) >> 2) + rr) * 64 + 16 * ((blk - 20) & 3) + qq;
+    else base = MW4 + rr * 64 + 16 * (blk - 36) + qq;
+    s4v v;
+    for (int j = 0; j < 4; ++j) v[j] = a.Wb[base + j];
+    sw[idx] = v;
+  }
+  for (int i = threadIdx.x; i < MLP_NBIAS; i += blockDim.x) sb[i] = a.bias[i];
+  __syncthreads();
+  const s4v* wl = sw + l;
+  const s4v* w1 = wl;
+  const s4v* w2 = wl + 4 * 64;
+  const s4v* w3 = wl + 20 * 64;
+  const s4v* w4 = wl + 36 * 64;
+  const f4v* bb1 = reinterpret_cast<const f4v*>(sb + MB1 + q);
+  const f4v* bb2 = reinterpret_cast<const f4v*>(sb + MB2 + q);
+  const f4v* bb3 = reinterpret_cast<const f4v*>(sb + MB3 + q);
+  const f4v bb4 = *reinterpret_cast<const f4v*>(sb + MB4 + q);
   const long ntiles = (a.n + 15) / 16;
   for (long tile = a.p0 / 16 + wave; tile < ntiles; tile += nwaves) {
+    __asm__ volatile("" ::: "memory");  // re-read the biases from LDS each tile (keeps them out of registers)
     const long p = tile * 16 + r;
     s4v x = {0, 0, 0, 0};
     if (p < a.n)
       for (int j = 0; j < 4; ++j) x[j] = a.X[p * 16 + q + j];
     s4v h1[4], h2[4], h3[4];
     for (int rb = 0; rb < 4; ++rb) {
-      f4v acc = {bb1[rb][0], bb1[rb][1], bb1[rb][2], bb1[rb][3]};
-      acc = mfma16(w1[rb], x, acc);
+      f4v acc = bb1[4 * rb];
+      acc = mfma16(w1[rb * 64], x, acc);
       for (int j = 0; j < 4; ++j) h1[rb][j] = f2bf(gelu(acc[j]));
     }
     for (int rb = 0; rb < 4; ++rb) {
-      f4v acc = {bb2[rb][0], bb2[rb][1], bb2[rb][2], bb2[rb][3]};
-      for (int cb = 0; cb < 4; ++cb) acc = mfma16(w2[rb * 4 + cb], h1[cb], acc);
+      f4v acc = bb2[4 * rb];
+      for (int cb = 0; cb < 4; ++cb) acc = mfma16(w2[(rb * 4 + cb) * 64], h1[cb], acc);
       for (int j = 0; j < 4; ++j) h2[rb][j] = f2bf(gelu(acc[j]));
     }
     for (int rb = 0; rb < 4; ++rb) {
-      f4v acc = {bb3[rb][0], bb3[rb][1], bb3[rb][2], bb3[rb][3]};
-      for (int cb = 0; cb < 4; ++cb) acc = mfma16(w3[rb * 4 + cb], h2[cb], acc);
+      f4v acc = bb3[4 * rb];
+      for (int cb = 0; cb < 4; ++cb) acc = mfma16(w3[(rb * 4 + cb) * 64], h2[cb], acc);
       for (int j = 0; j < 4; ++j) h3[rb][j] = f2bf(gelu(acc[j]));
     }
-    f4v y = {bb4[0], bb4[1], bb4[2], bb4[3]};
-    for (int cb = 0; cb < 4; ++cb) y = mfma16(w4[cb], h3[cb], y);
+    f4v y = bb4;
+    for (int cb = 0; cb < 4; ++cb) y = mfma16(w4[cb * 64], h3[cb], y);
     // lanes 0..15 hold outputs 0..3 of point r; the range flag of point r is
     // element 3 of lane 48 + r's feature registers
     const int flag_bits = __shfl((int)(unsigned short)x[3], 48 + r, 64);

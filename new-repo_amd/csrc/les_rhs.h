@@ -52,10 +52,13 @@ __global__ void k_set_planes(SetPlaneArgs s) {
 }
 
 // compute_dealias_convec (src/compute_convective_padd.f90:115-262)
-// Column-pair tiles of 128 contiguous bytes per row.
+// Column-pair tiles of LES_COLRES_TILE_BYTES contiguous bytes per row.
+#ifndef LES_COLRES_TILE_BYTES
+#define LES_COLRES_TILE_BYTES 64
+#endif
 template <typename T>
 static void launch_cols_resample(Les& L, ColResArgs<T>& a, int nf, const FftPlan& p1, const FftPlan& p2) {
-  constexpr int CP = 64 / (int)sizeof(T);
+  constexpr int CP = LES_COLRES_TILE_BYTES / 2 / (int)sizeof(T);
   const int nm = a.h1 > a.h2 ? a.h1 : a.h2;
   a.ls = nm + 1;
   a.ncp = (a.W / 2 + CP - 1) / CP;

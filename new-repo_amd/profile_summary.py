@@ -1,18 +1,19 @@
 """Summaries of rocprofv3 CSV output for the benchmark's evidence.
 
 kernels DIR            per-kernel table from <DIR>/**/*kernel_stats.csv
-traffic FETCH WRITE    HBM bytes per launch of a kernel family from two
+traffic FETCH WRITE    HBM bytes per launch of each kernel family from two
                        counter passes (FETCH_SIZE, WRITE_SIZE, KiB per
                        dispatch), each scaled by what the same counter reads
                        for the copy-calibration kernel of known size
                        (bench.py --copy-calibration MIB, an elementwise
-                       y = x * 1 over float32)
+                       y = x * 1 over float32), against the algorithmic bytes
+                       of the same dispatches (bench.py --steps 0 output)
 
 Usage:
   python new-repo_amd/profile_summary.py kernels gpurun_out/trace
   python new-repo_amd/profile_summary.py traffic gpurun_out/pmc_fetch gpurun_out/pmc_write \\
-      --family cols --pattern k_cols --variant learned --grid 256,256,256 --copy-mib 256 \\
-      --algorithmic-bytes B --command "..." --out profiles/pmc_traffic.json
+      --bench-json gpurun_out/pmc_fetch.json --variant learned --copy-mib 256 \\
+      --command "..." --out profiles/pmc_traffic.json
 """
 from __future__ import annotations
 
@@ -66,29 +67,49 @@ def _per_dispatch(d: str, counter: str):
     return [(k[1], v) for k, v in vals.items()]
 
 
+# kernel-name patterns of the benchmark's kernel families
+FAMILY_KERNELS = {"rows_r2c": "k_rows_r2c<", "rows_c2r": "k_rows_c2r<", "cols": "k_cols<",
+                  "rows_resample": "k_rows_resample<", "cols_resample": "k_cols_resample<",
+                  "tridiag": "k_tridiag", "lagrangian": "k_lagr<"}
+
+
 def traffic(args):
-    def avg(d, counter, pat):
-        v = [x for n, x in _per_dispatch(d, counter) if pat in n]
+    """Per family: counter bytes summed over its dispatches (scaled by the
+    copy calibration) against the algorithmic bytes the library accumulated
+    for the same dispatches (bench.py --steps 0 output of the counter run)."""
+    lines = [ln for ln in pathlib.Path(args.bench_json).read_text().splitlines() if ln.startswith("{")]
+    bench = json.loads(lines[-1])
+    fam = bench["variants"][args.variant]["families"]
+    fd, wd = _per_dispatch(args.fetch, "FETCH_SIZE"), _per_dispatch(args.write, "WRITE_SIZE")
+
+    def cal(d, counter):
+        v = [x for n, x in d if args.copy_pattern in n]
         if not v:
-            raise SystemExit(f"no {counter} dispatches matching {pat!r} in {d}")
-        return sum(v) / len(v), len(v)
-    copy_pat = args.copy_pattern
-    exp_kib = args.copy_mib * 1024.0
-    f_cal, nf_cal = avg(args.fetch, "FETCH_SIZE", copy_pat)
-    w_cal, nw_cal = avg(args.write, "WRITE_SIZE", copy_pat)
-    f_k, nf = avg(args.fetch, "FETCH_SIZE", args.pattern)
-    w_k, nw = avg(args.write, "WRITE_SIZE", args.pattern)
-    sf, sw = exp_kib / f_cal, exp_kib / w_cal
-    tbytes = (f_k * sf + w_k * sw) * 1024.0
-    out = {"variant": args.variant, "kernel_family": args.family, "kernel_pattern": args.pattern,
-           "grid": [int(x) for x in args.grid.split(",")],
-           "fetch_kib_per_dispatch": f_k, "write_kib_per_dispatch": w_k, "dispatches": [nf, nw],
-           "copy_calibration": {"pattern": copy_pat, "mib": args.copy_mib, "fetch_kib": f_cal, "write_kib": w_cal,
-                                "fetch_scale": sf, "write_scale": sw, "dispatches": [nf_cal, nw_cal]},
-           "traffic_bytes_per_launch": tbytes,
-           "algorithmic_bytes_per_launch": args.algorithmic_bytes,
-           "traffic_over_algorithmic": tbytes / args.algorithmic_bytes if args.algorithmic_bytes else None,
-           "commands": args.command}
+            raise SystemExit(f"no {counter} dispatches of the copy kernel ({args.copy_pattern!r})")
+        return args.copy_mib * 1024.0 / (sum(v) / len(v)), len(v)
+
+    sf, ncf = cal(fd, "FETCH_SIZE")
+    sw, ncw = cal(wd, "WRITE_SIZE")
+    out = {"variant": args.variant, "grid": bench["grid"],
+           "copy_calibration": {"pattern": args.copy_pattern, "mib": args.copy_mib, "fetch_scale": sf,
+                                "write_scale": sw, "dispatches": [ncf, ncw]},
+           "commands": args.command, "families": {}}
+    for name, pat in FAMILY_KERNELS.items():
+        if name not in fam:
+            continue
+        f = [x for n, x in fd if pat in n]
+        w = [x for n, x in wd if pat in n]
+        if not f or not w:
+            continue
+        calls = fam[name]["calls"]
+        tb = (sum(f) * sf + sum(w) * sw) * 1024.0
+        out["families"][name] = {
+            "kernel_pattern": pat, "dispatches": [len(f), len(w)], "calls": calls,
+            "fetch_bytes_per_launch": sum(f) * sf * 1024.0 / len(f),
+            "write_bytes_per_launch": sum(w) * sw * 1024.0 / len(w),
+            "traffic_bytes_per_launch": tb / len(f),
+            "algorithmic_bytes_per_launch": fam[name]["bytes"] / calls,
+            "traffic_over_algorithmic": tb / fam[name]["bytes"]}
     return out
 
 
@@ -101,13 +122,10 @@ def main(argv=None):
     t = sub.add_parser("traffic")
     t.add_argument("fetch")
     t.add_argument("write")
-    t.add_argument("--family", required=True)
-    t.add_argument("--pattern", required=True)
-    t.add_argument("--copy-pattern", default="elementwise")
+    t.add_argument("--bench-json", required=True, help="stdout of the counter run (bench.py --steps 0)")
+    t.add_argument("--copy-pattern", default="MulFunctor")
     t.add_argument("--copy-mib", type=int, required=True)
     t.add_argument("--variant", required=True)
-    t.add_argument("--grid", required=True)
-    t.add_argument("--algorithmic-bytes", type=float, default=0.0)
     t.add_argument("--command", action="append", default=[])
     t.add_argument("--out")
     a = ap.parse_args(argv)
