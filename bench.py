@@ -272,7 +272,13 @@ def main(argv=None):
                 "frac": ach / PEAK_HBM, "traffic": tr["traffic_bytes_per_launch"] if tr else None,
                 "kernel_family": dom, "avg_launch_ms": d["avg_ms"], "bytes_per_launch": d["bytes"] / d["calls"]}
         if tr:
-            roof["traffic_source"] = "profiles/pmc_traffic.json (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)"
+            # measured on the counter run's own launches (steps 1-5 of a fresh
+            # state, whose first dynamic update is a full plain update), so
+            # compare it with the algorithmic bytes of those same launches
+            roof["traffic_algorithmic_per_launch"] = tr["algorithmic_bytes_per_launch"]
+            roof["traffic_ratio"] = tr["traffic_over_algorithmic"]
+            roof["traffic_source"] = ("profiles/pmc_traffic.json: rocprofv3 FETCH_SIZE and WRITE_SIZE passes over "
+                                      "steps 1-5, each scaled by a 256 MiB copy kernel in the same process")
     step_key = "fp64" if head == "fp64" else "mixed"
     step_bytes = STEP_BYTES[step_key] * nx * ny * nz
     out = {
