@@ -31,15 +31,10 @@ struct RowR2CArgs {
 
 // One wavefront per row pair: load, transform and unpack without any
 // block-level barrier.  LDS: one line of nx points per wavefront.
-template <typename TI, typename T>
-__global__ void __launch_bounds__(LES_BLOCK) k_rows_r2c(RowR2CArgs<TI, T> a, FftPlan pl, const Cplx<T>* __restrict__ tw) {
-  extern __shared__ unsigned char smem_raw[];
+template <typename TI, typename T, typename FFT>
+__device__ __forceinline__ void rows_r2c_body(const RowR2CArgs<TI, T>& a, Cplx<T>* z, int lane, long job, FFT&& fft) {
   const int nx = a.nx, lhx = a.lhx;
-  const int lane = threadIdx.x & (LES_WAVE - 1), wave = threadIdx.x / LES_WAVE;
-  Cplx<T>* z = reinterpret_cast<Cplx<T>*>(smem_raw) + wave * nx;
   const int npairs = a.ny / 2;
-  const long job = (long)blockIdx.x * (blockDim.x / LES_WAVE) + wave;
-  if (job >= (long)a.nplanes * npairs) return;
   const int f = blockIdx.y;
   const int k = a.k0 + (int)(job / npairs), jp = (int)(job % npairs);
   const TI* __restrict__ src = a.src[f] + ((long)k * a.ny + 2 * jp) * nx;
@@ -55,7 +50,7 @@ __global__ void __launch_bounds__(LES_BLOCK) k_rows_r2c(RowR2CArgs<TI, T> a, Fft
     z[n] = {v0, v1};
   }
   wave_sync();
-  wave_fft(z, pl, tw, false, lane);
+  fft(z, false);
   Cplx<T>* __restrict__ dst = a.dst[f] + ((long)k * a.ny + 2 * jp) * lhx;
   for (int kk = lane; kk < lhx; kk += LES_WAVE) {
     const Cplx<T> zk = z[kk], zm = z[kk == 0 ? 0 : nx - kk];
@@ -63,6 +58,22 @@ __global__ void __launch_bounds__(LES_BLOCK) k_rows_r2c(RowR2CArgs<TI, T> a, Fft
     const Cplx<T> B = {(T)0.5 * (zk.y + zm.y), (T)0.5 * (zm.x - zk.x)};
     dst[kk] = A;
     dst[kk + lhx] = B;
+  }
+}
+
+template <typename TI, typename T>
+__global__ void __launch_bounds__(LES_BLOCK) k_rows_r2c(RowR2CArgs<TI, T> a, FftPlan pl, const Cplx<T>* __restrict__ tw) {
+  extern __shared__ unsigned char smem_raw[];
+  const int lane = threadIdx.x & (LES_WAVE - 1), wave = threadIdx.x / LES_WAVE;
+  Cplx<T>* z = reinterpret_cast<Cplx<T>*>(smem_raw) + wave * a.nx;
+  const long job = (long)blockIdx.x * (blockDim.x / LES_WAVE) + wave;
+  if (job >= (long)a.nplanes * (a.ny / 2)) return;
+  if (pl.n == 256) {
+    TwReg<T, 256> tr;
+    tw_load(tr, tw + pl.twf, lane);
+    rows_r2c_body(a, z, lane, job, [&](Cplx<T>* p, bool inv) { wave_fft_t(p, tr, inv, lane); });
+  } else {
+    rows_r2c_body(a, z, lane, job, [&](Cplx<T>* p, bool inv) { wave_fft(p, pl, tw, inv, lane); });
   }
 }
 
@@ -75,15 +86,10 @@ struct RowC2RArgs {
   int k0, nplanes;
 };
 
-template <typename T, typename TO>
-__global__ void __launch_bounds__(LES_BLOCK) k_rows_c2r(RowC2RArgs<T, TO> a, FftPlan pl, const Cplx<T>* __restrict__ tw) {
-  extern __shared__ unsigned char smem_raw[];
+template <typename T, typename TO, typename FFT>
+__device__ __forceinline__ void rows_c2r_body(const RowC2RArgs<T, TO>& a, Cplx<T>* z, int lane, long job, FFT&& fft) {
   const int nx = a.nx, lhx = a.lhx, h = nx / 2;
-  const int lane = threadIdx.x & (LES_WAVE - 1), wave = threadIdx.x / LES_WAVE;
-  Cplx<T>* z = reinterpret_cast<Cplx<T>*>(smem_raw) + wave * nx;
   const int npairs = a.ny / 2;
-  const long job = (long)blockIdx.x * (blockDim.x / LES_WAVE) + wave;
-  if (job >= (long)a.nplanes * npairs) return;
   const int f = blockIdx.y;
   const int k = a.k0 + (int)(job / npairs), jp = (int)(job % npairs);
   const Cplx<T>* __restrict__ src = a.src[f] + ((long)k * a.ny + 2 * jp) * lhx;
@@ -97,12 +103,28 @@ __global__ void __launch_bounds__(LES_BLOCK) k_rows_c2r(RowC2RArgs<T, TO> a, Fft
     if (kk != 0 && kk != h) z[nx - kk] = {A.x + B.y, B.x - A.y};
   }
   wave_sync();
-  wave_fft(z, pl, tw, true, lane);
+  fft(z, true);
   TO* __restrict__ dst = a.dst[f] + ((long)k * a.ny + 2 * jp) * nx;
   for (int n = lane; n < nx; n += LES_WAVE) {
     const Cplx<T> v = z[n];
     dst[n] = (TO)v.x;
     dst[n + nx] = (TO)v.y;
+  }
+}
+
+template <typename T, typename TO>
+__global__ void __launch_bounds__(LES_BLOCK) k_rows_c2r(RowC2RArgs<T, TO> a, FftPlan pl, const Cplx<T>* __restrict__ tw) {
+  extern __shared__ unsigned char smem_raw[];
+  const int lane = threadIdx.x & (LES_WAVE - 1), wave = threadIdx.x / LES_WAVE;
+  Cplx<T>* z = reinterpret_cast<Cplx<T>*>(smem_raw) + wave * a.nx;
+  const long job = (long)blockIdx.x * (blockDim.x / LES_WAVE) + wave;
+  if (job >= (long)a.nplanes * (a.ny / 2)) return;
+  if (pl.n == 256) {
+    TwReg<T, 256> tr;
+    tw_load(tr, tw + pl.twf, lane);
+    rows_c2r_body(a, z, lane, job, [&](Cplx<T>* p, bool inv) { wave_fft_t(p, tr, inv, lane); });
+  } else {
+    rows_c2r_body(a, z, lane, job, [&](Cplx<T>* p, bool inv) { wave_fft(p, pl, tw, inv, lane); });
   }
 }
 
@@ -156,15 +178,10 @@ __device__ __forceinline__ Cplx<T> col_op(const ColArgs<T>& a, int o, int kx, in
   }
 }
 
-// A tile of CB adjacent kx columns of one plane: loaded row by row (CB
-// contiguous complex values per row), transposed into LDS so every column is
-// a contiguous line, transformed along y by one wavefront per column,
-// combined by the spectral op and transformed back, then stored row by row.
-template <typename T, int CB>
-__global__ void __launch_bounds__(LES_BLOCK) k_cols(ColArgs<T> a, FftPlan pl, const Cplx<T>* __restrict__ tw) {
-  extern __shared__ unsigned char smem_raw[];
+template <typename T, int CB, typename FFT>
+__device__ __forceinline__ void cols_body(const ColArgs<T>& a, unsigned char* smem, FFT&& fft) {
   const int ny = a.ny, LS = a.ls;
-  Cplx<T>* X0 = reinterpret_cast<Cplx<T>*>(smem_raw);
+  Cplx<T>* X0 = reinterpret_cast<Cplx<T>*>(smem);
   Cplx<T>* X1 = X0 + CB * LS;
   Cplx<T>* Y = X0 + (a.nin > 1 ? 2 : 1) * CB * LS;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -185,8 +202,8 @@ __global__ void __launch_bounds__(LES_BLOCK) k_cols(ColArgs<T> a, FftPlan pl, co
   const int lane = threadIdx.x & (LES_WAVE - 1), wave = threadIdx.x / LES_WAVE, nw = blockDim.x / LES_WAVE;
   if (a.fwd) {
     for (int c = wave; c < nc; c += nw) {
-      wave_fft(X0 + c * LS, pl, tw, false, lane);
-      if (a.nin > 1) wave_fft(X1 + c * LS, pl, tw, false, lane);
+      fft(X0 + c * LS, false);
+      if (a.nin > 1) fft(X1 + c * LS, false);
     }
   }
   for (int o = 0; o < a.nout; ++o) {
@@ -196,7 +213,7 @@ __global__ void __launch_bounds__(LES_BLOCK) k_cols(ColArgs<T> a, FftPlan pl, co
       Cplx<T>* y = Y + c * LS;
       for (int j = lane; j < ny; j += LES_WAVE) y[j] = col_op(a, o, c0 + c, j, x0[j], x1[j]);
       wave_sync();
-      if (a.inv) wave_fft(y, pl, tw, true, lane);
+      if (a.inv) fft(y, true);
     }
     __syncthreads();
     Cplx<T>* __restrict__ dst = a.out[o] + pbase;
@@ -205,6 +222,24 @@ __global__ void __launch_bounds__(LES_BLOCK) k_cols(ColArgs<T> a, FftPlan pl, co
       if (c < nc) dst[(long)j * a.lhx + c] = Y[c * LS + j];
     }
     __syncthreads();
+  }
+}
+
+// A tile of CB adjacent kx columns of one plane: loaded row by row (CB
+// contiguous complex values per row), transposed into LDS so every column is
+// a contiguous line, transformed along y by one wavefront per column,
+// combined by the spectral op and transformed back, then stored row by row.
+// Length 256 runs on register twiddles.
+template <typename T, int CB>
+__global__ void __launch_bounds__(LES_BLOCK) k_cols(ColArgs<T> a, FftPlan pl, const Cplx<T>* __restrict__ tw) {
+  extern __shared__ unsigned char smem_raw[];
+  const int lane = threadIdx.x & (LES_WAVE - 1);
+  if (pl.n == 256) {
+    TwReg<T, 256> tr;
+    tw_load(tr, tw + pl.twf, lane);
+    cols_body<T, CB>(a, smem_raw, [&](Cplx<T>* p, bool inv) { wave_fft_t(p, tr, inv, lane); });
+  } else {
+    cols_body<T, CB>(a, smem_raw, [&](Cplx<T>* p, bool inv) { wave_fft(p, pl, tw, inv, lane); });
   }
 }
 
@@ -254,17 +289,11 @@ struct RowResArgs {
   long src_plane, dst_plane;  // plane strides (elements)
 };
 
-template <typename T, typename TI, typename TO>
-__global__ void __launch_bounds__(LES_BLOCK) k_rows_resample(RowResArgs<T, TI, TO> a, FftPlan p1, FftPlan p2,
-                                                             const Cplx<T>* __restrict__ tw) {
-  extern __shared__ unsigned char smem_raw[];
+template <typename T, typename TI, typename TO, typename RES>
+__device__ __forceinline__ void rows_resample_body(const RowResArgs<T, TI, TO>& a, Cplx<T>* z, int lane, long job,
+                                                   RES&& res) {
   const int n1 = a.n1, n2 = a.n2;
-  const int nm = n1 > n2 ? n1 : n2;
-  const int lane = threadIdx.x & (LES_WAVE - 1), wave = threadIdx.x / LES_WAVE;
-  Cplx<T>* z = reinterpret_cast<Cplx<T>*>(smem_raw) + wave * nm;
   const int npairs = a.ny / 2;
-  const long job = (long)blockIdx.x * (blockDim.x / LES_WAVE) + wave;
-  if (job >= (long)a.nplanes * npairs) return;
   const int f = blockIdx.y;
   const int k = a.k0 + (int)(job / npairs), jp = (int)(job % npairs);
   const long np = (long)n1 * a.ny;
@@ -278,15 +307,56 @@ __global__ void __launch_bounds__(LES_BLOCK) k_rows_resample(RowResArgs<T, TI, T
     for (int n = lane; n < n1; n += LES_WAVE) z[n] = {src[n], src[n + n1]};
   }
   wave_sync();
-  wave_fft(z, p1, tw, false, lane);
-  wave_remap(z, n1, n2, lane);
-  wave_fft(z, p2, tw, true, lane);
+  res(z);
   TO* __restrict__ dst = a.dst[f] + (long)k * a.dst_plane + (long)(2 * jp) * n2;
   for (int n = lane; n < n2; n += LES_WAVE) {
     const Cplx<T> v = z[n];
     dst[n] = (TO)v.x;
     dst[n + n2] = (TO)v.y;
   }
+}
+
+// The FP32 256 <-> 384 pair runs on register twiddles (37 per lane).
+template <typename T, typename TI, typename TO>
+__global__ void __launch_bounds__(LES_BLOCK) k_rows_resample(RowResArgs<T, TI, TO> a, FftPlan p1, FftPlan p2,
+                                                             const Cplx<T>* __restrict__ tw) {
+  extern __shared__ unsigned char smem_raw[];
+  const int nm = a.n1 > a.n2 ? a.n1 : a.n2;
+  const int lane = threadIdx.x & (LES_WAVE - 1), wave = threadIdx.x / LES_WAVE;
+  Cplx<T>* z = reinterpret_cast<Cplx<T>*>(smem_raw) + wave * nm;
+  const long job = (long)blockIdx.x * (blockDim.x / LES_WAVE) + wave;
+  if (job >= (long)a.nplanes * (a.ny / 2)) return;
+  if constexpr (sizeof(T) == 4) {
+    if (p1.n == 256 && p2.n == 384) {
+      TwReg<T, 256> ta;
+      TwReg<T, 384> tb;
+      tw_load(ta, tw + p1.twf, lane);
+      tw_load(tb, tw + p2.twf, lane);
+      rows_resample_body(a, z, lane, job, [&](Cplx<T>* q) {
+        wave_fft_t(q, ta, false, lane);
+        wave_remap(q, 256, 384, lane);
+        wave_fft_t(q, tb, true, lane);
+      });
+      return;
+    }
+    if (p1.n == 384 && p2.n == 256) {
+      TwReg<T, 384> ta;
+      TwReg<T, 256> tb;
+      tw_load(ta, tw + p1.twf, lane);
+      tw_load(tb, tw + p2.twf, lane);
+      rows_resample_body(a, z, lane, job, [&](Cplx<T>* q) {
+        wave_fft_t(q, ta, false, lane);
+        wave_remap(q, 384, 256, lane);
+        wave_fft_t(q, tb, true, lane);
+      });
+      return;
+    }
+  }
+  rows_resample_body(a, z, lane, job, [&](Cplx<T>* q) {
+    wave_fft(q, p1, tw, false, lane);
+    wave_remap(q, a.n1, a.n2, lane);
+    wave_fft(q, p2, tw, true, lane);
+  });
 }
 
 // Column resampling: column pairs (i, i+1) of a real (W, h1) plane -> (W, h2).
@@ -304,12 +374,10 @@ struct ColResArgs {
   int ncp;  // column-pair tiles per plane
 };
 
-template <typename T, int CP>
-__global__ void __launch_bounds__(LES_BLOCK) k_cols_resample(ColResArgs<T> a, FftPlan p1, FftPlan p2,
-                                                             const Cplx<T>* __restrict__ tw) {
-  extern __shared__ unsigned char smem_raw[];
+template <typename T, int CP, typename RES>
+__device__ __forceinline__ void cols_resample_body(const ColResArgs<T>& a, unsigned char* smem, RES&& res) {
   const int h1 = a.h1, h2 = a.h2, LS = a.ls;
-  Cplx<T>* buf = reinterpret_cast<Cplx<T>*>(smem_raw);
+  Cplx<T>* buf = reinterpret_cast<Cplx<T>*>(smem);
   const int f = blockIdx.y;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int k = a.k0 + tile / a.ncp;
@@ -327,13 +395,8 @@ __global__ void __launch_bounds__(LES_BLOCK) k_cols_resample(ColResArgs<T> a, Ff
     }
   }
   __syncthreads();
-  const int lane = threadIdx.x & (LES_WAVE - 1), wave = threadIdx.x / LES_WAVE, nw = blockDim.x / LES_WAVE;
-  for (int l = wave; l < nl; l += nw) {
-    Cplx<T>* z = buf + l * LS;
-    wave_fft(z, p1, tw, false, lane);
-    wave_remap(z, h1, h2, lane);
-    wave_fft(z, p2, tw, true, lane);
-  }
+  const int wave = threadIdx.x / LES_WAVE, nw = blockDim.x / LES_WAVE;
+  for (int l = wave; l < nl; l += nw) res(buf + l * LS);
   __syncthreads();
   T* __restrict__ dst = a.dst[f] + (long)k * a.dst_plane + 2 * p0;
   for (int idx = threadIdx.x; idx < 2 * CP * h2; idx += blockDim.x) {
@@ -344,4 +407,42 @@ __global__ void __launch_bounds__(LES_BLOCK) k_cols_resample(ColResArgs<T> a, Ff
       dst[(long)j * a.W + r] = (r & 1) ? z.y : z.x;
     }
   }
+}
+
+template <typename T, int CP>
+__global__ void __launch_bounds__(LES_BLOCK) k_cols_resample(ColResArgs<T> a, FftPlan p1, FftPlan p2,
+                                                             const Cplx<T>* __restrict__ tw) {
+  extern __shared__ unsigned char smem_raw[];
+  const int lane = threadIdx.x & (LES_WAVE - 1);
+  if constexpr (sizeof(T) == 4) {
+    if (p1.n == 256 && p2.n == 384) {
+      TwReg<T, 256> ta;
+      TwReg<T, 384> tb;
+      tw_load(ta, tw + p1.twf, lane);
+      tw_load(tb, tw + p2.twf, lane);
+      cols_resample_body<T, CP>(a, smem_raw, [&](Cplx<T>* q) {
+        wave_fft_t(q, ta, false, lane);
+        wave_remap(q, 256, 384, lane);
+        wave_fft_t(q, tb, true, lane);
+      });
+      return;
+    }
+    if (p1.n == 384 && p2.n == 256) {
+      TwReg<T, 384> ta;
+      TwReg<T, 256> tb;
+      tw_load(ta, tw + p1.twf, lane);
+      tw_load(tb, tw + p2.twf, lane);
+      cols_resample_body<T, CP>(a, smem_raw, [&](Cplx<T>* q) {
+        wave_fft_t(q, ta, false, lane);
+        wave_remap(q, 384, 256, lane);
+        wave_fft_t(q, tb, true, lane);
+      });
+      return;
+    }
+  }
+  cols_resample_body<T, CP>(a, smem_raw, [&](Cplx<T>* q) {
+    wave_fft(q, p1, tw, false, lane);
+    wave_remap(q, a.h1, a.h2, lane);
+    wave_fft(q, p2, tw, true, lane);
+  });
 }

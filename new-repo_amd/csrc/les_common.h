@@ -82,18 +82,30 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <typename T, int R>
+// k = i mod p of butterfly i: KM 0 mask (p a power of two: every radix-4
+// and radix-2 stage, which precede the radix-3 ones, and the first radix-3
+// stage), KM 1 identity (last stage: i < n/R = p), KM 2 true modulo (middle
+// radix-3 stages).  A compile-time choice, so no division is ever evaluated
+// where it is not needed.
+template <int KM>
+__device__ __forceinline__ int stage_k(int i, int p) {
+  if constexpr (KM == 0) return i & (p - 1);
+  else if constexpr (KM == 1) return i;
+  else return i % p;
+}
+
+template <typename T, int R, int KM>
 __device__ __forceinline__ void wave_stage(Cplx<T>* z, int n, int p, const Cplx<T>* __restrict__ tw, int sgn, int lane) {
   constexpr int MAXB = (LES_MAX_N / R + LES_WAVE - 1) / LES_WAVE;
   const int m = n / R;
-  const int pm = p - 1;
-  const bool pow2 = (p & pm) == 0;
   Cplx<T> x[MAXB][4];
+  int kk[MAXB];
 #pragma unroll
   for (int t = 0; t < MAXB; ++t) {
     const int i = lane + LES_WAVE * t;
     if (i < m) {
-      const int k = pow2 ? (i & pm) : (i % p);
+      const int k = stage_k<KM>(i, p);
+      kk[t] = k;
       x[t][0] = z[i];
 #pragma unroll
       for (int r = 1; r < R; ++r) x[t][r] = c_mul(z[i + r * m], tw[(r - 1) * p + k]);
@@ -104,7 +116,7 @@ __device__ __forceinline__ void wave_stage(Cplx<T>* z, int n, int p, const Cplx<
   for (int t = 0; t < MAXB; ++t) {
     const int i = lane + LES_WAVE * t;
     if (i < m) {
-      const int k = pow2 ? (i & pm) : (i % p);
+      const int k = kk[t];
       bfly(x[t], R, sgn);
       const int j = (i - k) * R + k;
 #pragma unroll
@@ -122,10 +134,109 @@ __device__ __forceinline__ void wave_fft(Cplx<T>* z, const FftPlan& pl, const Cp
   for (int s = 0; s < pl.nst; ++s) {
     const int R = pl.R[s], p = pl.P[s];
     const Cplx<T>* tw = tw0 + pl.off[s];
-    if (R == 4) wave_stage<T, 4>(z, pl.n, p, tw, sgn, lane);
-    else if (R == 2) wave_stage<T, 2>(z, pl.n, p, tw, sgn, lane);
-    else wave_stage<T, 3>(z, pl.n, p, tw, sgn, lane);
+    if (R == 4) {
+      wave_stage<T, 4, 0>(z, pl.n, p, tw, sgn, lane);
+    } else if (R == 2) {
+      wave_stage<T, 2, 0>(z, pl.n, p, tw, sgn, lane);
+    } else if (p * 3 == pl.n) {
+      wave_stage<T, 3, 1>(z, pl.n, p, tw, sgn, lane);
+    } else if ((p & (p - 1)) == 0) {
+      wave_stage<T, 3, 0>(z, pl.n, p, tw, sgn, lane);
+    } else {
+      wave_stage<T, 3, 2>(z, pl.n, p, tw, sgn, lane);
+    }
   }
+}
+
+// ---------------------------------------------------------------- fixed lengths
+// Compile-time plans for the production lengths, built by the host's rule
+// (radix-4 stages, one radix-2 stage when the power of two is odd, then
+// radix-3 stages).  A wavefront loads its lanes' forward twiddles of every
+// stage into registers once (TwReg), so the stages touch only LDS; inverse
+// transforms use the conjugates, which equal the inverse table bit for bit
+// (the host stores (cos, -sin) and (cos, sin)).
+constexpr int cp_a(int n) { int a = 0; while (n % 2 == 0) { n /= 2; ++a; } return a; }
+constexpr int cp_b(int n) { while (n % 2 == 0) n /= 2; int b = 0; while (n % 3 == 0) { n /= 3; ++b; } return b; }
+constexpr int cp_nst(int n) { return cp_a(n) / 2 + (cp_a(n) & 1) + cp_b(n); }
+constexpr int cp_R(int n, int s) {
+  return s < cp_a(n) / 2 ? 4 : ((cp_a(n) & 1) && s == cp_a(n) / 2 ? 2 : 3);
+}
+constexpr int cp_P(int n, int s) { int p = 1; for (int t = 0; t < s; ++t) p *= cp_R(n, t); return p; }
+constexpr int cp_OFF(int n, int s) { int o = 0; for (int t = 0; t < s; ++t) o += (cp_R(n, t) - 1) * cp_P(n, t); return o; }
+constexpr int cp_MB(int n, int s) { return (n / cp_R(n, s) + LES_WAVE - 1) / LES_WAVE; }
+constexpr int cp_BASE(int n, int s) { int c = 0; for (int t = 0; t < s; ++t) c += cp_MB(n, t) * (cp_R(n, t) - 1); return c; }
+constexpr int cp_NTW(int n) { return cp_BASE(n, cp_nst(n)); }
+
+template <typename T, int N>
+struct TwReg {
+  Cplx<T> w[cp_NTW(N)];
+};
+
+template <typename T, int N, int S>
+__device__ __forceinline__ void tw_load_s(Cplx<T>* w, const Cplx<T>* __restrict__ twf, int lane) {
+  if constexpr (S < cp_nst(N)) {
+    constexpr int R = cp_R(N, S), P = cp_P(N, S), OFF = cp_OFF(N, S), MB = cp_MB(N, S), BASE = cp_BASE(N, S);
+    constexpr int M = N / R;
+#pragma unroll
+    for (int b = 0; b < MB; ++b) {
+      const int i = lane + LES_WAVE * b;
+      if (i < M) {
+        const int k = i % P;
+#pragma unroll
+        for (int r = 1; r < R; ++r) w[BASE + b * (R - 1) + r - 1] = twf[OFF + (r - 1) * P + k];
+      }
+    }
+    tw_load_s<T, N, S + 1>(w, twf, lane);
+  }
+}
+
+// the forward table of the plan of length N (twpool + plan.twf)
+template <typename T, int N>
+__device__ __forceinline__ void tw_load(TwReg<T, N>& t, const Cplx<T>* __restrict__ twf, int lane) {
+  tw_load_s<T, N, 0>(t.w, twf, lane);
+}
+
+template <typename T, int N, int S>
+__device__ __forceinline__ void wave_fft_s(Cplx<T>* z, const Cplx<T>* w, bool inverse, int lane) {
+  if constexpr (S < cp_nst(N)) {
+    constexpr int R = cp_R(N, S), P = cp_P(N, S), MB = cp_MB(N, S), BASE = cp_BASE(N, S);
+    constexpr int M = N / R;
+    const int sgn = inverse ? 1 : -1;
+    Cplx<T> x[MB][4];
+#pragma unroll
+    for (int b = 0; b < MB; ++b) {
+      const int i = lane + LES_WAVE * b;
+      if (i < M) {
+        x[b][0] = z[i];
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          Cplx<T> tw = w[BASE + b * (R - 1) + r - 1];
+          if (inverse) tw.y = -tw.y;
+          x[b][r] = c_mul(z[i + r * M], tw);
+        }
+      }
+    }
+    wave_sync();
+#pragma unroll
+    for (int b = 0; b < MB; ++b) {
+      const int i = lane + LES_WAVE * b;
+      if (i < M) {
+        const int k = i % P;
+        bfly(x[b], R, sgn);
+        const int j = (i - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) z[j + r * P] = x[b][r];
+      }
+    }
+    wave_sync();
+    wave_fft_s<T, N, S + 1>(z, w, inverse, lane);
+  }
+}
+
+// Same stages, arithmetic and order as wave_fft, with register twiddles.
+template <typename T, int N>
+__device__ __forceinline__ void wave_fft_t(Cplx<T>* z, const TwReg<T, N>& t, bool inverse, int lane) {
+  wave_fft_s<T, N, 0>(z, t.w, inverse, lane);
 }
 
 // XCD-aware tile order.  Workgroups are dispatched round-robin over the 8

@@ -124,10 +124,13 @@ def test_operators_match_oracle_individually():
         _compare(g, o, prm.nz, ["p", "dpdx", "dpdy"], tol=0.0)
 
 
-def test_mixed_precision_error_vs_fp64_oracle():
+# 256x256 runs the FP32 transforms of the production lengths (256, 384) on
+# register twiddles; a wrong twiddle there would give O(1) errors
+@pytest.mark.parametrize("grid", [(32, 32, 16), (256, 256, 4)])
+def test_mixed_precision_error_vs_fp64_oracle(grid):
     pkg, O = load_pkg(), load_oracle()
     P = load_params()
-    prm, u, v, w = _setup(32, 32, 16, seed=5)
+    prm, u, v, w = _setup(*grid, seed=5)
     o = O.Oracle(prm)
     o.set_ic(u, v, w)
     prm.precision = P.PREC_MIXED
@@ -138,9 +141,15 @@ def test_mixed_precision_error_vs_fp64_oracle():
         errs = {}
         for n in ("u", "v", "w", "p", "cs_opt2", "beta", "F_LM", "F_MM"):
             errs[n] = rel_l2(_interior(n, g.download(n), prm.nz), _interior(n, o.get(n), prm.nz))
-        print("mixed vs FP64 oracle after 12 steps:", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+        # Cs^2 on the lid (level nz+1), where |S| is at round-off level and the
+        # dynamic ratio is ill-conditioned, is reported apart
+        nz = prm.nz
+        cs_g, cs_o = g.download("cs_opt2"), o.get("cs_opt2")
+        errs["cs_opt2_1..nz"] = rel_l2(cs_g[1:nz + 1], cs_o[1:nz + 1])
+        errs["cs_opt2_lid"] = rel_l2(cs_g[nz + 1], cs_o[nz + 1])
+        print(f"mixed vs FP64 oracle after 12 steps at {grid}:", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
         assert errs["u"] < 1e-6 and errs["v"] < 1e-4 and errs["w"] < 1e-4
-        assert errs["p"] < 1e-3 and errs["cs_opt2"] < 1e-2
+        assert errs["p"] < 1e-3 and errs["cs_opt2_1..nz"] < 1e-2
 
 
 def test_nan_is_reported():
