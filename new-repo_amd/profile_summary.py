@@ -1,6 +1,7 @@
 """Summaries of rocprofv3 CSV output for the benchmark's evidence.
 
 kernels DIR            per-kernel table from <DIR>/**/*kernel_stats.csv
+report DIR             markdown report of a kernel trace (see --help)
 traffic FETCH WRITE    HBM bytes per launch of each kernel family from two
                        counter passes (FETCH_SIZE, WRITE_SIZE, KiB per
                        dispatch), each scaled by what the same counter reads
@@ -113,6 +114,38 @@ def traffic(args):
     return out
 
 
+def report(args):
+    """Markdown kernel-trace report: top kernels, HIP-event family averages
+    of the traced process beside rocprofv3's, and the step figures of a
+    default benchmark line."""
+    text, tab = kernels(args.trace, args.top)
+    lines = [ln for ln in pathlib.Path(args.trace_json).read_text().splitlines() if ln.startswith("{")]
+    tj = json.loads(lines[-1])
+    fam = tj["kernel_families"]
+    rows = []
+    for f, pat in FAMILY_KERNELS.items():
+        if f in fam:
+            c = sum(t[1] for t in tab if pat in t[0])
+            tot = sum(t[2] for t in tab if pat in t[0])
+            if c:
+                rows.append(f"| {f} | `{pat}` | {fam[f]['avg_ms']:.4f} | {tot / c / 1e6:.4f} | {c} |")
+    bb = json.loads(pathlib.Path(args.bench_json).read_text().splitlines()[-1])
+    steps = [f"| {k} | {v['ms_per_step']:.3f} | {v['mpts']:.0f} |" for k, v in bb["variants"].items() if "mpts" in v]
+    r = bb["roofline"]
+    out = [f"# {args.title}", "", "Commands (repository root, GPU box):", "", "```"] + args.command + ["```", "",
+           args.workload, "", "## Top kernels (rocprofv3 --stats)", "", text, "",
+           "## HIP-event family averages (same traced process) vs rocprofv3", "",
+           "bench.py times every launch of its 5-step cycle with HIP events on the library's stream; "
+           "rocprofv3 averages all launches of the same kernels over the whole trace.", "",
+           "| family | kernels | bench avg ms | rocprof avg ms | rocprof launches |", "|---|---|---|---|---|"] + rows + [
+           "", "## Default benchmark line (`python3 bench.py`, separate run)", "",
+           "| variant | ms/step | Mpt-steps/s |", "|---|---|---|"] + steps + [
+           "", f"Dominant single kernel: `{r['kernel_family']}` at {r['achieved']:.0f} GB/s algorithmic "
+           f"({r['frac']:.3f} of 8 TB/s); whole step {bb['step_roofline']['achieved_GBs']:.0f} GB/s "
+           f"({bb['step_roofline']['frac']:.3f}) at {bb['step_roofline']['bytes_per_point_step']} B per point-step.", ""]
+    return "\n".join(out)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -128,9 +161,23 @@ def main(argv=None):
     t.add_argument("--variant", required=True)
     t.add_argument("--command", action="append", default=[])
     t.add_argument("--out")
+    rp = sub.add_parser("report")
+    rp.add_argument("trace")
+    rp.add_argument("--trace-json", required=True, help="stdout of the traced bench.py run")
+    rp.add_argument("--bench-json", required=True, help="stdout of a default bench.py run")
+    rp.add_argument("--title", required=True)
+    rp.add_argument("--workload", required=True)
+    rp.add_argument("--command", action="append", default=[])
+    rp.add_argument("--top", type=int, default=30)
+    rp.add_argument("--out")
     a = ap.parse_args(argv)
     if a.cmd == "kernels":
         text, _ = kernels(a.dir, a.top)
+        print(text)
+    elif a.cmd == "report":
+        text = report(a)
+        if a.out:
+            pathlib.Path(a.out).write_text(text)
         print(text)
     else:
         res = traffic(a)
