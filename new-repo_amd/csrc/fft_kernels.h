@@ -147,6 +147,7 @@ struct ColArgs {
   int nin, nout;
   int op;
   int fwd, inv;  // apply forward y transform to inputs / inverse to outputs
+  int ntiles;    // ncb * nplanes
 };
 
 template <typename T>
@@ -225,22 +226,112 @@ __device__ __forceinline__ void cols_body(const ColArgs<T>& a, unsigned char* sm
   }
 }
 
+// Persistent, software-pipelined version for ny = 256 (tiles of CB columns,
+// E = CB * 256 / LES_BLOCK elements per thread and input): the block walks a
+// contiguous run of tiles of its XCD (stride = blocks of that XCD, so the
+// concurrently running blocks of one XCD take adjacent tiles) and loads the
+// next tile into registers while it transforms the current one.
+template <typename T, int CB, typename FFT>
+__device__ __forceinline__ void cols_body_pipe(const ColArgs<T>& a, unsigned char* smem, FFT&& fft) {
+  constexpr int NY = 256, E = CB * NY / LES_BLOCK;
+  static_assert(E * LES_BLOCK == CB * NY, "tile must split evenly over the block");
+  const int LS = a.ls;
+  Cplx<T>* X0 = reinterpret_cast<Cplx<T>*>(smem);
+  Cplx<T>* X1 = X0 + CB * LS;
+  Cplx<T>* Y = X0 + (a.nin > 1 ? 2 : 1) * CB * LS;
+  const int x = blockIdx.x % LES_NXCD, m = blockIdx.x / LES_NXCD, nb = gridDim.x / LES_NXCD;
+  const int q = a.ntiles / LES_NXCD, r = a.ntiles % LES_NXCD;
+  const int tbeg = x * q + (x < r ? x : r), tend = tbeg + q + (x < r ? 1 : 0);
+  int t = tbeg + m;
+  if (t >= tend) return;
+  const int lane = threadIdx.x & (LES_WAVE - 1), wave = threadIdx.x / LES_WAVE, nw = blockDim.x / LES_WAVE;
+  Cplx<T> pre[2][E];
+  auto fetch = [&](int tt) {
+    const int k = a.k0 + tt / a.ncb, c0 = (tt % a.ncb) * CB;
+    int nc = a.lhx - c0;
+    if (nc > CB) nc = CB;
+    const long pb = (long)k * NY * a.lhx + c0;
+#pragma unroll
+    for (int in = 0; in < 2; ++in) {
+      if (in < a.nin) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const int idx = threadIdx.x + e * LES_BLOCK, j = idx / CB, c = idx % CB;
+          pre[in][e] = c < nc ? a.in[in][pb + (long)j * a.lhx + c] : Cplx<T>{(T)0, (T)0};
+        }
+      }
+    }
+  };
+  fetch(t);
+  while (true) {
+    const int k = a.k0 + t / a.ncb, c0 = (t % a.ncb) * CB;
+    int nc = a.lhx - c0;
+    if (nc > CB) nc = CB;
+    const long pbase = (long)k * NY * a.lhx + c0;
+#pragma unroll
+    for (int in = 0; in < 2; ++in) {
+      if (in < a.nin) {
+        Cplx<T>* X = in == 0 ? X0 : X1;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const int idx = threadIdx.x + e * LES_BLOCK, j = idx / CB, c = idx % CB;
+          X[c * LS + j] = pre[in][e];
+        }
+      }
+    }
+    block_sync_lds();
+    const int tn = t + nb;
+    if (tn < tend) fetch(tn);  // in flight during this tile's transforms
+    if (a.fwd) {
+      for (int c = wave; c < nc; c += nw) {
+        fft(X0 + c * LS, false);
+        if (a.nin > 1) fft(X1 + c * LS, false);
+      }
+    }
+    for (int o = 0; o < a.nout; ++o) {
+      for (int c = wave; c < nc; c += nw) {
+        const Cplx<T>* x0 = X0 + c * LS;
+        const Cplx<T>* x1 = (a.nin > 1 ? X1 : X0) + c * LS;
+        Cplx<T>* y = Y + c * LS;
+        for (int j = lane; j < NY; j += LES_WAVE) y[j] = col_op(a, o, c0 + c, j, x0[j], x1[j]);
+        wave_sync();
+        if (a.inv) fft(y, true);
+      }
+      block_sync_lds();
+      Cplx<T>* __restrict__ dst = a.out[o] + pbase;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int idx = threadIdx.x + e * LES_BLOCK, j = idx / CB, c = idx % CB;
+        if (c < nc) dst[(long)j * a.lhx + c] = Y[c * LS + j];
+      }
+      block_sync_lds();
+    }
+    if (tn >= tend) break;
+    t = tn;
+  }
+}
+
 // A tile of CB adjacent kx columns of one plane: loaded row by row (CB
 // contiguous complex values per row), transposed into LDS so every column is
 // a contiguous line, transformed along y by one wavefront per column,
 // combined by the spectral op and transformed back, then stored row by row.
-// Length 256 runs on register twiddles.
+// Length 256 runs on register twiddles and the pipelined persistent body.
 template <typename T, int CB>
 __global__ void __launch_bounds__(LES_BLOCK) k_cols(ColArgs<T> a, FftPlan pl, const Cplx<T>* __restrict__ tw) {
   extern __shared__ unsigned char smem_raw[];
   const int lane = threadIdx.x & (LES_WAVE - 1);
-  if (pl.n == 256) {
-    TwReg<T, 256> tr;
-    tw_load(tr, tw + pl.twf, lane);
-    cols_body<T, CB>(a, smem_raw, [&](Cplx<T>* p, bool inv) { wave_fft_t(p, tr, inv, lane); });
-  } else {
-    cols_body<T, CB>(a, smem_raw, [&](Cplx<T>* p, bool inv) { wave_fft(p, pl, tw, inv, lane); });
-  }
+  cols_body<T, CB>(a, smem_raw, [&](Cplx<T>* p, bool inv) { wave_fft(p, pl, tw, inv, lane); });
+}
+
+// ny = 256: a kernel of its own, so its register budget is not set by the
+// runtime-plan path (whose radix-2 stages need twice the butterflies).
+template <typename T, int CB>
+__global__ void __launch_bounds__(LES_BLOCK) k_cols_256(ColArgs<T> a, FftPlan pl, const Cplx<T>* __restrict__ tw) {
+  extern __shared__ unsigned char smem_raw[];
+  const int lane = threadIdx.x & (LES_WAVE - 1);
+  TwReg<T, 256> tr;
+  tw_load(tr, tw + pl.twf, lane);
+  cols_body_pipe<T, CB>(a, smem_raw, [&](Cplx<T>* p, bool inv) { wave_fft_t(p, tr, inv, lane); });
 }
 
 // ------------------------------------------------------------------ 3/2 resampling

@@ -239,6 +239,13 @@ __device__ __forceinline__ void wave_fft_t(Cplx<T>* z, const TwReg<T, N>& t, boo
   wave_fft_s<T, N, 0>(z, t.w, inverse, lane);
 }
 
+// Block barrier for LDS exchange only: waits for the wavefront's LDS
+// operations, not for its outstanding global loads (a prefetch stays in
+// flight across it, unlike __syncthreads, whose workgroup fence can wait for
+// them).  The memory clobber keeps the compiler from moving memory accesses
+// across it.
+__device__ __forceinline__ void block_sync_lds() { __asm__ volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 // XCD-aware tile order.  Workgroups are dispatched round-robin over the 8
 // XCDs (block b on XCD b % 8), each with its own L2.  Tiles that share
 // 128-byte lines (adjacent column tiles of narrower rows) must run on one XCD

@@ -165,10 +165,26 @@ static int col_tile(int nin) {
   return cb < 4 ? 4 : cb;
 }
 
+// ny = 256 runs the persistent pipelined kernel body: a grid of (resident
+// blocks per CU) x (CUs), a multiple of the 8 XCDs; other lengths one block
+// per tile.
 template <typename T, int CB>
 static void launch_cols(Les& L, const ColArgs<T>& a, int npl) {
   const size_t sh = (size_t)(a.nin + 1) * CB * a.ls * sizeof(Cplx<T>);
-  hipLaunchKernelGGL((k_cols<T, CB>), dim3((unsigned)(a.ncb * npl)), dim3(LES_BLOCK), sh, L.stream, a, L.py, twpool<T>(L));
+  if (L.ny == 256) {
+    static int occ[2] = {0, 0};  // per input count
+    int& o = occ[a.nin > 1];
+    if (o == 0) {
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_cols_256<T, CB>, LES_BLOCK, sh) != hipSuccess || o < 1) o = 1;
+    }
+    long g = (long)o * L.ncu;
+    long need = ((long)a.ntiles + LES_NXCD - 1) / LES_NXCD * LES_NXCD;
+    if (g > need) g = need;
+    const unsigned grid = (unsigned)((g + LES_NXCD - 1) / LES_NXCD * LES_NXCD);
+    hipLaunchKernelGGL((k_cols_256<T, CB>), dim3(grid), dim3(LES_BLOCK), sh, L.stream, a, L.py, twpool<T>(L));
+  } else {
+    hipLaunchKernelGGL((k_cols<T, CB>), dim3((unsigned)a.ntiles), dim3(LES_BLOCK), sh, L.stream, a, L.py, twpool<T>(L));
+  }
 }
 
 template <typename T>
@@ -186,6 +202,7 @@ static void cols(Les& L, int op, const Cplx<T>* in0, const Cplx<T>* in1, Cplx<T>
   a.nout = nout; a.op = op; a.fwd = fwd; a.inv = inv;
   const int cb = col_tile<T>(a.nin);
   a.ncb = (L.lhx + cb - 1) / cb;
+  a.ntiles = a.ncb * npl;
   kt_begin(L, KT_COLS);
   if (cb == 4) launch_cols<T, 4>(L, a, npl);
   else if (cb == 8) launch_cols<T, 8>(L, a, npl);
@@ -483,6 +500,8 @@ static int setup_kernel_attrs() {
   hipFuncSetAttribute((const void*)k_rows_c2r<double, double>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
   hipFuncSetAttribute((const void*)k_rows_c2r<float, float>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
   hipFuncSetAttribute((const void*)k_rows_c2r<float, double>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+  hipFuncSetAttribute((const void*)k_cols_256<double, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+  hipFuncSetAttribute((const void*)k_cols_256<float, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
   hipFuncSetAttribute((const void*)k_cols<double, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
   hipFuncSetAttribute((const void*)k_cols<double, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
   hipFuncSetAttribute((const void*)k_cols<float, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
@@ -551,6 +570,13 @@ int les_init(const les_params* p) {
   L->F_coeff_init = P.lagran_init != 0;
   L->mixed = P.precision != 0;
   if (hipStreamCreate(&L->stream) != hipSuccess) { delete L; les_set_error("stream"); return 5; }
+  {
+    int dev = 0;
+    L->ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&L->ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        L->ncu < 1)
+      L->ncu = 256;
+  }
   for (int i = 0; i < KT_COUNT; ++i) { hipEventCreate(&L->ev_a[i]); hipEventCreate(&L->ev_b[i]); }
   for (int i = 0; i < 6; ++i) hipEventCreate(&L->ph_ev[i]);
   g_les = L;

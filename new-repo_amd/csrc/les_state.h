@@ -61,6 +61,7 @@ struct Les {
   struct MlpState* mlp;
   double judge[4];        // rel. L2 of Cs^2 and beta vs the plain procedure, -, step
   int nband;              // levels 1..nband (z / delta < learned_zmin) keep the plain procedure
+  int ncu;                // compute units of the device (persistent grids)
 
   // timing
   int timing;
