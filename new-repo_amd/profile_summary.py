@@ -22,6 +22,7 @@ import argparse
 import csv
 import json
 import pathlib
+import re
 import sys
 
 
@@ -69,9 +70,10 @@ def _per_dispatch(d: str, counter: str):
 
 
 # kernel-name patterns of the benchmark's kernel families
-FAMILY_KERNELS = {"rows_r2c": "k_rows_r2c<", "rows_c2r": "k_rows_c2r<", "cols": "k_cols<",
-                  "rows_resample": "k_rows_resample<", "cols_resample": "k_cols_resample<",
-                  "tridiag": "k_tridiag", "lagrangian": "k_lagr<"}
+FAMILY_KERNELS = {"rows_r2c": r"\bk_rows_r2c<", "rows_c2r": r"\bk_rows_c2r<",
+                  "cols": r"\bk_cols(_256)?<", "rows_resample": r"\bk_rows_resample<",
+                  "cols_resample": r"\bk_cols_resample<", "tridiag": r"\bk_tridiag",
+                  "lagrangian": r"\bk_lagr<"}
 
 
 def traffic(args):
@@ -98,8 +100,8 @@ def traffic(args):
     for name, pat in FAMILY_KERNELS.items():
         if name not in fam:
             continue
-        f = [x for n, x in fd if pat in n]
-        w = [x for n, x in wd if pat in n]
+        f = [x for n, x in fd if re.search(pat, n)]
+        w = [x for n, x in wd if re.search(pat, n)]
         if not f or not w:
             continue
         calls = fam[name]["calls"]
@@ -125,8 +127,8 @@ def report(args):
     rows = []
     for f, pat in FAMILY_KERNELS.items():
         if f in fam:
-            c = sum(t[1] for t in tab if pat in t[0])
-            tot = sum(t[2] for t in tab if pat in t[0])
+            c = sum(t[1] for t in tab if re.search(pat, t[0]))
+            tot = sum(t[2] for t in tab if re.search(pat, t[0]))
             if c:
                 rows.append(f"| {f} | `{pat}` | {fam[f]['avg_ms']:.4f} | {tot / c / 1e6:.4f} | {c} |")
     bb = json.loads(pathlib.Path(args.bench_json).read_text().splitlines()[-1])
