@@ -144,6 +144,39 @@ int les_projection(double tadv1);
 int les_fft2d_r2c(const void* d_in, void* d_out, int nplanes, double scale);
 int les_fft2d_c2r(const void* d_in, void* d_out, int nplanes);
 
+/* --- running statistics and diagnostics (post-step, outside the step) ----- */
+#define LES_NSTAT 26
+#define LES_NPROF 21
+/* io_stat_init (src/IO_statistics.f90:40-244): allocates the running sums on
+ * the first call, then zeroes them and the step count. */
+int les_stat_init(void);
+/* One call of io_stat_averages' accumulation (src/IO_statistics.f90:266-293,
+ * 300-324, 363-440): FP64 running sums over levels 1..nz of, in the order of
+ * RAV_<jt>_3D.dat, u v w p u2 v2 w2 u3 v3 w3 u4 v4 w4 uv uw vw txx tyy tzz
+ * txy txz tyz dudz dvdz nu_t sqrt(cs_opt2) (uw, vw with u, v averaged to the
+ * w node, zero at level 1), and of u* (RAV_<jt>_2D.dat). */
+int les_stat_accumulate(void);
+/* Host copies: sums3d = LES_NSTAT x (nx, ny, nz) float64 in the file's order
+ * (may be NULL), ustar2d = (nx, ny) (may be NULL), *iterations = the number
+ * of accumulated steps (iterations_av, RAV_<jt>_info.txt). */
+int les_stat_download(double* sums3d, double* ustar2d, int* iterations);
+/* Zeroes the sums and the step count, as after a RAV write
+ * (src/IO_statistics.f90:286,351,501-509). */
+int les_stat_reset(void);
+/* Plane-mean profiles of the time averages (sum / steps, SGS stresses
+ * negated), get_data_struct_RAV + get_profile_struct + compute_Cij
+ * (matlab_src/get_data_struct_RAV.m:21-35; matlab_src/get_profile_struct.m:9-59;
+ * matlab_src/compute_Cij.m:1-36).  prof = LES_NPROF rows of nz levels:
+ * u v w p var_u var_v var_w txx tyy tzz R13 txz St13 R23 tyz St23 tke cs nut
+ * C13 C23 (R_i3 = <u_i w> - u_i(w node) w; St_i3 = R_i3 + tau_i3 + C_i3;
+ * tke = (var_u + var_v + var_w on the uvp node) / 2).  Error 1 before any
+ * accumulation. */
+int les_stat_profiles(double* prof);
+/* compute_running_diagnistics (src/running_diagnostics.f90:80-305) of the
+ * current state: out[0] CFL, out[1] MKE, out[2] plane-mean u*, out[3] rms of
+ * du/dx + dv/dy + dw/dz over levels 1..nz. */
+int les_running_diagnostics(double* out);
+
 /* --- learned SGS surrogate (no counterpart in the reference) -------------- */
 /* A per-point MLP 12 -> 64 -> 64 -> 64 -> 2 (GELU, BF16 MFMA, FP32
  * accumulation) predicting the Lagrangian dynamic model's Cs^2 and beta from

@@ -26,6 +26,12 @@ FIELDS = ["u", "v", "w", "dudx", "dudy", "dudz", "dvdx", "dvdy", "dvdz", "dwdx",
           "beta", "u_lag", "v_lag", "w_lag", "s11", "s12", "s13", "s22", "s23", "s33"]
 FID: Dict[str, int] = {n: i for i, n in enumerate(FIELDS)}
 LAYERS = {"lbc_u": 0, "lbc_v": 1, "ustar": 2, "bottom_wk": 3, "top_wk": 4, "z0": 5}
+# running sums, in the order of the reference's RAV_<jt>_3D.dat
+STATS = ["u", "v", "w", "p", "u2", "v2", "w2", "u3", "v3", "w3", "u4", "v4", "w4", "uv", "uw", "vw",
+         "txx", "tyy", "tzz", "txy", "txz", "tyz", "dudz", "dvdz", "nut", "cs"]
+# rows of les_stat_profiles
+PROFILES = ["u", "v", "w", "p", "var_u", "var_v", "var_w", "txx", "tyy", "tzz", "R13", "txz", "St13",
+            "R23", "tyz", "St23", "tke", "cs", "nut", "C13", "C23"]
 
 STATUS = {0: "ok", 1: "invalid option or model", 3: "out of memory", 4: "double initialisation",
           5: "HIP runtime error", 9: "NaN or singular solve", 10: "bad shape or id"}
@@ -89,6 +95,8 @@ def library() -> ctypes.CDLL:
             "les_mlp_grad": [vp, vp, ctypes.c_long, ctypes.POINTER(ctypes.c_float), dp],
             "les_mlp_train_step": [vp, vp, ctypes.c_long, ctypes.c_double, dp],
             "les_mlp_forward": [vp, ctypes.c_long, vp],
+            "les_stat_init": [], "les_stat_reset": [], "les_stat_accumulate": [],
+            "les_stat_download": [dp, dp, ip], "les_stat_profiles": [dp], "les_running_diagnostics": [dp],
         }
         for name, args in sig.items():
             f = getattr(lib, name)
@@ -275,6 +283,35 @@ class LesGpu:
         out = np.zeros(4)
         check(self.lib.les_mlp_judge_stats(_dp(out)))
         return out
+
+    # ---- running statistics and diagnostics
+    def stat_init(self) -> None:
+        check(self.lib.les_stat_init())
+
+    def stat_reset(self) -> None:
+        check(self.lib.les_stat_reset())
+
+    def stat_accumulate(self) -> None:
+        check(self.lib.les_stat_accumulate())
+
+    def stat_download(self):
+        """(sums, ustar_sum, steps): sums[name] has shape (nz, ny, nx)."""
+        nz, ny, nx = self.prm.nz, self.prm.ny, self.prm.nx
+        s = np.empty((len(STATS), nz, ny, nx))
+        us = np.empty((ny, nx))
+        it = ctypes.c_int()
+        check(self.lib.les_stat_download(_dp(s), _dp(us), ctypes.byref(it)))
+        return dict(zip(STATS, s)), us, it.value
+
+    def stat_profiles(self) -> Dict[str, np.ndarray]:
+        out = np.empty((len(PROFILES), self.prm.nz))
+        check(self.lib.les_stat_profiles(_dp(out)))
+        return dict(zip(PROFILES, out))
+
+    def running_diagnostics(self) -> Dict[str, float]:
+        out = np.zeros(4)
+        check(self.lib.les_running_diagnostics(_dp(out)))
+        return {"cfl": out[0], "mke": out[1], "u_star": out[2], "divu": out[3]}
 
     def fft2d_r2c(self, d_in: int, d_out: int, nplanes: int, scale: float = 1.0) -> None:
         check(self.lib.les_fft2d_r2c(ctypes.c_void_p(d_in), ctypes.c_void_p(d_out), nplanes, float(scale)))

@@ -369,6 +369,7 @@ static void sgs_stress(Les& L, int jtt) {
 #include "les_press.h"
 #include "les_mlp.h"
 #include "les_learn.h"
+#include "les_stats.h"
 
 // ---------------------------------------------------------------- mixed-mode Nyquist filter
 // In the mixed-precision mode the velocity stays FP64, so the Nyquist-free
@@ -681,6 +682,8 @@ int les_init(const les_params* p) {
   for (int i = 0; i < 4; ++i) L->Fold[i] = (double*)dalloc(*L, 8 * L->nf);
   L->nyq_rows = (double*)dalloc(*L, 8 * (size_t)(L->nx + L->ny + 2) * L->nzg);
   L->nanflag = (int*)dalloc(*L, 64);
+  L->dg_part = (double*)dalloc(*L, sizeof(double) * LES_DIAG_BLOCKS * DG_N);
+  L->dg_out = (double*)dalloc(*L, sizeof(double) * 4);
   if (L->alloc_failed) { les_finalize(); les_set_error("out of memory (workspaces)"); return 3; }
   for (int i = 0; i < 64; ++i) if (!L->wk_real[i]) { les_finalize(); les_set_error("out of memory"); return 3; }
   setup_kernel_attrs();
@@ -1036,4 +1039,121 @@ int les_test_filter(int f, int alpha) {
   LES_CHECK(hipGetLastError());
   return 0;
 }
+}  // extern "C"
+
+// ---- running statistics and diagnostics
+template <typename T>
+static void stat_accumulate(Les& L) {
+  StatAccArgs<T> a;
+  a.u = D(L, LES_U); a.v = D(L, LES_V); a.w = D(L, LES_W); a.p = D(L, LES_P);
+  a.ustar = L.layer[LES_USTAR];
+  a.txx = F<T>(L, LES_TXX); a.tyy = F<T>(L, LES_TYY); a.tzz = F<T>(L, LES_TZZ);
+  a.txy = F<T>(L, LES_TXY); a.txz = F<T>(L, LES_TXZ); a.tyz = F<T>(L, LES_TYZ);
+  a.dudz = F<T>(L, LES_DUDZ); a.dvdz = F<T>(L, LES_DVDZ); a.nut = F<T>(L, LES_NU_T); a.cs = F<T>(L, LES_CS_OPT2);
+  a.s = L.st_sum; a.sus = L.st_ustar;
+  a.np = L.np; a.n = L.np * L.nz;
+  hipLaunchKernelGGL(k_stat_accum<T>, dim3(nblk(a.n)), dim3(LES_BLOCK), 0, L.stream, a);
+}
+
+template <typename T>
+static void diagnostics(Les& L) {
+  DiagArgs<T> a;
+  a.u = D(L, LES_U); a.v = D(L, LES_V); a.w = D(L, LES_W); a.ustar = L.layer[LES_USTAR];
+  a.dudx = F<T>(L, LES_DUDX); a.dvdy = F<T>(L, LES_DVDY); a.dwdz = F<T>(L, LES_DWDZ);
+  a.part = L.dg_part; a.np = L.np; a.n = L.np * L.nz;
+  hipLaunchKernelGGL(k_diag<T>, dim3(LES_DIAG_BLOCKS), dim3(LES_BLOCK), 0, L.stream, a);
+  DiagFinalArgs b;
+  b.part = L.dg_part; b.out = L.dg_out;
+  b.dt = L.P.dt; b.dx = L.dx; b.dy = L.dy; b.dz = L.dz;
+  b.nxyz = (double)L.nx * (double)L.ny * (double)L.nz;
+  b.nxy = (double)L.nx * (double)L.ny;
+  hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(LES_BLOCK), 0, L.stream, b);
+}
+
+static void stat_zero(Les& L) {
+  const long n = (long)LES_NSTAT * L.np * L.nz;
+  hipLaunchKernelGGL(k_fill_zero, dim3(nblk(n)), dim3(LES_BLOCK), 0, L.stream, L.st_sum, n);
+  hipLaunchKernelGGL(k_fill_zero, dim3(nblk(L.np)), dim3(LES_BLOCK), 0, L.stream, L.st_ustar, L.np);
+  L.st_iters = 0;
+}
+
+extern "C" {
+
+int les_stat_init(void) {
+  Les* L = g_les;
+  if (!L) return 1;
+  if (!L->st_sum) {
+    L->st_sum = (double*)dalloc(*L, sizeof(double) * LES_NSTAT * (size_t)L->np * L->nz);
+    L->st_ustar = (double*)dalloc(*L, sizeof(double) * L->np);
+    L->st_part = (double*)dalloc(*L, sizeof(double) * (size_t)L->nz * LES_PROF_SLICES * PS_N);
+    L->st_prof = (double*)dalloc(*L, sizeof(double) * (size_t)LES_NPROF * L->nz);
+    if (!L->st_sum || !L->st_ustar || !L->st_part || !L->st_prof) {
+      les_set_error("Out of memory when initialising io module variables");
+      return 3;
+    }
+  }
+  stat_zero(*L);
+  LES_CHECK(hipGetLastError());
+  return 0;
+}
+
+int les_stat_reset(void) {
+  Les* L = g_les;
+  if (!L) return 1;
+  if (!L->st_sum) { les_set_error("statistics not initialised (les_stat_init)"); return 1; }
+  stat_zero(*L);
+  LES_CHECK(hipGetLastError());
+  return 0;
+}
+
+int les_stat_accumulate(void) {
+  Les* L = g_les;
+  if (!L) return 1;
+  if (!L->st_sum) { les_set_error("statistics not initialised (les_stat_init)"); return 1; }
+  if (L->mixed) stat_accumulate<float>(*L); else stat_accumulate<double>(*L);
+  LES_CHECK(hipGetLastError());
+  L->st_iters += 1;
+  return 0;
+}
+
+int les_stat_download(double* sums3d, double* ustar2d, int* iterations) {
+  Les* L = g_les;
+  if (!L) return 1;
+  if (!L->st_sum) { les_set_error("statistics not initialised (les_stat_init)"); return 1; }
+  LES_CHECK(hipStreamSynchronize(L->stream));
+  if (sums3d)
+    LES_CHECK(hipMemcpy(sums3d, L->st_sum, sizeof(double) * LES_NSTAT * (size_t)L->np * L->nz, hipMemcpyDeviceToHost));
+  if (ustar2d) LES_CHECK(hipMemcpy(ustar2d, L->st_ustar, sizeof(double) * L->np, hipMemcpyDeviceToHost));
+  if (iterations) *iterations = L->st_iters;
+  return 0;
+}
+
+int les_stat_profiles(double* prof) {
+  Les* L = g_les;
+  if (!L) return 1;
+  if (!L->st_sum) { les_set_error("statistics not initialised (les_stat_init)"); return 1; }
+  if (L->st_iters < 1) { les_set_error("no accumulated statistics"); return 1; }
+  StatPlaneArgs a;
+  a.s = L->st_sum; a.part = L->st_part; a.np = L->np; a.n = L->np * L->nz;
+  a.nx = L->nx; a.nz = L->nz; a.nt = (double)L->st_iters;
+  hipLaunchKernelGGL(k_stat_planes, dim3(LES_PROF_SLICES, L->nz), dim3(LES_BLOCK), 0, L->stream, a);
+  hipLaunchKernelGGL(k_stat_profiles, dim3(nblk(L->nz, 64)), dim3(64), 0, L->stream, (const double*)L->st_part,
+                     L->st_prof, L->np, L->nz);
+  LES_CHECK(hipGetLastError());
+  LES_CHECK(hipMemcpyAsync(prof, L->st_prof, sizeof(double) * LES_NPROF * (size_t)L->nz, hipMemcpyDeviceToHost,
+                           L->stream));
+  LES_CHECK(hipStreamSynchronize(L->stream));
+  return 0;
+}
+
+int les_running_diagnostics(double* out) {
+  Les* L = g_les;
+  if (!L) return 1;
+  if (L->mixed) diagnostics<float>(*L); else diagnostics<double>(*L);
+  LES_CHECK(hipGetLastError());
+  LES_CHECK(hipMemcpyAsync(out, L->dg_out, sizeof(double) * 4, hipMemcpyDeviceToHost, L->stream));
+  LES_CHECK(hipStreamSynchronize(L->stream));
+  return 0;
+}
+
 }  // extern "C"

@@ -63,6 +63,15 @@ struct Les {
   int nband;              // levels 1..nband (z / delta < learned_zmin) keep the plain procedure
   int ncu;                // compute units of the device (persistent grids)
 
+  // running statistics (allocated by les_stat_init) and diagnostics
+  double* st_sum;         // LES_NSTAT sums of nx*ny*nz
+  double* st_ustar;       // sum of u* (nx*ny)
+  double* st_part;        // profile partial sums
+  double* st_prof;        // LES_NPROF x nz profiles
+  int st_iters;           // accumulated steps
+  double* dg_part;        // diagnostics partials
+  double* dg_out;         // cfl, mke, u*, rms div
+
   // timing
   int timing;
   hipEvent_t ev_a[KT_COUNT], ev_b[KT_COUNT];

@@ -20,6 +20,11 @@
 //   1-rank halo of the Lagrangian stencil ..... src/decomp_2d_custom.f90:902-1024
 //   wall law (obukhov, wallstress) ............ src/compute_wall_law.f90:107-254
 //   set_bc ..................................... src/boundary_condition_mpi.f90:33-63
+//   running statistics ........................ src/IO_statistics.f90:266-513
+//   profiles of the running averages .......... matlab_src/get_data_struct_RAV.m:21-35,
+//                                               matlab_src/get_profile_struct.m:9-59,
+//                                               matlab_src/compute_Cij.m:1-36
+//   running diagnostics ....................... src/running_diagnostics.f90:80-305
 //
 // Deliberate restatement choices (documented in the repository's design notes):
 //   * FFTs use the algorithm in fftcore.h, not FFTW; they are unnormalised like
@@ -78,6 +83,9 @@ struct Oracle {
   int jt_total = 0;
   bool F_coeff_init = false;
   bool lagran_flag = false;
+  // running statistics: 26 sums of (nx, ny, nz) and the u* sum
+  std::vector<double> st_sum, st_ustar;
+  int st_iters = 0;
 
   size_t I(int i, int j, int k) const { return (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * (size_t)k); }
   size_t S(int kx, int j, int k) const { return (size_t)kx + (size_t)lhx * ((size_t)j + (size_t)ny * (size_t)k); }
@@ -934,6 +942,140 @@ static void step(Oracle& o) {
   set_bc_all(o);
 }
 
+// ---------------------------------------------------------------- statistics
+// Sums in the order of RAV_<jt>_3D.dat (src/IO_statistics.f90:458-490).
+enum {
+  SU = 0, SV, SW, SP, SU2, SV2, SW2, SU3, SV3, SW3, SU4, SV4, SW4, SUV, SUW, SVW,
+  STXX, STYY, STZZ, STXY, STXZ, STYZ, SDUDZ, SDVDZ, SNUT, SCS, SN
+};
+
+static void stat_reset(Oracle& o) {
+  o.st_sum.assign((size_t)SN * o.np * o.nz, 0.0);
+  o.st_ustar.assign(o.np, 0.0);
+  o.st_iters = 0;
+}
+
+// One call of io_stat_averages (src/IO_statistics.f90:273, 319-324,
+// 372-440), 1-rank: Mav_uw, Mav_vw reset at level 1, then k = 2..nz with u, v
+// averaged to the w node; the other sums over k = 1..nz.  Integer powers as
+// products: u**2 = u*u, u**3 = (u*u)*u, u**4 = (u*u)*(u*u).
+static void stat_accumulate(Oracle& o) {
+  if (o.st_sum.empty()) stat_reset(o);
+  o.st_iters += 1;
+  const size_t n = o.np * o.nz;
+  double* S = o.st_sum.data();
+  for (size_t q = 0; q < o.np; ++q) o.st_ustar[q] = o.st_ustar[q] + o.ustar[q];
+  for (size_t q = 0; q < o.np; ++q) { S[SUW * n + q] = 0.0; S[SVW * n + q] = 0.0; }
+  for (int k = 2; k <= o.nz; ++k)
+    for (size_t q = 0; q < o.np; ++q) {
+      const size_t a = o.I(0, 0, k) + q, t = (size_t)(k - 1) * o.np + q;
+      S[SUW * n + t] = S[SUW * n + t] + o.w[a] * (0.5 * (o.u[a - o.np] + o.u[a]));
+      S[SVW * n + t] = S[SVW * n + t] + o.w[a] * (0.5 * (o.v[a - o.np] + o.v[a]));
+    }
+  for (int k = 1; k <= o.nz; ++k)
+    for (size_t q = 0; q < o.np; ++q) {
+      const size_t a = o.I(0, 0, k) + q, t = (size_t)(k - 1) * o.np + q;
+      const double u = o.u[a], v = o.v[a], w = o.w[a];
+      auto add = [&](int f, double x) { S[f * n + t] = S[f * n + t] + x; };
+      add(SU, u); add(SV, v); add(SW, w); add(SP, o.p[a]);
+      add(SU2, u * u); add(SV2, v * v); add(SW2, w * w);
+      add(SU3, (u * u) * u); add(SV3, (v * v) * v); add(SW3, (w * w) * w);
+      add(SU4, (u * u) * (u * u)); add(SV4, (v * v) * (v * v)); add(SW4, (w * w) * (w * w));
+      add(SUV, u * v);
+      add(STXX, o.txx[a]); add(STYY, o.tyy[a]); add(STZZ, o.tzz[a]);
+      add(STXY, o.txy[a]); add(STXZ, o.txz[a]); add(STYZ, o.tyz[a]);
+      add(SDUDZ, o.dudz[a]); add(SDVDZ, o.dvdz[a]);
+      add(SNUT, o.nu_t[a]); add(SCS, std::sqrt(o.cs_opt2[a]));
+    }
+}
+
+// Plane mean as squeeze(mean(mean(X, 2), 1)): mean over y for each x, then
+// over x.
+template <typename Fn>
+static double plane_mean(const Oracle& o, Fn X) {
+  double sx = 0.0;
+  for (int i = 0; i < o.nx; ++i) {
+    double sy = 0.0;
+    for (int j = 0; j < o.ny; ++j) sy += X((size_t)i + (size_t)o.nx * j);
+    sx += sy / o.ny;
+  }
+  return sx / o.nx;
+}
+
+// Profiles of the time averages (sum / nt; SGS stresses negated by
+// get_data_struct_RAV.m:31-34), as get_profile_struct.m:9-59 and
+// compute_Cij.m:1-36 on one output window: colocate_var(X, 'w') is 0 at
+// level 1 and (X(k-1) + X(k)) / 2 above; colocate_var(X, 'uvp') is
+// (X(k) + X(k+1)) / 2 below level nz and 0 at nz.  Rows: u v w p var_u
+// var_v var_w txx tyy tzz R13 txz St13 R23 tyz St23 tke cs nut C13 C23.
+static void stat_profiles(const Oracle& o, double* prof) {
+  const int nz = o.nz;
+  const size_t n = o.np * o.nz;
+  const double nt = (double)o.st_iters;
+  auto A = [&](int f, int k, size_t q) { return o.st_sum[f * n + (size_t)(k - 1) * o.np + q] / nt; };
+  for (int k = 1; k <= nz; ++k) {
+    auto col = [&](int f, size_t q) { return k == 1 ? 0.0 : 0.5 * (A(f, k - 1, q) + A(f, k, q)); };
+    auto var = [&](int f, int f2, int kk, size_t q) { const double m = A(f, kk, q); return A(f2, kk, q) - m * m; };
+    const double u = plane_mean(o, [&](size_t q) { return A(SU, k, q); });
+    const double v = plane_mean(o, [&](size_t q) { return A(SV, k, q); });
+    const double w = plane_mean(o, [&](size_t q) { return A(SW, k, q); });
+    const double p = plane_mean(o, [&](size_t q) { return A(SP, k, q); });
+    const double vu = plane_mean(o, [&](size_t q) { return var(SU, SU2, k, q); });
+    const double vv = plane_mean(o, [&](size_t q) { return var(SV, SV2, k, q); });
+    const double vw = plane_mean(o, [&](size_t q) { return var(SW, SW2, k, q); });
+    const double vw_uvp = plane_mean(o, [&](size_t q) {
+      return k < nz ? 0.5 * (var(SW, SW2, k, q) + var(SW, SW2, k + 1, q)) : 0.0; });
+    const double txx = plane_mean(o, [&](size_t q) { return -A(STXX, k, q); });
+    const double tyy = plane_mean(o, [&](size_t q) { return -A(STYY, k, q); });
+    const double tzz = plane_mean(o, [&](size_t q) { return -A(STZZ, k, q); });
+    auto r13 = [&](size_t q) { return A(SUW, k, q) - col(SU, q) * A(SW, k, q); };
+    auto r23 = [&](size_t q) { return A(SVW, k, q) - col(SV, q) * A(SW, k, q); };
+    const double R13 = plane_mean(o, r13), R23 = plane_mean(o, r23);
+    const double txz = plane_mean(o, [&](size_t q) { return -A(STXZ, k, q); });
+    const double tyz = plane_mean(o, [&](size_t q) { return -A(STYZ, k, q); });
+    const double st13 = plane_mean(o, [&](size_t q) { return r13(q) + (-A(STXZ, k, q)); });
+    const double st23 = plane_mean(o, [&](size_t q) { return r23(q) + (-A(STYZ, k, q)); });
+    const double C13 = plane_mean(o, [&](size_t q) { return col(SU, q) * A(SW, k, q); }) -
+                       plane_mean(o, [&](size_t q) { return col(SU, q); }) * w;
+    const double C23 = plane_mean(o, [&](size_t q) { return col(SV, q) * A(SW, k, q); }) -
+                       plane_mean(o, [&](size_t q) { return col(SV, q); }) * w;
+    const double cs = plane_mean(o, [&](size_t q) { return A(SCS, k, q); });
+    const double nut = plane_mean(o, [&](size_t q) { return A(SNUT, k, q); });
+    const double row[21] = {u, v, w, p, vu, vv, vw, txx, tyy, tzz, R13, txz, st13 + C13,
+                            R23, tyz, st23 + C23, 0.5 * (vu + vv + vw_uvp), cs, nut, C13, C23};
+    for (int r = 0; r < 21; ++r) prof[(size_t)r * nz + (k - 1)] = row[r];
+  }
+}
+
+// compute_running_diagnistics (src/running_diagnostics.f90:80-305), 1 rank:
+// CFL = dt (max|u|/dx + max|v|/dy + max|w|/dz) over levels 1..nz; MKE =
+// (U^2 + V^2 + W^2) / 2 of the domain means (w averaged to the uvp node);
+// plane mean of u*; rms of (du/dx + dv/dy + dw/dz)**2 over levels 1..nz.
+static void diagnostics(const Oracle& o, double* out) {
+  double um = -1234567890.0, vm = -1234567890.0, wm = -1234567890.0;
+  double us = 0.0, vs = 0.0, ws = 0.0, rms = 0.0;
+  for (int k = 1; k <= o.nz; ++k)
+    for (size_t q = 0; q < o.np; ++q) {
+      const size_t a = o.I(0, 0, k) + q;
+      if (std::fabs(o.u[a]) > um) um = std::fabs(o.u[a]);
+      if (std::fabs(o.v[a]) > vm) vm = std::fabs(o.v[a]);
+      if (std::fabs(o.w[a]) > wm) wm = std::fabs(o.w[a]);
+      us = us + o.u[a];
+      vs = vs + o.v[a];
+      ws = ws + 0.5 * (o.w[a] + o.w[a + o.np]);
+      const double d = o.dudx[a] + o.dvdy[a] + o.dwdz[a];
+      rms = rms + d * d;
+    }
+  const double nxyz = (double)o.nx * (double)o.ny * (double)o.nz;
+  out[0] = o.P.dt * (um / o.dx + vm / o.dy + wm / o.dz);
+  us = us / nxyz; vs = vs / nxyz; ws = ws / nxyz;
+  out[1] = 0.5 * (us * us + vs * vs + ws * ws);
+  double s = 0.0;
+  for (size_t q = 0; q < o.np; ++q) s = s + o.ustar[q];
+  out[2] = s / (double)(o.nx * o.ny);
+  out[3] = std::sqrt(rms / nxyz);
+}
+
 }  // namespace orc
 
 using namespace orc;
@@ -1150,6 +1292,24 @@ int orc_op(int which, double arg) {
 }
 
 int orc_set_jt_total(int jt) { if (!g) return 1; g->jt_total = jt; return 0; }
+
+// running statistics and diagnostics
+int orc_stat_reset(void) { if (!g) return 1; stat_reset(*g); return 0; }
+int orc_stat_accumulate(void) { if (!g) return 1; stat_accumulate(*g); return 0; }
+int orc_stat_get(double* sums3d, double* ustar2d, int* iterations) {
+  if (!g) return 1;
+  if (g->st_sum.empty()) stat_reset(*g);
+  if (sums3d) std::memcpy(sums3d, g->st_sum.data(), sizeof(double) * g->st_sum.size());
+  if (ustar2d) std::memcpy(ustar2d, g->st_ustar.data(), sizeof(double) * g->np);
+  if (iterations) *iterations = g->st_iters;
+  return 0;
+}
+int orc_stat_profiles(double* prof) {
+  if (!g || g->st_iters < 1) return 1;
+  stat_profiles(*g, prof);
+  return 0;
+}
+int orc_diagnostics(double* out) { if (!g) return 1; diagnostics(*g, out); return 0; }
 int orc_set_2d(int which, const double* in) {
   if (!g) return 1;
   std::vector<double>* f = which == 0 ? &g->lbc_u : which == 1 ? &g->lbc_v : which == 2 ? &g->ustar

@@ -43,6 +43,8 @@ def lib():
             "orc_tridiag": [ctypes.c_int, ctypes.c_int, d, d], "orc_table": [ctypes.c_int, d],
             "orc_op": [ctypes.c_int, ctypes.c_double], "orc_set_jt_total": [ctypes.c_int],
             "orc_set_threads": [ctypes.c_int], "orc_jt_total": [],
+            "orc_stat_reset": [], "orc_stat_accumulate": [], "orc_stat_get": [d, d, i],
+            "orc_stat_profiles": [d], "orc_diagnostics": [d],
         }.items():
             f = getattr(_lib, name)
             f.argtypes = args
@@ -168,3 +170,34 @@ class Oracle:
 
     def jt_total(self):
         return lib().orc_jt_total()
+
+    # ---- running statistics and diagnostics
+    STATS = ["u", "v", "w", "p", "u2", "v2", "w2", "u3", "v3", "w3", "u4", "v4", "w4", "uv", "uw", "vw",
+             "txx", "tyy", "tzz", "txy", "txz", "tyz", "dudz", "dvdz", "nut", "cs"]
+    PROFILES = ["u", "v", "w", "p", "var_u", "var_v", "var_w", "txx", "tyy", "tzz", "R13", "txz", "St13",
+                "R23", "tyz", "St23", "tke", "cs", "nut", "C13", "C23"]
+
+    def stat_reset(self):
+        lib().orc_stat_reset()
+
+    def stat_accumulate(self):
+        lib().orc_stat_accumulate()
+
+    def stat_get(self):
+        nz, ny, nx = self.prm.nz, self.prm.ny, self.prm.nx
+        s = np.zeros((len(self.STATS), nz, ny, nx))
+        us = np.zeros((ny, nx))
+        it = ctypes.c_int()
+        lib().orc_stat_get(_dp(s), _dp(us), ctypes.byref(it))
+        return dict(zip(self.STATS, s)), us, it.value
+
+    def stat_profiles(self):
+        out = np.zeros((len(self.PROFILES), self.prm.nz))
+        if lib().orc_stat_profiles(_dp(out)):
+            raise ValueError("no accumulated statistics")
+        return dict(zip(self.PROFILES, out))
+
+    def diagnostics(self):
+        out = np.zeros(4)
+        lib().orc_diagnostics(_dp(out))
+        return {"cfl": out[0], "mke": out[1], "u_star": out[2], "divu": out[3]}
