@@ -5,8 +5,10 @@
 !! (src/LES_engine.f90:77-438): read parameters.nlst, initial fields from
 !! vel_u/v/w.dat (ic_type='files') or a checkpoint, then per step the wall
 !! law, velocity gradients, SGS stress, right-hand side, AB2 advance,
-!! pressure and projection, with the reference's NaN check, checkpoint files
-!! and profiling.txt.  All field data stays on the device between steps.
+!! pressure and projection, with the reference's running diagnostics
+!! (running_diagnostics.txt), NaN check, running-average statistics (RAV
+!! files, stat_flag), checkpoint files and profiling.txt.  All field data
+!! stays on the device between steps.
 !------------------------------------------------------------------------------
 module LES_engine
   use iso_c_binding
@@ -18,6 +20,8 @@ module LES_engine
   use compute_pressure
   use compute_sgs
   use compute_wall_law
+  use running_diagnositcs
+  use mod_IO_statistics
   implicit none
   private
   public :: LES_main_engine
@@ -48,12 +52,15 @@ contains
        tt = jt_total*dt
     end if
     call les_check(les_init(cparams), 'les_init')
+    call running_diagnostics_init()
+    if (stat_flag) call io_stat_init()
     if (.not. restart_flag) then
        call set_initial_conditions()
     else
        call checkpoint_in(jt_total)
     end if
     call les_check(les_set_jt_total(int(jt_total, c_int)), 'les_set_jt_total')
+    call compute_running_diagnistics(jt, jt_total, tt, .false.)
     write(*,*) '## Flow initialized - ready for simulation'
 
     do jt = 1, nt
@@ -93,7 +100,9 @@ contains
        call les_check(les_projection(tadv1), 'projection step')
        if (profiling) time_loop = time_loop + wtime(.true.) - time_tmp
 
+       call compute_running_diagnistics(jt, jt_total, tt, .true.)
        if (mod(jt, NaN_check) == 0) call check_NaN(jt)
+       if (stat_flag .and. jt_total > start_stats) call io_stat_averages(jt, jt_total)
        if (mod(jt_total, checkpoint) == 0) call checkpoint_out(jt_total)
     end do
 
@@ -103,6 +112,7 @@ contains
     end if
     if (mod(jt_total, checkpoint) /= 0) call checkpoint_out(jt_total)
     call write_parameters_restart(jt_total)
+    if (stat_flag) call io_stat_finalize()
 
     write(*,'(A,F10.4,A)') ' wall time for time loop: ', time_loop, 's'
     open(99, file='./profiling.txt', status='replace')

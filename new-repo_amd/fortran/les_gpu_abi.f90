@@ -123,7 +123,31 @@ module les_gpu_abi
        import :: c_int, c_double
        real(c_double), value :: tadv1
      end function les_projection
+     integer(c_int) function les_stat_init() bind(C, name="les_stat_init")
+       import :: c_int
+     end function les_stat_init
+     integer(c_int) function les_stat_reset() bind(C, name="les_stat_reset")
+       import :: c_int
+     end function les_stat_reset
+     integer(c_int) function les_stat_accumulate() bind(C, name="les_stat_accumulate")
+       import :: c_int
+     end function les_stat_accumulate
+     integer(c_int) function les_stat_download(sums3d, ustar2d, iterations) bind(C, name="les_stat_download")
+       import :: c_int, c_double
+       real(c_double), intent(out) :: sums3d(*), ustar2d(*)
+       integer(c_int), intent(out) :: iterations
+     end function les_stat_download
+     integer(c_int) function les_stat_profiles(prof) bind(C, name="les_stat_profiles")
+       import :: c_int, c_double
+       real(c_double), intent(out) :: prof(*)
+     end function les_stat_profiles
+     integer(c_int) function les_running_diagnostics(out) bind(C, name="les_running_diagnostics")
+       import :: c_int, c_double
+       real(c_double), intent(out) :: out(4)
+     end function les_running_diagnostics
   end interface
+
+  integer(c_int), parameter :: LES_NSTAT = 26, LES_NPROF = 21
 
 contains
 

@@ -2,7 +2,9 @@
 library through iso_c_binding: reference-style parameters.nlst and vel_*.dat
 in, reference-format checkpoint files out.  The checkpoint after N steps must
 equal the CPU oracle's state, and a checkpoint/restart continuation must
-reproduce an uninterrupted run bit for bit."""
+reproduce an uninterrupted run bit for bit.  With stat_flag the RAV files must
+hold the oracle's running sums bit for bit, and running_diagnostics.txt the
+oracle's diagnostics to its printed precision."""
 import pathlib
 import subprocess
 
@@ -16,7 +18,7 @@ pytestmark = pytest.mark.gpu
 DRIVER = PKG_DIR / "les_driver"
 
 NLST = """&simulation_param  p_on_y=1 p_on_z=1 nx={nx} ny={ny} nz={nz} nt={nt} dt=2.0d-4 dim_dt_flag=.false. restart_flag={restart} /
-&output_param      out_path='./' output=.false. step_out=1000 stat_flag=.false. start_stats=0 running_period=100
+&output_param      out_path='./' output=.false. step_out=1000 stat_flag={stat} start_stats={start} running_period={period}
                    tke_flag=.false. start_tke=0 tke_period=100 checkpoint={ckpt} scheckpoint=100000 /
 &physical_param    XY_pi=.true. lx=2.0d0 ly=2.0d0 lz=1.0d0 u_scale=1.0d0 zi=1.0d0 lbc_dz0=1.0d-4 Re=1.0d5
                    pfx=1.0d0 pfy=0.0d0 coriolis_flag=.false. ug_dim=0.0d0 vg_dim=0.0d0 geostr_rot=.false. geostr_start=100000 /
@@ -26,9 +28,11 @@ NLST = """&simulation_param  p_on_y=1 p_on_z=1 nx={nx} ny={ny} nz={nz} nt={nt} d
 """
 
 
-def _run(d, nx, ny, nz, nt, restart=False, ckpt=100000):
+def _run(d, nx, ny, nz, nt, restart=False, ckpt=100000, stats=None):
+    stat, start, period = (".true.", *stats) if stats else (".false.", 0, 100)
     (d / "parameters.nlst").write_text(NLST.format(nx=nx, ny=ny, nz=nz, nt=nt, ckpt=ckpt,
-                                                   restart=".true." if restart else ".false."))
+                                                   restart=".true." if restart else ".false.",
+                                                   stat=stat, start=start, period=period))
     r = subprocess.run([str(DRIVER)], cwd=d, capture_output=True, text=True, timeout=120)
     print(r.stdout[-800:], r.stderr[-800:], flush=True)
     assert r.returncode == 0
@@ -70,3 +74,42 @@ def test_fortran_driver_matches_oracle_and_restarts(tmp_path):
     cont = _read_ckpt(runs["part1"] / "checkpoint_000000010", nx, ny, nz, 11)
     assert np.array_equal(cont, full)
     assert (runs["full"] / "profiling.txt").exists()
+
+
+def test_fortran_driver_statistics_and_diagnostics_files(tmp_path):
+    if not DRIVER.exists():
+        pytest.skip("flang driver not built")
+    P, IC, O = load_params(), load_ic(), load_oracle()
+    nx, ny, nz = 16, 16, 8
+    prm = P.LesParams(nx=nx, ny=ny, nz=nz)
+    u, v, w = IC.log_law_ic(nx, ny, nz, prm.lz, prm.z0, 0.1, seed=23)
+    IC.write_vel_files(tmp_path, u, v, w)
+    print("fortran driver: 100 steps, statistics from step 3 every 4 steps", flush=True)
+    _run(tmp_path, nx, ny, nz, 100, stats=(2, 4))
+    info = (tmp_path / "RAV_000000008_info.txt").read_text().split()
+    assert [int(x) for x in info] == [nx, ny, nz, 4]
+    assert [int(x) for x in (tmp_path / "RAV_000000004_info.txt").read_text().split()][3] == 2
+    assert (tmp_path / "RAV_000000100_3D.dat").exists()
+    s3 = np.fromfile(tmp_path / "RAV_000000008_3D.dat", dtype="<f8").reshape(26, nz, ny, nx)
+    s2 = np.fromfile(tmp_path / "RAV_000000008_2D.dat", dtype="<f8").reshape(ny, nx)
+    o = O.Oracle(prm)
+    o.set_ic(u, v, w)
+    o.step(4)
+    o.stat_reset()
+    diag = {}
+    for jt in range(5, 9):
+        o.step(1)
+        o.stat_accumulate()
+        diag[jt] = o.diagnostics()
+    sums, us, it = o.stat_get()
+    assert it == 4 and np.array_equal(s2, us)
+    for i, name in enumerate(O.Oracle.STATS):
+        assert np.array_equal(s3[i], sums[name]), name
+    rows = np.loadtxt(tmp_path / "running_diagnostics.txt")
+    assert rows.shape == (100, 6) and np.array_equal(rows[:, 0], np.arange(1, 101))
+    for jt in range(5, 9):
+        r = rows[jt - 1]
+        assert r[1] == pytest.approx(jt * prm.dt, rel=1e-6)
+        for col, key in ((2, "cfl"), (3, "mke"), (4, "u_star"), (5, "divu")):
+            # E16.9: 9 significant digits, relative rounding up to 5e-9
+            assert r[col] == pytest.approx(diag[jt][key], rel=6e-9, abs=1e-18), (jt, key)
