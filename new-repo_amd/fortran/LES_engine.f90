@@ -7,7 +7,8 @@
 !! law, velocity gradients, SGS stress, right-hand side, AB2 advance,
 !! pressure and projection, with the reference's running diagnostics
 !! (running_diagnostics.txt), NaN check, running-average statistics (RAV
-!! files, stat_flag), checkpoint files and profiling.txt.  All field data
+!! files, stat_flag), field snapshots (2decomp_sys files, output),
+!! checkpoint files and profiling.txt.  All field data
 !! stays on the device between steps.
 !------------------------------------------------------------------------------
 module LES_engine
@@ -22,6 +23,7 @@ module LES_engine
   use compute_wall_law
   use running_diagnositcs
   use mod_IO_statistics
+  use mod_IO_main
   implicit none
   private
   public :: LES_main_engine
@@ -53,6 +55,7 @@ contains
     end if
     call les_check(les_init(cparams), 'les_init')
     call running_diagnostics_init()
+    if (output) call io_init()
     if (stat_flag) call io_stat_init()
     if (.not. restart_flag) then
        call set_initial_conditions()
@@ -60,6 +63,7 @@ contains
        call checkpoint_in(jt_total)
     end if
     call les_check(les_set_jt_total(int(jt_total, c_int)), 'les_set_jt_total')
+    if (output) call output_loop(0, jt_total)
     call compute_running_diagnistics(jt, jt_total, tt, .false.)
     write(*,*) '## Flow initialized - ready for simulation'
 
@@ -103,6 +107,7 @@ contains
        call compute_running_diagnistics(jt, jt_total, tt, .true.)
        if (mod(jt, NaN_check) == 0) call check_NaN(jt)
        if (stat_flag .and. jt_total > start_stats) call io_stat_averages(jt, jt_total)
+       if (output .and. mod(jt, step_out) == 0) call output_loop(jt, jt_total)
        if (mod(jt_total, checkpoint) == 0) call checkpoint_out(jt_total)
     end do
 
@@ -113,6 +118,7 @@ contains
     if (mod(jt_total, checkpoint) /= 0) call checkpoint_out(jt_total)
     call write_parameters_restart(jt_total)
     if (stat_flag) call io_stat_finalize()
+    if (output) call io_finalize()
 
     write(*,'(A,F10.4,A)') ' wall time for time loop: ', time_loop, 's'
     open(99, file='./profiling.txt', status='replace')

@@ -18,7 +18,7 @@ pytestmark = pytest.mark.gpu
 DRIVER = PKG_DIR / "les_driver"
 
 NLST = """&simulation_param  p_on_y=1 p_on_z=1 nx={nx} ny={ny} nz={nz} nt={nt} dt=2.0d-4 dim_dt_flag=.false. restart_flag={restart} /
-&output_param      out_path='./' output=.false. step_out=1000 stat_flag={stat} start_stats={start} running_period={period}
+&output_param      out_path='./' output={output} step_out={step_out} stat_flag={stat} start_stats={start} running_period={period}
                    tke_flag=.false. start_tke=0 tke_period=100 checkpoint={ckpt} scheckpoint=100000 /
 &physical_param    XY_pi=.true. lx=2.0d0 ly=2.0d0 lz=1.0d0 u_scale=1.0d0 zi=1.0d0 lbc_dz0=1.0d-4 Re=1.0d5
                    pfx=1.0d0 pfy=0.0d0 coriolis_flag=.false. ug_dim=0.0d0 vg_dim=0.0d0 geostr_rot=.false. geostr_start=100000 /
@@ -28,11 +28,13 @@ NLST = """&simulation_param  p_on_y=1 p_on_z=1 nx={nx} ny={ny} nz={nz} nt={nt} d
 """
 
 
-def _run(d, nx, ny, nz, nt, restart=False, ckpt=100000, stats=None):
+def _run(d, nx, ny, nz, nt, restart=False, ckpt=100000, stats=None, step_out=None):
     stat, start, period = (".true.", *stats) if stats else (".false.", 0, 100)
     (d / "parameters.nlst").write_text(NLST.format(nx=nx, ny=ny, nz=nz, nt=nt, ckpt=ckpt,
                                                    restart=".true." if restart else ".false.",
-                                                   stat=stat, start=start, period=period))
+                                                   stat=stat, start=start, period=period,
+                                                   output=".true." if step_out else ".false.",
+                                                   step_out=step_out or 1000))
     r = subprocess.run([str(DRIVER)], cwd=d, capture_output=True, text=True, timeout=120)
     print(r.stdout[-800:], r.stderr[-800:], flush=True)
     assert r.returncode == 0
@@ -85,7 +87,7 @@ def test_fortran_driver_statistics_and_diagnostics_files(tmp_path):
     u, v, w = IC.log_law_ic(nx, ny, nz, prm.lz, prm.z0, 0.1, seed=23)
     IC.write_vel_files(tmp_path, u, v, w)
     print("fortran driver: 100 steps, statistics from step 3 every 4 steps", flush=True)
-    _run(tmp_path, nx, ny, nz, 100, stats=(2, 4))
+    _run(tmp_path, nx, ny, nz, 100, stats=(2, 4), step_out=8)
     info = (tmp_path / "RAV_000000008_info.txt").read_text().split()
     assert [int(x) for x in info] == [nx, ny, nz, 4]
     assert [int(x) for x in (tmp_path / "RAV_000000004_info.txt").read_text().split()][3] == 2
@@ -103,6 +105,13 @@ def test_fortran_driver_statistics_and_diagnostics_files(tmp_path):
         diag[jt] = o.diagnostics()
     sums, us, it = o.stat_get()
     assert it == 4 and np.array_equal(s2, us)
+    # field snapshot after step 8 (24 fields, levels 1..nz) and the initial one
+    snap = np.fromfile(tmp_path / "2decomp_sys_000000008.dat", dtype="<f8").reshape(24, nz, ny, nx)
+    for i, name in enumerate(["u", "v", "w", "p", "rhsx", "rhsy", "rhsz", "dudx", "dudy", "dudz", "dvdx", "dvdy",
+                              "dvdz", "dwdx", "dwdy", "dwdz", "txx", "txy", "txz", "tyy", "tyz", "tzz", "nu_t",
+                              "cs_opt2"]):
+        assert np.array_equal(snap[i], o.get(name)[1:nz + 1]), name
+    assert (tmp_path / "2decomp_sys_000000000.dat").exists()
     for i, name in enumerate(O.Oracle.STATS):
         assert np.array_equal(s3[i], sums[name]), name
     rows = np.loadtxt(tmp_path / "running_diagnostics.txt")
