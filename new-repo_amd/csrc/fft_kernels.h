@@ -63,7 +63,7 @@ __device__ __forceinline__ void rows_r2c_body(const RowR2CArgs<TI, T>& a, Cplx<T
 
 template <typename TI, typename T>
 __global__ void __launch_bounds__(LES_BLOCK) k_rows_r2c(RowR2CArgs<TI, T> a, FftPlan pl, const Cplx<T>* __restrict__ tw) {
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
   const int lane = threadIdx.x & (LES_WAVE - 1), wave = threadIdx.x / LES_WAVE;
   Cplx<T>* z = reinterpret_cast<Cplx<T>*>(smem_raw) + wave * a.nx;
   const long job = (long)blockIdx.x * (blockDim.x / LES_WAVE) + wave;
@@ -114,7 +114,7 @@ __device__ __forceinline__ void rows_c2r_body(const RowC2RArgs<T, TO>& a, Cplx<T
 
 template <typename T, typename TO>
 __global__ void __launch_bounds__(LES_BLOCK) k_rows_c2r(RowC2RArgs<T, TO> a, FftPlan pl, const Cplx<T>* __restrict__ tw) {
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
   const int lane = threadIdx.x & (LES_WAVE - 1), wave = threadIdx.x / LES_WAVE;
   Cplx<T>* z = reinterpret_cast<Cplx<T>*>(smem_raw) + wave * a.nx;
   const long job = (long)blockIdx.x * (blockDim.x / LES_WAVE) + wave;
@@ -318,7 +318,7 @@ __device__ __forceinline__ void cols_body_pipe(const ColArgs<T>& a, unsigned cha
 // Length 256 runs on register twiddles and the pipelined persistent body.
 template <typename T, int CB>
 __global__ void __launch_bounds__(LES_BLOCK) k_cols(ColArgs<T> a, FftPlan pl, const Cplx<T>* __restrict__ tw) {
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
   const int lane = threadIdx.x & (LES_WAVE - 1);
   cols_body<T, CB>(a, smem_raw, [&](Cplx<T>* p, bool inv) { wave_fft(p, pl, tw, inv, lane); });
 }
@@ -327,7 +327,7 @@ __global__ void __launch_bounds__(LES_BLOCK) k_cols(ColArgs<T> a, FftPlan pl, co
 // runtime-plan path (whose radix-2 stages need twice the butterflies).
 template <typename T, int CB>
 __global__ void __launch_bounds__(LES_BLOCK) k_cols_256(ColArgs<T> a, FftPlan pl, const Cplx<T>* __restrict__ tw) {
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
   const int lane = threadIdx.x & (LES_WAVE - 1);
   TwReg<T, 256> tr;
   tw_load(tr, tw + pl.twf, lane);
@@ -411,7 +411,7 @@ __device__ __forceinline__ void rows_resample_body(const RowResArgs<T, TI, TO>& 
 template <typename T, typename TI, typename TO>
 __global__ void __launch_bounds__(LES_BLOCK) k_rows_resample(RowResArgs<T, TI, TO> a, FftPlan p1, FftPlan p2,
                                                              const Cplx<T>* __restrict__ tw) {
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
   const int nm = a.n1 > a.n2 ? a.n1 : a.n2;
   const int lane = threadIdx.x & (LES_WAVE - 1), wave = threadIdx.x / LES_WAVE;
   Cplx<T>* z = reinterpret_cast<Cplx<T>*>(smem_raw) + wave * nm;
@@ -503,7 +503,7 @@ __device__ __forceinline__ void cols_resample_body(const ColResArgs<T>& a, unsig
 template <typename T, int CP>
 __global__ void __launch_bounds__(LES_BLOCK) k_cols_resample(ColResArgs<T> a, FftPlan p1, FftPlan p2,
                                                              const Cplx<T>* __restrict__ tw) {
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
   const int lane = threadIdx.x & (LES_WAVE - 1);
   if constexpr (sizeof(T) == 4) {
     if (p1.n == 256 && p2.n == 384) {

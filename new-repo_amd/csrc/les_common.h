@@ -76,6 +76,37 @@ __device__ __forceinline__ void bfly(Cplx<T>* x, int R, int sgn) {
 #define LES_WAVE 64
 #define LES_MAX_N 1024
 
+// LDS placement inside a transform.  The Stockham writes out[(i-k)R + k +
+// r p] of the early stages hit few banks (stride R, or runs of p < 16), so
+// between the first stage's reads and the last stage's writes (both
+// contiguous across lanes, hence conflict-free) element e of a line lives at
+// lds_sw(e): within each aligned block of 16 elements, e ^ (5 (e >> 4) mod
+// 16).  This spreads every stage's 16 consecutive lanes over distinct bank
+// pairs for 8-byte elements (one group of 8 lanes per 16-byte element).  The
+// line is in natural order outside the transform.  Needs n % 16 == 0.
+__device__ __forceinline__ int lds_sw(int e) { return e ^ (((e >> 4) * 5) & 15); }
+
+// One 8- or 16-byte LDS access per complex value (ds_read_b64 / _b128)
+// instead of two 4- or 8-byte ones.
+template <typename T>
+struct Vec2T;
+template <>
+struct Vec2T<float> { using type = float2; };
+template <>
+struct Vec2T<double> { using type = double2; };
+template <typename T>
+__device__ __forceinline__ Cplx<T> lds_ld(const Cplx<T>* p) {
+  const typename Vec2T<T>::type v = *reinterpret_cast<const typename Vec2T<T>::type*>(p);
+  return {v.x, v.y};
+}
+template <typename T>
+__device__ __forceinline__ void lds_st(Cplx<T>* p, Cplx<T> a) {
+  typename Vec2T<T>::type v;
+  v.x = a.x;
+  v.y = a.y;
+  *reinterpret_cast<typename Vec2T<T>::type*>(p) = v;
+}
+
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -94,21 +125,28 @@ __device__ __forceinline__ int stage_k(int i, int p) {
   else return i % p;
 }
 
+// swr / sww: the stage reads / writes through lds_sw.
 template <typename T, int R, int KM>
-__device__ __forceinline__ void wave_stage(Cplx<T>* z, int n, int p, const Cplx<T>* __restrict__ tw, int sgn, int lane) {
+__device__ __forceinline__ void wave_stage(Cplx<T>* z, int n, int p, const Cplx<T>* __restrict__ tw, int sgn, int lane,
+                                           bool swr, bool sww) {
   constexpr int MAXB = (LES_MAX_N / R + LES_WAVE - 1) / LES_WAVE;
   const int m = n / R;
   Cplx<T> x[MAXB][4];
   int kk[MAXB];
+  // addresses are recomputed in each stage, not kept live across stages
+  __asm__ volatile("" : "+v"(lane));
 #pragma unroll
   for (int t = 0; t < MAXB; ++t) {
     const int i = lane + LES_WAVE * t;
     if (i < m) {
       const int k = stage_k<KM>(i, p);
       kk[t] = k;
-      x[t][0] = z[i];
+      x[t][0] = lds_ld(z + (swr ? lds_sw(i) : i));
 #pragma unroll
-      for (int r = 1; r < R; ++r) x[t][r] = c_mul(z[i + r * m], tw[(r - 1) * p + k]);
+      for (int r = 1; r < R; ++r) {
+        const int e = i + r * m;
+        x[t][r] = c_mul(lds_ld(z + (swr ? lds_sw(e) : e)), tw[(r - 1) * p + k]);
+      }
     }
   }
   wave_sync();
@@ -120,7 +158,10 @@ __device__ __forceinline__ void wave_stage(Cplx<T>* z, int n, int p, const Cplx<
       bfly(x[t], R, sgn);
       const int j = (i - k) * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) z[j + r * p] = x[t][r];
+      for (int r = 0; r < R; ++r) {
+        const int e = j + r * p;
+        lds_st(z + (sww ? lds_sw(e) : e), x[t][r]);
+      }
     }
   }
   wave_sync();
@@ -131,19 +172,21 @@ __device__ __forceinline__ void wave_fft(Cplx<T>* z, const FftPlan& pl, const Cp
                                          int lane) {
   const int sgn = inverse ? 1 : -1;
   const Cplx<T>* tw0 = twpool + (inverse ? pl.twi : pl.twf);
+  const bool swz = (pl.n & 15) == 0;
   for (int s = 0; s < pl.nst; ++s) {
     const int R = pl.R[s], p = pl.P[s];
     const Cplx<T>* tw = tw0 + pl.off[s];
+    const bool swr = swz && s > 0, sww = swz && s < pl.nst - 1;
     if (R == 4) {
-      wave_stage<T, 4, 0>(z, pl.n, p, tw, sgn, lane);
+      wave_stage<T, 4, 0>(z, pl.n, p, tw, sgn, lane, swr, sww);
     } else if (R == 2) {
-      wave_stage<T, 2, 0>(z, pl.n, p, tw, sgn, lane);
+      wave_stage<T, 2, 0>(z, pl.n, p, tw, sgn, lane, swr, sww);
     } else if (p * 3 == pl.n) {
-      wave_stage<T, 3, 1>(z, pl.n, p, tw, sgn, lane);
+      wave_stage<T, 3, 1>(z, pl.n, p, tw, sgn, lane, swr, sww);
     } else if ((p & (p - 1)) == 0) {
-      wave_stage<T, 3, 0>(z, pl.n, p, tw, sgn, lane);
+      wave_stage<T, 3, 0>(z, pl.n, p, tw, sgn, lane, swr, sww);
     } else {
-      wave_stage<T, 3, 2>(z, pl.n, p, tw, sgn, lane);
+      wave_stage<T, 3, 2>(z, pl.n, p, tw, sgn, lane, swr, sww);
     }
   }
 }
@@ -198,21 +241,26 @@ __device__ __forceinline__ void tw_load(TwReg<T, N>& t, const Cplx<T>* __restric
 
 template <typename T, int N, int S>
 __device__ __forceinline__ void wave_fft_s(Cplx<T>* z, const Cplx<T>* w, bool inverse, int lane) {
+  static_assert(N % 16 == 0, "the LDS swizzle works on blocks of 16 elements");
   if constexpr (S < cp_nst(N)) {
     constexpr int R = cp_R(N, S), P = cp_P(N, S), MB = cp_MB(N, S), BASE = cp_BASE(N, S);
     constexpr int M = N / R;
+    constexpr bool SWR = S > 0, SWW = S < cp_nst(N) - 1;
     const int sgn = inverse ? 1 : -1;
     Cplx<T> x[MB][4];
+    // addresses are recomputed in each stage, not kept live across stages
+    __asm__ volatile("" : "+v"(lane));
 #pragma unroll
     for (int b = 0; b < MB; ++b) {
       const int i = lane + LES_WAVE * b;
       if (i < M) {
-        x[b][0] = z[i];
+        x[b][0] = lds_ld(z + (SWR ? lds_sw(i) : i));
 #pragma unroll
         for (int r = 1; r < R; ++r) {
           Cplx<T> tw = w[BASE + b * (R - 1) + r - 1];
           if (inverse) tw.y = -tw.y;
-          x[b][r] = c_mul(z[i + r * M], tw);
+          const int e = i + r * M;
+          x[b][r] = c_mul(lds_ld(z + (SWR ? lds_sw(e) : e)), tw);
         }
       }
     }
@@ -225,7 +273,10 @@ __device__ __forceinline__ void wave_fft_s(Cplx<T>* z, const Cplx<T>* w, bool in
         bfly(x[b], R, sgn);
         const int j = (i - k) * R + k;
 #pragma unroll
-        for (int r = 0; r < R; ++r) z[j + r * P] = x[b][r];
+        for (int r = 0; r < R; ++r) {
+          const int e = j + r * P;
+          lds_st(z + (SWW ? lds_sw(e) : e), x[b][r]);
+        }
       }
     }
     wave_sync();
