@@ -142,7 +142,7 @@ struct ColArgs {
   const T* g1;
   int lhx, ny;
   int k0, nplanes;
-  int ls;        // LDS line stride (ny + 1: keeps the transposed stores off one bank)
+  int ls;        // LDS line stride (at least ny, chosen so the tile transposes hit distinct banks)
   int ncb;       // column blocks per plane
   int nin, nout;
   int op;
@@ -461,7 +461,7 @@ struct ColResArgs {
   int W, h1, h2;
   int k0, nplanes;
   long src_plane, dst_plane;
-  int ls;   // LDS line stride (max(h1, h2) + 1)
+  int ls;   // LDS line stride (at least max(h1, h2))
   int ncp;  // column-pair tiles per plane
 };
 

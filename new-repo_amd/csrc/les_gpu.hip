@@ -159,6 +159,19 @@ static void rows_c2r(Les& L, const Cplx<T>* const* src, TO* const* dst, int nf, 
 #ifndef LES_COL_TILE_BYTES
 #define LES_COL_TILE_BYTES 32
 #endif
+
+// LDS line stride (in line elements of eb bytes, at least n) for a tile of
+// `lines` lines filled and drained row by row (consecutive lanes take
+// consecutive lines, then the next row).  One LDS cycle serves 128 bytes:
+// g = 128 / eb lanes spread over r = g / lines rows.  A stride of r (mod g)
+// puts line c, row j at bank slot c r + j, all distinct.
+static int lds_stride(int n, int lines, int eb) {
+  const int g = 128 / eb;
+  const int r = lines >= g ? 1 : g / lines;
+  int st = n;
+  while (st % g != r % g) ++st;
+  return st;
+}
 template <typename T>
 static int col_tile(int nin) {
   int cb = (nin > 1 ? LES_COL_TILE_BYTES / 2 : LES_COL_TILE_BYTES) / (int)sizeof(Cplx<T>);
@@ -197,10 +210,10 @@ static void cols(Les& L, int op, const Cplx<T>* in0, const Cplx<T>* in1, Cplx<T>
   a.kky = sizeof(T) == 8 ? (const T*)L.kky64 : (const T*)L.kky32;
   a.g0 = g0; a.g1 = g1;
   a.lhx = L.lhx; a.ny = L.ny; a.k0 = k0; a.nplanes = npl;
-  a.ls = L.ny + 1;
   a.nin = in1 ? 2 : 1;
   a.nout = nout; a.op = op; a.fwd = fwd; a.inv = inv;
   const int cb = col_tile<T>(a.nin);
+  a.ls = lds_stride(L.ny, cb, (int)sizeof(Cplx<T>));
   a.ncb = (L.lhx + cb - 1) / cb;
   a.ntiles = a.ncb * npl;
   kt_begin(L, KT_COLS);

@@ -60,7 +60,7 @@ template <typename T>
 static void launch_cols_resample(Les& L, ColResArgs<T>& a, int nf, const FftPlan& p1, const FftPlan& p2) {
   constexpr int CP = LES_COLRES_TILE_BYTES / 2 / (int)sizeof(T);
   const int nm = a.h1 > a.h2 ? a.h1 : a.h2;
-  a.ls = nm + 1;
+  a.ls = lds_stride(nm, CP, (int)sizeof(Cplx<T>));
   a.ncp = (a.W / 2 + CP - 1) / CP;
   const size_t sh = (size_t)CP * a.ls * sizeof(Cplx<T>);
   hipLaunchKernelGGL((k_cols_resample<T, CP>), dim3((unsigned)(a.ncp * a.nplanes), nf), dim3(LES_BLOCK), sh, L.stream, a,
