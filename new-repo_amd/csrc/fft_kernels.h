@@ -27,10 +27,23 @@ struct RowR2CArgs {
   int k0, nplanes;  // planes [k0, k0+nplanes)
   int mode;
   T scale, div;
+  int ls;           // LDS line stride per wavefront (row_line<T>(nx))
 };
 
+// LDS layout of the row transforms: the register-twiddle 256-point
+// transform in FP32 uses the padded placement (these kernels are bound by
+// integer address work), everything else the XOR placement on nx points.
+template <typename T>
+__host__ __device__ constexpr int row_layout(int n) {
+  return (sizeof(T) == 4 && n == 256) ? LES_LAY_PAD : LES_LAY_XOR;
+}
+template <typename T>
+__host__ __device__ constexpr int row_line(int n) {
+  return row_layout<T>(n) == LES_LAY_PAD ? lds_line_cap(n) : n;
+}
+
 // One wavefront per row pair: load, transform and unpack without any
-// block-level barrier.  LDS: one line of nx points per wavefront.
+// block-level barrier.  LDS: one line of a.ls points per wavefront.
 template <typename TI, typename T, typename FFT>
 __device__ __forceinline__ void rows_r2c_body(const RowR2CArgs<TI, T>& a, Cplx<T>* z, int lane, long job, FFT&& fft) {
   const int nx = a.nx, lhx = a.lhx;
@@ -65,13 +78,13 @@ template <typename TI, typename T>
 __global__ void __launch_bounds__(LES_BLOCK) k_rows_r2c(RowR2CArgs<TI, T> a, FftPlan pl, const Cplx<T>* __restrict__ tw) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int lane = threadIdx.x & (LES_WAVE - 1), wave = threadIdx.x / LES_WAVE;
-  Cplx<T>* z = reinterpret_cast<Cplx<T>*>(smem_raw) + wave * a.nx;
+  Cplx<T>* z = reinterpret_cast<Cplx<T>*>(smem_raw) + wave * a.ls;
   const long job = (long)blockIdx.x * (blockDim.x / LES_WAVE) + wave;
   if (job >= (long)a.nplanes * (a.ny / 2)) return;
   if (pl.n == 256) {
     TwReg<T, 256> tr;
     tw_load(tr, tw + pl.twf, lane);
-    rows_r2c_body(a, z, lane, job, [&](Cplx<T>* p, bool inv) { wave_fft_t(p, tr, inv, lane); });
+    rows_r2c_body(a, z, lane, job, [&](Cplx<T>* p, bool inv) { wave_fft_t<T, 256, row_layout<T>(256)>(p, tr, inv, lane); });
   } else {
     rows_r2c_body(a, z, lane, job, [&](Cplx<T>* p, bool inv) { wave_fft(p, pl, tw, inv, lane); });
   }
@@ -84,6 +97,7 @@ struct RowC2RArgs {
   TO* dst[LES_MAX_FIELDS];
   int nx, ny, lhx;
   int k0, nplanes;
+  int ls;  // LDS line stride per wavefront (row_line<T>(nx))
 };
 
 template <typename T, typename TO, typename FFT>
@@ -116,13 +130,13 @@ template <typename T, typename TO>
 __global__ void __launch_bounds__(LES_BLOCK) k_rows_c2r(RowC2RArgs<T, TO> a, FftPlan pl, const Cplx<T>* __restrict__ tw) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int lane = threadIdx.x & (LES_WAVE - 1), wave = threadIdx.x / LES_WAVE;
-  Cplx<T>* z = reinterpret_cast<Cplx<T>*>(smem_raw) + wave * a.nx;
+  Cplx<T>* z = reinterpret_cast<Cplx<T>*>(smem_raw) + wave * a.ls;
   const long job = (long)blockIdx.x * (blockDim.x / LES_WAVE) + wave;
   if (job >= (long)a.nplanes * (a.ny / 2)) return;
   if (pl.n == 256) {
     TwReg<T, 256> tr;
     tw_load(tr, tw + pl.twf, lane);
-    rows_c2r_body(a, z, lane, job, [&](Cplx<T>* p, bool inv) { wave_fft_t(p, tr, inv, lane); });
+    rows_c2r_body(a, z, lane, job, [&](Cplx<T>* p, bool inv) { wave_fft_t<T, 256, row_layout<T>(256)>(p, tr, inv, lane); });
   } else {
     rows_c2r_body(a, z, lane, job, [&](Cplx<T>* p, bool inv) { wave_fft(p, pl, tw, inv, lane); });
   }
@@ -378,7 +392,19 @@ struct RowResArgs {
   int n1, n2, ny;
   int k0, nplanes;
   long src_plane, dst_plane;  // plane strides (elements)
+  int ls;                     // LDS line stride per wavefront (res_line<T>(n1, n2))
 };
+
+// The FP32 256 <-> 384 pair runs on register twiddles with the padded
+// placement; other lengths on the generic transform (XOR placement).
+template <typename T>
+__host__ __device__ constexpr bool res_fixed(int n1, int n2) {
+  return sizeof(T) == 4 && ((n1 == 256 && n2 == 384) || (n1 == 384 && n2 == 256));
+}
+template <typename T>
+__host__ __device__ constexpr int res_line(int n1, int n2) {
+  return res_fixed<T>(n1, n2) ? lds_line_cap(n1 > n2 ? n1 : n2) : (n1 > n2 ? n1 : n2);
+}
 
 template <typename T, typename TI, typename TO, typename RES>
 __device__ __forceinline__ void rows_resample_body(const RowResArgs<T, TI, TO>& a, Cplx<T>* z, int lane, long job,
@@ -412,33 +438,32 @@ template <typename T, typename TI, typename TO>
 __global__ void __launch_bounds__(LES_BLOCK) k_rows_resample(RowResArgs<T, TI, TO> a, FftPlan p1, FftPlan p2,
                                                              const Cplx<T>* __restrict__ tw) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int nm = a.n1 > a.n2 ? a.n1 : a.n2;
   const int lane = threadIdx.x & (LES_WAVE - 1), wave = threadIdx.x / LES_WAVE;
-  Cplx<T>* z = reinterpret_cast<Cplx<T>*>(smem_raw) + wave * nm;
+  Cplx<T>* z = reinterpret_cast<Cplx<T>*>(smem_raw) + wave * a.ls;
   const long job = (long)blockIdx.x * (blockDim.x / LES_WAVE) + wave;
   if (job >= (long)a.nplanes * (a.ny / 2)) return;
   if constexpr (sizeof(T) == 4) {
-    if (p1.n == 256 && p2.n == 384) {
+    if (res_fixed<T>(p1.n, p2.n) && p1.n == 256) {
       TwReg<T, 256> ta;
       TwReg<T, 384> tb;
       tw_load(ta, tw + p1.twf, lane);
       tw_load(tb, tw + p2.twf, lane);
       rows_resample_body(a, z, lane, job, [&](Cplx<T>* q) {
-        wave_fft_t(q, ta, false, lane);
+        wave_fft_t<T, 256, LES_LAY_PAD>(q, ta, false, lane);
         wave_remap(q, 256, 384, lane);
-        wave_fft_t(q, tb, true, lane);
+        wave_fft_t<T, 384, LES_LAY_PAD>(q, tb, true, lane);
       });
       return;
     }
-    if (p1.n == 384 && p2.n == 256) {
+    if (res_fixed<T>(p1.n, p2.n) && p1.n == 384) {
       TwReg<T, 384> ta;
       TwReg<T, 256> tb;
       tw_load(ta, tw + p1.twf, lane);
       tw_load(tb, tw + p2.twf, lane);
       rows_resample_body(a, z, lane, job, [&](Cplx<T>* q) {
-        wave_fft_t(q, ta, false, lane);
+        wave_fft_t<T, 384, LES_LAY_PAD>(q, ta, false, lane);
         wave_remap(q, 384, 256, lane);
-        wave_fft_t(q, tb, true, lane);
+        wave_fft_t<T, 256, LES_LAY_PAD>(q, tb, true, lane);
       });
       return;
     }

@@ -86,6 +86,18 @@ __device__ __forceinline__ void bfly(Cplx<T>* x, int R, int sgn) {
 // line is in natural order outside the transform.  Needs n % 16 == 0.
 __device__ __forceinline__ int lds_sw(int e) { return e ^ (((e >> 4) * 5) & 15); }
 
+// Alternative placement for the compile-time transforms: lds_pad(e) = e +
+// 2 (e >> 4) + (e >> 5), two pad slots after every 16 elements and one more
+// after every 32 (a line then holds lds_line_cap(n) elements).  Being
+// additive, it makes each access of a stage one per-lane base plus an
+// immediate offset, where the XOR costs a few integer instructions per
+// access; it leaves slightly more bank conflicts (about 1.1-1.25 of the
+// ideal cycles against 1.0-1.1).  The row kernels in FP32, integer-bound,
+// use it; everything else uses lds_sw.
+__host__ __device__ constexpr int lds_pad(int e) { return e + 2 * (e >> 4) + (e >> 5); }
+__host__ __device__ constexpr int lds_line_cap(int n) { return lds_pad(n - 1) + 1; }
+enum { LES_LAY_XOR = 0, LES_LAY_PAD = 1 };
+
 // One 8- or 16-byte LDS access per complex value (ds_read_b64 / _b128)
 // instead of two 4- or 8-byte ones.
 template <typename T>
@@ -239,55 +251,86 @@ __device__ __forceinline__ void tw_load(TwReg<T, N>& t, const Cplx<T>* __restric
   tw_load_s<T, N, 0>(t.w, twf, lane);
 }
 
-template <typename T, int N, int S>
+// Offsets lds_pad(W + r P) - lds_pad(W) of the writes of stage S are the
+// same for every butterfly W (checked at compile time for each plan used).
+template <int N, int S>
+constexpr bool cp_wsplit() {
+  constexpr int R = cp_R(N, S), P = cp_P(N, S), M = N / R;
+  for (int i = 0; i < M; ++i) {
+    const int k = i % P, W = (i - k) * R + k;
+    for (int r = 0; r < R; ++r)
+      if (lds_pad(W + r * P) - lds_pad(W) != lds_pad(r * P)) return false;
+  }
+  return true;
+}
+
+template <typename T, int N, int S, int LAY>
 __device__ __forceinline__ void wave_fft_s(Cplx<T>* z, const Cplx<T>* w, bool inverse, int lane) {
-  static_assert(N % 16 == 0, "the LDS swizzle works on blocks of 16 elements");
+  static_assert(N % 16 == 0, "the LDS placements work on blocks of 16 elements");
   if constexpr (S < cp_nst(N)) {
     constexpr int R = cp_R(N, S), P = cp_P(N, S), MB = cp_MB(N, S), BASE = cp_BASE(N, S);
     constexpr int M = N / R;
     constexpr bool SWR = S > 0, SWW = S < cp_nst(N) - 1;
+    constexpr bool PAD = LAY == LES_LAY_PAD;
+    // padded reads lane + 64 b + r M split into lds_pad(lane) + lds_pad(64 b +
+    // r M) when the constant is a multiple of 32
+    static_assert(!PAD || !SWR || M % 32 == 0, "padded reads need M % 32 == 0");
+    static_assert(!PAD || !SWW || cp_wsplit<N, S>(), "padded write offsets must not depend on the lane");
     const int sgn = inverse ? 1 : -1;
     Cplx<T> x[MB][4];
-    // addresses are recomputed in each stage, not kept live across stages
-    __asm__ volatile("" : "+v"(lane));
-#pragma unroll
-    for (int b = 0; b < MB; ++b) {
-      const int i = lane + LES_WAVE * b;
-      if (i < M) {
-        x[b][0] = lds_ld(z + (SWR ? lds_sw(i) : i));
-#pragma unroll
-        for (int r = 1; r < R; ++r) {
-          Cplx<T> tw = w[BASE + b * (R - 1) + r - 1];
-          if (inverse) tw.y = -tw.y;
-          const int e = i + r * M;
-          x[b][r] = c_mul(lds_ld(z + (SWR ? lds_sw(e) : e)), tw);
-        }
-      }
+    if constexpr (!PAD) {
+      // XOR addresses are recomputed in each stage, not kept live across stages
+      __asm__ volatile("" : "+v"(lane));
     }
-    wave_sync();
+    const int rb = (PAD && SWR) ? lds_pad(lane) : lane;
 #pragma unroll
     for (int b = 0; b < MB; ++b) {
       const int i = lane + LES_WAVE * b;
       if (i < M) {
-        const int k = i % P;
-        bfly(x[b], R, sgn);
-        const int j = (i - k) * R + k;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-          const int e = j + r * P;
-          lds_st(z + (SWW ? lds_sw(e) : e), x[b][r]);
+          const int c = LES_WAVE * b + r * M;
+          int a;
+          if constexpr (PAD) a = rb + (SWR ? lds_pad(c) : c);
+          else a = SWR ? lds_sw(i + r * M) : i + r * M;
+          Cplx<T> v = lds_ld(z + a);
+          if (r > 0) {
+            Cplx<T> tw = w[BASE + b * (R - 1) + r - 1];
+            if (inverse) tw.y = -tw.y;
+            v = c_mul(v, tw);
+          }
+          x[b][r] = v;
         }
       }
     }
     wave_sync();
-    wave_fft_s<T, N, S + 1>(z, w, inverse, lane);
+#pragma unroll
+    for (int b = 0; b < MB; ++b) {
+      const int i = lane + LES_WAVE * b;
+      if (i < M) {
+        const int k = (int)((unsigned)i % (unsigned)P);
+        bfly(x[b], R, sgn);
+        const int j = (i - k) * R + k;
+        const int wb = (PAD && SWW) ? lds_pad(j) : j;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          int a;
+          if constexpr (PAD) a = wb + (SWW ? lds_pad(r * P) : r * P);
+          else a = SWW ? lds_sw(j + r * P) : j + r * P;
+          lds_st(z + a, x[b][r]);
+        }
+      }
+    }
+    wave_sync();
+    wave_fft_s<T, N, S + 1, LAY>(z, w, inverse, lane);
   }
 }
 
-// Same stages, arithmetic and order as wave_fft, with register twiddles.
-template <typename T, int N>
+// Same stages, arithmetic and order as wave_fft, with register twiddles; the
+// line holds n (LES_LAY_XOR) or lds_line_cap(n) (LES_LAY_PAD) elements.
+template <typename T, int N, int LAY = LES_LAY_XOR>
 __device__ __forceinline__ void wave_fft_t(Cplx<T>* z, const TwReg<T, N>& t, bool inverse, int lane) {
-  wave_fft_s<T, N, 0>(z, t.w, inverse, lane);
+  wave_fft_s<T, N, 0, LAY>(z, t.w, inverse, lane);
 }
 
 // Block barrier for LDS exchange only: waits for the wavefront's LDS

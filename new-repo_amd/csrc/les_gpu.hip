@@ -129,9 +129,10 @@ static void rows_r2c(Les& L, const TI* const* src, Cplx<T>* const* dst, int nf, 
   for (int f = 0; f < nf; ++f) { a.src[f] = src[f]; a.dst[f] = dst[f]; }
   a.nx = L.nx; a.ny = ny_override ? ny_override : L.ny; a.lhx = L.lhx; a.k0 = k0; a.nplanes = npl;
   a.mode = mode; a.scale = scale; a.div = div;
+  a.ls = row_line<T>(L.nx);
   long jobs = (long)npl * (a.ny / 2);
   dim3 grid((unsigned)((jobs + kWavesPerBlock - 1) / kWavesPerBlock), nf);
-  size_t sh = (size_t)kWavesPerBlock * L.nx * sizeof(Cplx<T>);
+  size_t sh = (size_t)kWavesPerBlock * a.ls * sizeof(Cplx<T>);
   kt_begin(L, KT_ROWS_R2C);
   hipLaunchKernelGGL((k_rows_r2c<TI, T>), grid, dim3(LES_BLOCK), sh, L.stream, a, L.px, twpool<T>(L));
   kt_end(L, KT_ROWS_R2C, (double)nf * npl * a.ny * ((double)L.nx * sizeof(TI) + (double)L.lhx * sizeof(Cplx<T>)));
@@ -142,9 +143,10 @@ static void rows_c2r(Les& L, const Cplx<T>* const* src, TO* const* dst, int nf, 
   RowC2RArgs<T, TO> a;
   for (int f = 0; f < nf; ++f) { a.src[f] = src[f]; a.dst[f] = dst[f]; }
   a.nx = L.nx; a.ny = L.ny; a.lhx = L.lhx; a.k0 = k0; a.nplanes = npl;
+  a.ls = row_line<T>(L.nx);
   long jobs = (long)npl * (L.ny / 2);
   dim3 grid((unsigned)((jobs + kWavesPerBlock - 1) / kWavesPerBlock), nf);
-  size_t sh = (size_t)kWavesPerBlock * L.nx * sizeof(Cplx<T>);
+  size_t sh = (size_t)kWavesPerBlock * a.ls * sizeof(Cplx<T>);
   kt_begin(L, KT_ROWS_C2R);
   hipLaunchKernelGGL((k_rows_c2r<T, TO>), grid, dim3(LES_BLOCK), sh, L.stream, a, L.px, twpool<T>(L));
   kt_end(L, KT_ROWS_C2R, (double)nf * npl * L.ny * ((double)L.lhx * sizeof(Cplx<T>) + (double)L.nx * sizeof(TO)));

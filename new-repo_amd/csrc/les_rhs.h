@@ -83,9 +83,9 @@ static void convection(Les& L, double* cx, double* cy, double* cz) {
     for (int f = 0; f < 6; ++f) { a.src[f] = nullptr; a.dst[f] = (T*)L.wk_hyb[f]; }
     a.n1 = nx; a.n2 = nx2; a.ny = ny; a.k0 = 1; a.nplanes = nz + 1;
     a.src_plane = (long)nx * ny; a.dst_plane = (long)nx2 * ny;
-    const int nm = nx2;
+    a.ls = res_line<T>(nx, nx2);
     long jobs = (long)a.nplanes * (ny / 2);
-    size_t sh = (size_t)kWavesPerBlock * nm * sizeof(Cplx<T>);
+    size_t sh = (size_t)kWavesPerBlock * a.ls * sizeof(Cplx<T>);
     kt_begin(L, KT_ROWS_RES);
     hipLaunchKernelGGL((k_rows_resample<T, T, T>), dim3((unsigned)((jobs + kWavesPerBlock - 1) / kWavesPerBlock), 6),
                        dim3(LES_BLOCK), sh, L.stream, a, L.px, L.px2, twpool<T>(L));
@@ -132,8 +132,9 @@ static void convection(Les& L, double* cx, double* cy, double* cz) {
     a.src_mode = 0;
     a.n1 = nx2; a.n2 = nx; a.ny = ny; a.k0 = 1; a.nplanes = nz;
     a.src_plane = (long)nx2 * ny; a.dst_plane = L.np;
+    a.ls = res_line<T>(nx2, nx);
     long jobs = (long)nz * (ny / 2);
-    size_t sh = (size_t)kWavesPerBlock * nx2 * sizeof(Cplx<T>);
+    size_t sh = (size_t)kWavesPerBlock * a.ls * sizeof(Cplx<T>);
     kt_begin(L, KT_ROWS_RES);
     hipLaunchKernelGGL((k_rows_resample<T, T, double>), dim3((unsigned)((jobs + kWavesPerBlock - 1) / kWavesPerBlock), 3),
                        dim3(LES_BLOCK), sh, L.stream, a, L.px2, L.px, twpool<T>(L));
