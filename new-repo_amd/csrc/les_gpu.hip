@@ -691,7 +691,8 @@ int les_init(const les_params* p) {
   for (int i = 0; i < 6; ++i) L->wk_big[i] = dalloc(*L, tb * (size_t)L->nbp * L->nzg);
   for (int i = 0; i < 3; ++i) L->wk_prod[i] = dalloc(*L, tb * (size_t)L->nbp * L->nzg);
   for (int i = 0; i < 3; ++i) L->wk_hyb3[i] = dalloc(*L, tb * (size_t)L->nx2 * L->ny * L->nzg);
-  for (int i = 0; i < 5; ++i) L->pspec[i] = (double*)dalloc(*L, 16 * (size_t)L->nsp);
+  for (int i = 0; i < 4; ++i) L->pspec[i] = (double*)dalloc(*L, 16 * (size_t)L->nsp);
+  L->tri_gam = (double*)dalloc(*L, 8 * (size_t)L->ns * (L->nz + 3));
   for (int i = 0; i < 4; ++i) L->playr[i] = (double*)dalloc(*L, 16 * (size_t)L->ns);
   for (int i = 0; i < 6; ++i) L->wk_layer[i] = dalloc(*L, 2 * tb * (size_t)L->ns + 2 * tb * L->np);
   for (int i = 0; i < 4; ++i) L->Fold[i] = (double*)dalloc(*L, 8 * L->nf);

@@ -11,20 +11,36 @@ struct TriArgs {
   const Cplx<double> *bot, *top;      // BC layer spectra
   const double *kkx, *kky;
   Cplx<double>* phat;                 // planes 0..nz+1
-  double* gam;                        // (nz+3) * ns scratch
+  double* gam;                        // (nz+3) * ns scratch, rows k at k * ns
   int lhx, ny, nz;
   long ns;
   double dz, idz2;
 };
 
-__global__ void k_tridiag(TriArgs a) {
-  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+// Two lanes per mode, one per component: given the real factors bet and gam
+// the recurrences of the real and imaginary parts are independent, so the
+// pair computes the same factors and one component each (twice the
+// parallelism of one thread per mode, whose few waves leave most SIMDs idle;
+// adjacent lanes read and write the two halves of each complex value).  The
+// sweeps are sequential in z but their loads are not: each pass over a chunk
+// of LES_TRI_U levels issues the chunk's loads first.  Per component the
+// arithmetic and its order are those of the plain sweep.
+#define LES_TRI_U 8
+__global__ void __launch_bounds__(64) k_tridiag(TriArgs a) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long q = t >> 1;
+  const int c = (int)(t & 1);
   if (q >= a.ns) return;
   const int i = (int)(q % a.lhx), j = (int)(q / a.lhx);
   const int nz = a.nz;
   const long ns = a.ns;
+  double* __restrict__ ph = reinterpret_cast<double*>(a.phat) + c;   // component c of phat
+  double* __restrict__ gam = a.gam;
+  const double* __restrict__ Hxo = reinterpret_cast<const double*>(a.Hx) + (1 - c);
+  const double* __restrict__ Hyo = reinterpret_cast<const double*>(a.Hy) + (1 - c);
+  const double* __restrict__ Hzc = reinterpret_cast<const double*>(a.Hz) + c;
   if (i == a.lhx - 1 || j == a.ny / 2) {
-    for (int k = 0; k <= nz + 1; ++k) a.phat[(long)k * ns + q] = {0.0, 0.0};
+    for (int k = 0; k <= nz + 1; ++k) ph[2 * ((long)k * ns + q)] = 0.0;
     return;
   }
   const double dz = a.dz, idz2 = a.idz2;
@@ -32,46 +48,73 @@ __global__ void k_tridiag(TriArgs a) {
   const double bmid = -((kx * kx + ky * ky) + 2.0 / (dz * dz));
   const bool mean = (i == 0 && j == 0);
   // row 1: bottom layer (mean mode pinned)
-  Cplx<double> r1 = mean ? Cplx<double>{0.0, 0.0} : Cplx<double>{a.bot[q].x * dz, a.bot[q].y * dz};
+  const double r1 = mean ? 0.0 : reinterpret_cast<const double*>(a.bot)[2 * q + c] * dz;
   double bet = mean ? 1.0 : -1.0;
   const double c1 = mean ? 0.0 : 1.0;
-  Cplx<double> x = {r1.x / bet, r1.y / bet};
-  a.phat[q] = x;
-  Cplx<double> hz = a.Hz[ns + q];
-  for (int k = 2; k <= nz + 2; ++k) {
-    Cplx<double> r;
-    double am, bk;
-    if (k <= nz + 1) {
-      const long p = (long)(k - 1) * ns + q;
-      const Cplx<double> hx = a.Hx[p], hy = a.Hy[p];
-      const Cplx<double> hzp = (k - 1 < nz) ? a.Hz[p + ns] : Cplx<double>{0.0, 0.0};
-      const double sr = kx * hx.x + ky * hy.x;
-      const double si = kx * hx.y + ky * hy.y;
-      r = {-si + (hzp.x - hz.x) / dz, sr + (hzp.y - hz.y) / dz};
-      hz = hzp;
-      am = idz2;
-      bk = bmid;
-    } else {
-      r = {a.top[q].x * dz, a.top[q].y * dz};
-      am = -1.0;
-      bk = 1.0;
+  double x = r1 / bet;
+  ph[2 * q] = x;
+  double hz = Hzc[2 * (ns + q)];
+  // rows 2..nz+1 (interior levels 1..nz): r = i kx Hx + i ky Hy + dHz/dz
+  for (int k0 = 2; k0 <= nz + 1; k0 += LES_TRI_U) {
+    double hxs[LES_TRI_U], hys[LES_TRI_U], hzs[LES_TRI_U];
+#pragma unroll
+    for (int u = 0; u < LES_TRI_U; ++u) {
+      const int k = k0 + u;
+      if (k <= nz + 1) {
+        const long p = (long)(k - 1) * ns + q;
+        hxs[u] = Hxo[2 * p];
+        hys[u] = Hyo[2 * p];
+        hzs[u] = (k - 1 < nz) ? Hzc[2 * (p + ns)] : 0.0;
+      }
     }
-    const double cprev = (k == 2) ? c1 : idz2;
-    const double g = cprev / bet;
-    a.gam[(long)k * ns + q] = g;
+#pragma unroll
+    for (int u = 0; u < LES_TRI_U; ++u) {
+      const int k = k0 + u;
+      if (k <= nz + 1) {
+        // real part: -(kx Hx.y + ky Hy.y) + dHz.x/dz; imaginary: (kx Hx.x + ky Hy.x) + dHz.y/dz
+        const double sk = kx * hxs[u] + ky * hys[u];
+        const double r = (c == 0 ? -sk : sk) + (hzs[u] - hz) / dz;
+        hz = hzs[u];
+        const double cprev = (k == 2) ? c1 : idz2;
+        const double g = cprev / bet;
+        gam[(long)k * ns + q] = g;  // both lanes of the pair store the same value
+        bet = bmid - idz2 * g;
+        x = (r - idz2 * x) / bet;
+        ph[2 * ((long)(k - 1) * ns + q)] = x;
+      }
+    }
+  }
+  // row nz+2: top layer
+  {
+    const int k = nz + 2;
+    const double r = reinterpret_cast<const double*>(a.top)[2 * q + c] * dz;
+    const double am = -1.0, bk = 1.0;
+    const double g = idz2 / bet;
+    gam[(long)k * ns + q] = g;
     bet = bk - am * g;
-    x = {(r.x - am * x.x) / bet, (r.y - am * x.y) / bet};
-    if (k <= nz + 1) a.phat[(long)(k - 1) * ns + q] = x;
+    x = (r - am * x) / bet;
   }
-  // back substitution: x holds x(nz+2)
-  for (int k = nz + 1; k >= 1; --k) {
-    const long p = (long)(k - 1) * ns + q;
-    const double g = a.gam[(long)(k + 1) * ns + q];
-    const Cplx<double> xk = a.phat[p];
-    x = {xk.x - g * x.x, xk.y - g * x.y};
-    a.phat[p] = x;
+  // back substitution: x holds x(nz+2); each lane reads back its own stores
+  for (int k1 = nz + 1; k1 >= 1; k1 -= LES_TRI_U) {
+    double gs[LES_TRI_U], xs[LES_TRI_U];
+#pragma unroll
+    for (int u = 0; u < LES_TRI_U; ++u) {
+      const int k = k1 - u;
+      if (k >= 1) {
+        gs[u] = gam[(long)(k + 1) * ns + q];
+        xs[u] = ph[2 * ((long)(k - 1) * ns + q)];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < LES_TRI_U; ++u) {
+      const int k = k1 - u;
+      if (k >= 1) {
+        x = xs[u] - gs[u] * x;
+        ph[2 * ((long)(k - 1) * ns + q)] = x;
+      }
+    }
   }
-  a.phat[(long)(nz + 1) * ns + q] = {0.0, 0.0};
+  ph[2 * ((long)(nz + 1) * ns + q)] = 0.0;
 }
 
 template <typename T>
@@ -100,11 +143,11 @@ static void press_stag_array(Les& L, double tadv1) {
   t.Hx = Hx; t.Hy = Hy; t.Hz = Hz; t.bot = lb; t.top = lt;
   t.kkx = L.kkx64; t.kky = L.kky64;
   t.phat = ph;
-  t.gam = L.pspec[4];
+  t.gam = L.tri_gam;
   t.lhx = L.lhx; t.ny = L.ny; t.nz = nz; t.ns = L.ns;
   t.dz = L.dz; t.idz2 = 1.0 / (L.dz * L.dz);
   kt_begin(L, KT_TRIDIAG);
-  hipLaunchKernelGGL(k_tridiag, dim3(nblk(L.ns, 64)), dim3(64), 0, L.stream, t);
+  hipLaunchKernelGGL(k_tridiag, dim3(nblk(2 * L.ns, 64)), dim3(64), 0, L.stream, t);
   kt_end(L, KT_TRIDIAG, 16.0 * L.ns * (3.0 * nz + 2.0 + (nz + 2.0)));
   // gradient: i kx p, i ky p and p back to physical space (src/compute_pressure.f90:320-357)
   cols<double>(L, COP_PGRAD, ph, nullptr, Hx, Hy, Hz, 3, 0, 1, 0, nz + 2);

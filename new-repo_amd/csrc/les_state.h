@@ -49,7 +49,8 @@ struct Les {
   void* wk_big[6];        // (nx2, ny2, nzg)
   void* wk_prod[3];       // (nx2, ny2, nzg)
   void* wk_hyb3[3];       // (nx2, ny, nzg)
-  double* pspec[5];       // pressure spectra (double): Hx, Hy, Hz, phat, tmp
+  double* pspec[4];       // pressure spectra (double): Hx, Hy, Hz, phat
+  double* tri_gam;        // Thomas scratch factors, (nz+3) rows of ns
   double* playr[4];       // bottom/top layer spectra and scratch
   void* wk_layer[6];      // (nx, ny) planes of flow precision and spectra
   double* Fold[4];        // previous F_* for the Lagrangian gather
