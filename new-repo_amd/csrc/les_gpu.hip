@@ -256,19 +256,14 @@ static void grad_filt(Les& L, int fid, int dx_id, int dy_id) {
   }
 }
 
-template <typename T>
-static void nyquist_remove_fp64(Les& L, int fid);
+static void nyquist_remove_fp64(Les& L);
 
 template <typename T>
 static void compute_vel_grad(Les& L) {
   grad_filt<T>(L, LES_U, LES_DUDX, LES_DUDY);
   grad_filt<T>(L, LES_V, LES_DVDX, LES_DVDY);
   grad_filt<T>(L, LES_W, LES_DWDX, LES_DWDY);
-  if (L.mixed) {
-    nyquist_remove_fp64<T>(L, LES_U);
-    nyquist_remove_fp64<T>(L, LES_V);
-    nyquist_remove_fp64<T>(L, LES_W);
-  }
+  if (L.mixed) nyquist_remove_fp64(L);
   BcDdzArgs<T> a;
   a.u = D(L, LES_U); a.v = D(L, LES_V); a.w = D(L, LES_W);
   a.dudz = F<T>(L, LES_DUDZ); a.dvdz = F<T>(L, LES_DVDZ); a.dwdz = F<T>(L, LES_DWDZ);
@@ -395,14 +390,21 @@ static void sgs_stress(Les& L, int jtt) {
 // Pass 1: one block per band of 4 rows of a plane (one row per wavefront):
 // A_j = sum_i (-1)^i f_ij / nx by wavefront reduction, and the band's
 // alternating column partial sums P_i = sum_{j in band} (-1)^j f_ij.  Every
-// sum has a fixed order, so results are reproducible.
-__global__ void __launch_bounds__(LES_BLOCK) k_nyq_reduce(const double* __restrict__ f, double* __restrict__ A,
-                                                          double* __restrict__ P, int nx, int ny) {
+// sum has a fixed order, so results are reproducible.  blockIdx.y selects
+// the field (u, v, w in one launch).
+struct NyqArgs {
+  double* f[3];
+  double *A, *B, *C, *P;  // per field: A ny*nzg, B nx*nzg, C nzg, P (ny/4)*nx*nzg
+  int nx, ny, nzg;
+};
+
+__global__ void __launch_bounds__(LES_BLOCK) k_nyq_reduce(NyqArgs a) {
   __shared__ double part[LES_BLOCK / LES_WAVE][LES_MAX_N];
+  const int nx = a.nx, ny = a.ny, fld = blockIdx.y;
   const int lane = threadIdx.x & (LES_WAVE - 1), wave = threadIdx.x / LES_WAVE;
   const long row = (long)blockIdx.x * (LES_BLOCK / LES_WAVE) + wave;  // global row (k * ny + j)
   const int j = (int)(row % ny);
-  const double* __restrict__ fr = f + row * nx;
+  const double* __restrict__ fr = a.f[fld] + row * nx;
   const double sj = (j & 1) ? -1.0 : 1.0;
   double s = 0.0;
   for (int i = lane; i < nx; i += LES_WAVE) {
@@ -411,57 +413,71 @@ __global__ void __launch_bounds__(LES_BLOCK) k_nyq_reduce(const double* __restri
     part[wave][i] = sj * v;
   }
   for (int o = LES_WAVE / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, LES_WAVE);
-  if (lane == 0) A[row] = s / (double)nx;
+  if (lane == 0) a.A[(long)fld * ny * a.nzg + row] = s / (double)nx;
   __syncthreads();
+  double* __restrict__ P = a.P + (long)fld * (ny / 4) * nx * a.nzg;
   for (int i = threadIdx.x; i < nx; i += blockDim.x)
     P[(long)blockIdx.x * nx + i] = ((part[0][i] + part[1][i]) + part[2][i]) + part[3][i];
 }
 
-// Pass 2: B_i = sum of the ny/4 band partials / ny per column of each plane;
-// C = sum_j (-1)^j A_j / ny per plane (thread i = 0).
-__global__ void k_nyq_bc(const double* __restrict__ A, const double* __restrict__ P, double* __restrict__ B,
-                         double* __restrict__ C, int nx, int ny, int nplanes) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)nx * nplanes) return;
-  const int i = (int)(t % nx), k = (int)(t / nx);
-  const int nb = ny / 4;
-  double s = 0.0;
-  for (int b = 0; b < nb; ++b) s += P[((long)k * nb + b) * nx + i];
-  B[t] = s / (double)ny;
-  if (i == 0) {
-    double c = 0.0;
-    for (int j = 0; j < ny; ++j) c += (j & 1) ? -A[(long)k * ny + j] : A[(long)k * ny + j];
-    C[k] = c / (double)ny;
+// Pass 2, one block per plane and field: B_i = sum of the ny/4 band
+// partials / ny per column; C = sum_j (-1)^j A_j / ny, each thread summing
+// its rows j in order, then a fixed tree.
+__global__ void __launch_bounds__(LES_BLOCK) k_nyq_bc(NyqArgs a) {
+  const int nx = a.nx, ny = a.ny, nb = ny / 4, fld = blockIdx.y;
+  const long k = blockIdx.x;
+  const double* __restrict__ P = a.P + (long)fld * nb * nx * a.nzg;
+  const double* __restrict__ A = a.A + (long)fld * ny * a.nzg + k * ny;
+  double* __restrict__ B = a.B + (long)fld * nx * a.nzg + k * nx;
+  for (int i = threadIdx.x; i < nx; i += blockDim.x) {
+    double s = 0.0;
+    for (int b = 0; b < nb; ++b) s += P[(k * nb + b) * nx + i];
+    B[i] = s / (double)ny;
   }
+  double c = 0.0;
+  for (int jj = threadIdx.x; jj < ny; jj += blockDim.x) c += (jj & 1) ? -A[jj] : A[jj];
+  __shared__ double red[LES_BLOCK];
+  red[threadIdx.x] = c;
+  __syncthreads();
+  for (int h = LES_BLOCK / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) a.C[(long)fld * a.nzg + k] = red[0] / (double)ny;
 }
 
 // Pass 3: f -= (-1)^i A_j + (-1)^j B_i - (-1)^(i+j) C, one block per field row.
-__global__ void k_nyq_apply(double* __restrict__ f, const double* __restrict__ A, const double* __restrict__ B,
-                            const double* __restrict__ C, int nx, int ny) {
+__global__ void __launch_bounds__(LES_BLOCK) k_nyq_apply(NyqArgs a) {
+  const int nx = a.nx, ny = a.ny, fld = blockIdx.y;
   const long r = blockIdx.x;
   const int j = (int)(r % ny);
   const long k = r / ny;
   const double sj = (j & 1) ? -1.0 : 1.0;
-  const double a = A[r], c = C[k];
+  const double av = a.A[(long)fld * ny * a.nzg + r], c = a.C[(long)fld * a.nzg + k];
+  const double* __restrict__ B = a.B + (long)fld * nx * a.nzg + k * nx;
+  double* __restrict__ f = a.f[fld] + r * nx;
   for (int i = threadIdx.x; i < nx; i += blockDim.x) {
     const double si = (i & 1) ? -1.0 : 1.0;
-    f[r * nx + i] -= si * a + sj * B[k * nx + i] - si * sj * c;
+    f[i] -= si * av + sj * B[i] - si * sj * c;
   }
 }
 
-template <typename T>
-static void nyquist_remove_fp64(Les& L, int fid) {
-  double* f = D(L, fid);
-  double* A = L.nyq_rows;
-  double* B = A + (long)L.ny * L.nzg;
-  double* C = B + (long)L.nx * L.nzg;
-  double* P = (double*)L.wk_real[62];  // ny/4 band partials per column and plane: nf/4 doubles
+// Nyquist rows and columns of u, v, w removed in FP64 (three passes, the
+// three fields together).
+static void nyquist_remove_fp64(Les& L) {
+  NyqArgs a;
+  a.f[0] = D(L, LES_U); a.f[1] = D(L, LES_V); a.f[2] = D(L, LES_W);
+  a.A = L.nyq_rows;
+  a.B = a.A + 3L * L.ny * L.nzg;
+  a.C = a.B + 3L * L.nx * L.nzg;
+  a.P = L.nyq_part;
+  a.nx = L.nx; a.ny = L.ny; a.nzg = L.nzg;
   const long nrows = (long)L.ny * L.nzg;
   kt_begin(L, KT_POINTWISE);
-  hipLaunchKernelGGL(k_nyq_reduce, dim3((unsigned)(nrows / 4)), dim3(LES_BLOCK), 0, L.stream, f, A, P, L.nx, L.ny);
-  hipLaunchKernelGGL(k_nyq_bc, dim3(nblk((long)L.nx * L.nzg)), dim3(LES_BLOCK), 0, L.stream, A, P, B, C, L.nx, L.ny, L.nzg);
-  hipLaunchKernelGGL(k_nyq_apply, dim3((unsigned)nrows), dim3(LES_BLOCK), 0, L.stream, f, A, B, C, L.nx, L.ny);
-  kt_end(L, KT_POINTWISE, 24.0 * L.nf);
+  hipLaunchKernelGGL(k_nyq_reduce, dim3((unsigned)(nrows / 4), 3), dim3(LES_BLOCK), 0, L.stream, a);
+  hipLaunchKernelGGL(k_nyq_bc, dim3((unsigned)L.nzg, 3), dim3(LES_BLOCK), 0, L.stream, a);
+  hipLaunchKernelGGL(k_nyq_apply, dim3((unsigned)nrows, 3), dim3(LES_BLOCK), 0, L.stream, a);
+  kt_end(L, KT_POINTWISE, 3.0 * 24.0 * L.nf);
 }
 
 // advection step (src/LES_engine.f90:253-283)
@@ -696,7 +712,8 @@ int les_init(const les_params* p) {
   for (int i = 0; i < 4; ++i) L->playr[i] = (double*)dalloc(*L, 16 * (size_t)L->ns);
   for (int i = 0; i < 6; ++i) L->wk_layer[i] = dalloc(*L, 2 * tb * (size_t)L->ns + 2 * tb * L->np);
   for (int i = 0; i < 4; ++i) L->Fold[i] = (double*)dalloc(*L, 8 * L->nf);
-  L->nyq_rows = (double*)dalloc(*L, 8 * (size_t)(L->nx + L->ny + 2) * L->nzg);
+  L->nyq_rows = (double*)dalloc(*L, 8 * 3 * (size_t)(L->nx + L->ny + 2) * L->nzg);
+  L->nyq_part = (double*)dalloc(*L, 8 * 3 * (size_t)(L->ny / 4) * L->nx * L->nzg);
   L->nanflag = (int*)dalloc(*L, 64);
   L->dg_part = (double*)dalloc(*L, sizeof(double) * LES_DIAG_BLOCKS * DG_N);
   L->dg_out = (double*)dalloc(*L, sizeof(double) * 4);

@@ -54,7 +54,8 @@ struct Les {
   double* playr[4];       // bottom/top layer spectra and scratch
   void* wk_layer[6];      // (nx, ny) planes of flow precision and spectra
   double* Fold[4];        // previous F_* for the Lagrangian gather
-  double* nyq_rows;       // mixed-mode FP64 Nyquist removal partial sums
+  double* nyq_rows;       // mixed-mode FP64 Nyquist removal sums (A, B, C of u, v, w)
+  double* nyq_part;       // their band partial sums
   int* nanflag;
   int alloc_failed;
 
