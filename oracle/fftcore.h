@@ -15,6 +15,8 @@
 //     i < n/R with k = i mod p, inputs x_r = in[i + r n/R] are multiplied by
 //     the twiddle exp(sign 2 pi i r k / (p R)) (r >= 1), combined by a size-R
 //     DFT and stored at out[(i - k) R + k + r p];
+//   * the twiddle product a w is (fma(a.x, w.x, -(a.y w.y)), fma(a.x, w.y,
+//     a.y w.x)): fused multiply-adds, each rounded once;
 //   * twiddles are computed once on the host in double with cos/sin of
 //     sign * 2 pi (r k) / (p R);
 //   * real rows are transformed two at a time: z = a + i b, Z = FFT(z),
@@ -35,7 +37,7 @@ struct Cx {
 
 static inline Cx cadd(Cx a, Cx b) { return {a.x + b.x, a.y + b.y}; }
 static inline Cx csub(Cx a, Cx b) { return {a.x - b.x, a.y - b.y}; }
-static inline Cx cmul(Cx a, Cx w) { return {a.x * w.x - a.y * w.y, a.x * w.y + a.y * w.x}; }
+static inline Cx cmul(Cx a, Cx w) { return {std::fma(a.x, w.x, -(a.y * w.y)), std::fma(a.x, w.y, a.y * w.x)}; }
 static inline Cx cscale(Cx a, double s) { return {a.x * s, a.y * s}; }
 
 static const double kTwoPi = 6.283185307179586476925286766559;
