@@ -348,6 +348,163 @@ __global__ void __launch_bounds__(LES_BLOCK) k_cols_256(ColArgs<T> a, FftPlan pl
   cols_body_pipe<T, CB>(a, smem_raw, [&](Cplx<T>* p, bool inv) { wave_fft_t(p, tr, inv, lane); });
 }
 
+// ------------------------------------------------------------------ columns, ny = 256, in registers
+// The 256-point column transform as two register-resident halves.  The four
+// radix-4 Stockham stages (engine above) fall into 16 closed groups twice:
+// stages 0-1 map inputs a + 16 t (t < 16) of thread a to its outputs 16 a + c
+// (c < 16), and stages 2-3 map inputs k + 16 t of thread k to outputs
+// k + 16 t.  So a thread loads its column's rows a + 16 t straight from
+// global memory, runs stages 0-1 in registers, trades through LDS (16 x 16
+// transpose per column, row e at 17 (e >> 4) + (e & 15): conflict-free), runs
+// stages 2-3 in registers and holds spectral rows a + 16 t, which is again
+// the input layout of stages 0-1, so the spectral op and the inverse
+// transform continue in registers and the result is stored from them.  The
+// butterflies, twiddles and their order are those of the Stockham stages.
+// Block: 16 columns (128-byte row segments) x 16 row classes; one LDS
+// exchange per transform instead of one per stage plus two tile transposes.
+#define LES_C16_LS 273  // column stride in the exchange buffer (= 1 mod 16)
+static_assert(cp_OFF(256, 1) == 3 && cp_OFF(256, 2) == 15 && cp_OFF(256, 3) == 63, "256-point twiddle layout");
+
+// Twiddles are read from the plan's table (4 KB at most, cache-resident)
+// where they are used: stages 0-1 read the same entries in every thread,
+// stages 2-3 the thread's own.
+template <typename T>
+__device__ __forceinline__ Cplx<T> tw16(const Cplx<T>* __restrict__ twf, int idx, bool inverse) {
+  Cplx<T> w = twf[idx];
+  if (inverse) w.y = -w.y;
+  return w;
+}
+
+// stages 0-1: X[t] = z[a + 16 t] -> X[c] = z'[16 a + c]
+template <typename T>
+__device__ __forceinline__ void c16_pair_a(Cplx<T>* X, const Cplx<T>* __restrict__ twf, bool inverse) {
+  const int sgn = inverse ? 1 : -1;
+  Cplx<T> Y[4][4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    Cplx<T> x[4] = {X[m], X[m + 4], X[m + 8], X[m + 12]};
+#pragma unroll
+    for (int r = 1; r < 4; ++r) x[r] = c_mul(x[r], tw16(twf, r - 1, inverse));
+    bfly(x, 4, sgn);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) Y[m][r] = x[r];
+  }
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    Cplx<T> x[4] = {Y[0][b], Y[1][b], Y[2][b], Y[3][b]};
+#pragma unroll
+    for (int r = 1; r < 4; ++r) x[r] = c_mul(x[r], tw16(twf, 3 + (r - 1) * 4 + b, inverse));
+    bfly(x, 4, sgn);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) X[b + 4 * r] = x[r];
+  }
+}
+
+// stages 2-3: X[t] = z'[k + 16 t] -> X[t] = Z[k + 16 t]
+template <typename T>
+__device__ __forceinline__ void c16_pair_b(Cplx<T>* X, const Cplx<T>* __restrict__ twf, int k, bool inverse) {
+  const int sgn = inverse ? 1 : -1;
+  Cplx<T> V[4][4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    Cplx<T> x[4] = {X[q], X[q + 4], X[q + 8], X[q + 12]};
+#pragma unroll
+    for (int r = 1; r < 4; ++r) x[r] = c_mul(x[r], tw16(twf, 15 + (r - 1) * 16 + k, inverse));
+    bfly(x, 4, sgn);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) V[q][r] = x[r];
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    Cplx<T> x[4] = {V[0][r], V[1][r], V[2][r], V[3][r]};
+#pragma unroll
+    for (int rp = 1; rp < 4; ++rp) x[rp] = c_mul(x[rp], tw16(twf, 63 + (rp - 1) * 64 + k + 16 * r, inverse));
+    bfly(x, 4, sgn);
+#pragma unroll
+    for (int r2 = 0; r2 < 4; ++r2) X[r + 4 * r2] = x[r2];
+  }
+}
+
+// Whole transform of thread (c, a)'s 16 values through the exchange buffer
+// (ex: this block's buffer; the caller's barrier protects earlier readers).
+template <typename T>
+__device__ __forceinline__ void c16_fft(Cplx<T>* X, const Cplx<T>* __restrict__ twf, bool inverse, Cplx<T>* ex, int c,
+                                        int a) {
+  c16_pair_a(X, twf, inverse);
+  Cplx<T>* col = ex + c * LES_C16_LS;
+#pragma unroll
+  for (int cc = 0; cc < 16; ++cc) lds_st(col + 17 * a + cc, X[cc]);
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < 16; ++t) X[t] = lds_ld(col + a + 17 * t);
+  __syncthreads();
+  c16_pair_b(X, twf, a, inverse);
+}
+
+// col_op with the operation fixed at compile time (same arithmetic).
+template <typename T, int OP>
+__device__ __forceinline__ Cplx<T> c16_op(const ColArgs<T>& a, int o, int kx, int j, Cplx<T> x0, Cplx<T> x1) {
+  const long q = (long)j * a.lhx + kx;
+  const bool nyq = (kx == a.lhx - 1) || (j == a.ny / 2);
+  const Cplx<T> zero = {(T)0, (T)0};
+  if constexpr (OP == COP_GRAD) {
+    const Cplx<T> v = nyq ? zero : x0;
+    if (o == 0) return c_ik(a.kkx[q], v);
+    if (o == 1) return c_ik(a.kky[q], v);
+    return v;
+  } else if constexpr (OP == COP_DIV) {
+    const Cplx<T> v0 = nyq ? zero : x0;
+    const Cplx<T> v1 = nyq ? zero : x1;
+    return c_add(c_ik(a.kkx[q], v0), c_ik(a.kky[q], v1));
+  } else if constexpr (OP == COP_FILT2) {
+    return c_scale(x0, o == 0 ? a.g0[q] : a.g1[q]);
+  } else if constexpr (OP == COP_FILT1) {
+    return c_scale(x0, a.g0[q]);
+  } else if constexpr (OP == COP_PGRAD) {
+    if (o == 0) return c_ik(a.kkx[q], x0);
+    if (o == 1) return c_ik(a.kky[q], x0);
+    return x0;
+  } else {
+    return x0;
+  }
+}
+
+template <typename T, int NIN, int OP, int NOUT>
+__global__ void __launch_bounds__(LES_BLOCK) k_cols_c16(ColArgs<T> a, const Cplx<T>* __restrict__ twf) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  Cplx<T>* ex = reinterpret_cast<Cplx<T>*>(smem_raw);
+  const int c = threadIdx.x & 15, ar = threadIdx.x >> 4;  // column in the tile, row class
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int k = a.k0 + tile / a.ncb;
+  const int c0 = (tile % a.ncb) * 16;
+  const bool valid = c0 + c < a.lhx;
+  const int kx = valid ? c0 + c : a.lhx - 1;
+  const long base = (long)k * 256 * a.lhx + kx;
+  Cplx<T> X0[16], X1[16];
+#pragma unroll
+  for (int t = 0; t < 16; ++t) X0[t] = valid ? a.in[0][base + (long)(ar + 16 * t) * a.lhx] : Cplx<T>{(T)0, (T)0};
+  if constexpr (NIN > 1) {
+#pragma unroll
+    for (int t = 0; t < 16; ++t) X1[t] = valid ? a.in[1][base + (long)(ar + 16 * t) * a.lhx] : Cplx<T>{(T)0, (T)0};
+  }
+  if (a.fwd) {
+    c16_fft(X0, twf, false, ex, c, ar);
+    if constexpr (NIN > 1) c16_fft(X1, twf, false, ex, c, ar);
+  }
+#pragma unroll
+  for (int o = 0; o < NOUT; ++o) {
+    Cplx<T> Y[16];
+#pragma unroll
+    for (int t = 0; t < 16; ++t) Y[t] = c16_op<T, OP>(a, o, kx, ar + 16 * t, X0[t], NIN > 1 ? X1[t] : X0[t]);
+    if (a.inv) c16_fft(Y, twf, true, ex, c, ar);
+    if (valid) {
+      Cplx<T>* __restrict__ dst = a.out[o] + base;
+#pragma unroll
+      for (int t = 0; t < 16; ++t) dst[(long)(ar + 16 * t) * a.lhx] = Y[t];
+    }
+  }
+}
+
 // ------------------------------------------------------------------ 3/2 resampling
 // Padding keeps modes |k| < n/2 (Nyquist dropped) of a length-n1 line in a
 // length-n2 spectrum; truncation is the mirror image

@@ -183,10 +183,32 @@ static int col_tile(int nin) {
 // ny = 256 runs the persistent pipelined kernel body: a grid of (resident
 // blocks per CU) x (CUs), a multiple of the 8 XCDs; other lengths one block
 // per tile.
+// ny = 256: register-resident transform (k_cols_c16); LES_COL16=0 selects
+// the LDS-engine kernel (k_cols_256) instead.
+#ifndef LES_COL16
+#define LES_COL16 1
+#endif
 template <typename T, int CB>
 static void launch_cols(Les& L, const ColArgs<T>& a, int npl) {
   const size_t sh = (size_t)(a.nin + 1) * CB * a.ls * sizeof(Cplx<T>);
-  if (L.ny == 256) {
+  if (L.ny == 256 && LES_COL16) {
+    ColArgs<T> b = a;
+    b.ncb = (L.lhx + 15) / 16;
+    b.ntiles = b.ncb * npl;
+    const size_t sh16 = (size_t)16 * LES_C16_LS * sizeof(Cplx<T>);
+    const dim3 g((unsigned)b.ntiles);
+    const Cplx<T>* tw = twpool<T>(L) + L.py.twf;
+#define LES_C16(NIN, OP, NOUT) \
+  hipLaunchKernelGGL((k_cols_c16<T, NIN, OP, NOUT>), g, dim3(LES_BLOCK), sh16, L.stream, b, tw)
+    if (a.op == COP_GRAD && a.nout == 3) LES_C16(1, COP_GRAD, 3);
+    else if (a.op == COP_GRAD) LES_C16(1, COP_GRAD, 2);
+    else if (a.op == COP_DIV) LES_C16(2, COP_DIV, 1);
+    else if (a.op == COP_FILT2) LES_C16(1, COP_FILT2, 2);
+    else if (a.op == COP_FILT1) LES_C16(1, COP_FILT1, 1);
+    else if (a.op == COP_PGRAD) LES_C16(1, COP_PGRAD, 3);
+    else LES_C16(1, COP_COPY, 1);
+#undef LES_C16
+  } else if (L.ny == 256) {
     static int occ[2] = {0, 0};  // per input count
     int& o = occ[a.nin > 1];
     if (o == 0) {
@@ -524,6 +546,15 @@ static int les_step_t(Les& L) {
 }
 
 // ------------------------------------------------------------------ set-up
+template <typename T>
+static void c16_attrs(int big) {
+  const void* k[] = {(const void*)k_cols_c16<T, 1, COP_GRAD, 3>, (const void*)k_cols_c16<T, 1, COP_GRAD, 2>,
+                     (const void*)k_cols_c16<T, 2, COP_DIV, 1>,  (const void*)k_cols_c16<T, 1, COP_FILT2, 2>,
+                     (const void*)k_cols_c16<T, 1, COP_FILT1, 1>, (const void*)k_cols_c16<T, 1, COP_PGRAD, 3>,
+                     (const void*)k_cols_c16<T, 1, COP_COPY, 1>};
+  for (const void* f : k) hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+}
+
 static int setup_kernel_attrs() {
   const int big = 160 * 1024;
   hipFuncSetAttribute((const void*)k_rows_r2c<double, double>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
@@ -533,6 +564,8 @@ static int setup_kernel_attrs() {
   hipFuncSetAttribute((const void*)k_rows_c2r<float, float>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
   hipFuncSetAttribute((const void*)k_rows_c2r<float, double>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
   hipFuncSetAttribute((const void*)k_cols_256<double, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+  c16_attrs<double>(big);
+  c16_attrs<float>(big);
   hipFuncSetAttribute((const void*)k_cols_256<float, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
   hipFuncSetAttribute((const void*)k_cols<double, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
   hipFuncSetAttribute((const void*)k_cols<double, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
