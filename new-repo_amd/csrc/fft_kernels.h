@@ -682,6 +682,174 @@ __device__ __forceinline__ void cols_resample_body(const ColResArgs<T>& a, unsig
   }
 }
 
+// ------------------------------------------------------------------ column resampling 256 <-> 384, in registers
+// The 384-point transform (stages radix 4, 4, 4, 2, 3) splits like the
+// 256-point one: stages 0-1 map inputs a + 24 t (t < 16) of thread a (a < 24)
+// to outputs 16 a + c, stages 2-4 stay in the residue class k mod 16 (thread
+// k < 16 holds k + 16 t, t < 24).  A block resamples 16 complex lines (32
+// real columns, 128-byte row segments) with 384 threads (line l = tid % 16,
+// row class r = tid / 16): the 256-point halves use row classes 0..15, the
+// 384-point first half all 24.  Between the transforms the spectrum is
+// re-laid through LDS with the padding / truncation of wave_remap.
+#define LES_C24_LS 417  // line stride in the exchange buffer (>= c16_pad(383) + 1, = 1 mod 16)
+// exchange-buffer row e of a line: one pad slot after every 16 rows
+__device__ __forceinline__ int c16_pad(int e) { return e + (e >> 4); }
+static_assert(cp_OFF(384, 1) == 3 && cp_OFF(384, 2) == 15 && cp_OFF(384, 3) == 63 && cp_OFF(384, 4) == 127,
+              "384-point twiddle layout");
+
+// stages 2-4 of the 384-point transform: X[t] = z''[k + 16 t] -> X[t] = Z[k + 16 t]
+template <typename T>
+__device__ __forceinline__ void c24_pair_b(Cplx<T>* X, const Cplx<T>* __restrict__ twf, int k, bool inverse) {
+  const int sgn = inverse ? 1 : -1;
+  Cplx<T> V[24];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {  // stage 2 (radix 4, span 16): butterfly 16 q + k
+    Cplx<T> x[4] = {X[q], X[q + 6], X[q + 12], X[q + 18]};
+#pragma unroll
+    for (int r = 1; r < 4; ++r) x[r] = c_mul(x[r], tw16(twf, 15 + (r - 1) * 16 + k, inverse));
+    bfly(x, 4, sgn);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) V[4 * q + r] = x[r];
+  }
+#pragma unroll
+  for (int u = 0; u < 12; ++u) {  // stage 3 (radix 2, span 64): butterfly 16 u + k
+    Cplx<T> x[4] = {V[u], c_mul(V[u + 12], tw16(twf, 63 + 16 * (u % 4) + k, inverse)), V[u], V[u]};
+    bfly(x, 2, sgn);
+    X[8 * (u / 4) + (u % 4)] = x[0];
+    X[8 * (u / 4) + (u % 4) + 4] = x[1];
+  }
+#pragma unroll
+  for (int v = 0; v < 8; ++v) {  // stage 4 (radix 3, span 128): butterfly k + 16 v
+    Cplx<T> x[4] = {X[v], X[v + 8], X[v + 16], X[v]};
+#pragma unroll
+    for (int r = 1; r < 3; ++r) x[r] = c_mul(x[r], tw16(twf, 127 + (r - 1) * 128 + k + 16 * v, inverse));
+    bfly(x, 3, sgn);
+    V[v] = x[0];
+    V[v + 8] = x[1];
+    V[v + 16] = x[2];
+  }
+#pragma unroll
+  for (int t = 0; t < 24; ++t) X[t] = V[t];
+}
+
+// PAD: 256 -> 384 rows (forward 256, pad, inverse 384); else 384 -> 256.
+template <typename T, bool PAD>
+__global__ void __launch_bounds__(384) k_colres_c16(ColResArgs<T> a, const Cplx<T>* __restrict__ tw256,
+                                                    const Cplx<T>* __restrict__ tw384) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  Cplx<T>* ex = reinterpret_cast<Cplx<T>*>(smem_raw);
+  const int l = threadIdx.x & 15, rc = threadIdx.x >> 4;  // line, row class (0..23)
+  const int f = blockIdx.y;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int k = a.k0 + tile / a.ncp;
+  const int p0 = (tile % a.ncp) * 16;  // first line (column pair)
+  const bool valid = p0 + l < a.W / 2;
+  const long col = 2L * (valid ? p0 + l : 0);
+  const Cplx<T>* __restrict__ src = reinterpret_cast<const Cplx<T>*>(a.src[f] + (long)k * a.src_plane + col);
+  Cplx<T>* __restrict__ dst = reinterpret_cast<Cplx<T>*>(a.dst[f] + (long)k * a.dst_plane + col);
+  const long rs = a.W / 2;  // row stride in complex elements
+  Cplx<T>* line = ex + l * LES_C24_LS;
+  const Cplx<T> zero = {(T)0, (T)0};
+  Cplx<T> X[24];
+  if constexpr (PAD) {
+    // forward 256: rows rc + 16 t
+    if (rc < 16) {
+#pragma unroll
+      for (int t = 0; t < 16; ++t) X[t] = valid ? src[(long)(rc + 16 * t) * rs] : zero;
+      c16_pair_a(X, tw256, false);
+#pragma unroll
+      for (int c = 0; c < 16; ++c) lds_st(line + c16_pad(16 * rc + c), X[c]);
+    }
+    __syncthreads();
+    if (rc < 16) {
+#pragma unroll
+      for (int t = 0; t < 16; ++t) X[t] = lds_ld(line + c16_pad(rc + 16 * t));
+    }
+    __syncthreads();
+    if (rc < 16) {
+      c16_pair_b(X, tw256, rc, false);
+      // spectrum rows m = rc + 16 t of 256 -> rows of 384: m < 128 kept, 128
+      // (Nyquist) dropped, m > 128 -> m + 128; rows 128..256 zero
+#pragma unroll
+      for (int t = 0; t < 16; ++t) {
+        const int m = rc + 16 * t;
+        if (m != 128) {
+          const int e = m < 128 ? m : m + 128;
+          lds_st(line + c16_pad(e), X[t]);
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < 8; ++t) lds_st(line + c16_pad(128 + rc + 16 * t), zero);
+      if (rc == 0) lds_st(line + c16_pad(256), zero);
+    }
+    __syncthreads();
+    // inverse 384, first half: rows rc + 24 t
+#pragma unroll
+    for (int t = 0; t < 16; ++t) X[t] = lds_ld(line + c16_pad(rc + 24 * t));
+    __syncthreads();
+    c16_pair_a(X, tw384, true);
+#pragma unroll
+    for (int c = 0; c < 16; ++c) lds_st(line + c16_pad(16 * rc + c), X[c]);
+    __syncthreads();
+    if (rc < 16) {
+#pragma unroll
+      for (int t = 0; t < 24; ++t) X[t] = lds_ld(line + c16_pad(rc + 16 * t));
+      c24_pair_b(X, tw384, rc, true);
+      if (valid) {
+#pragma unroll
+        for (int t = 0; t < 24; ++t) dst[(long)(rc + 16 * t) * rs] = X[t];
+      }
+    }
+  } else {
+    // forward 384, first half: rows rc + 24 t
+#pragma unroll
+    for (int t = 0; t < 16; ++t) X[t] = valid ? src[(long)(rc + 24 * t) * rs] : zero;
+    c16_pair_a(X, tw384, false);
+#pragma unroll
+    for (int c = 0; c < 16; ++c) lds_st(line + c16_pad(16 * rc + c), X[c]);
+    __syncthreads();
+    if (rc < 16) {
+#pragma unroll
+      for (int t = 0; t < 24; ++t) X[t] = lds_ld(line + c16_pad(rc + 16 * t));
+    }
+    __syncthreads();
+    if (rc < 16) {
+      c24_pair_b(X, tw384, rc, false);
+      // spectrum rows m = rc + 16 t of 384 -> rows of 256: m < 128 kept,
+      // m > 256 -> m - 128, the rest dropped; row 128 zero
+#pragma unroll
+      for (int t = 0; t < 24; ++t) {
+        const int m = rc + 16 * t;
+        if (m < 128) lds_st(line + c16_pad(m), X[t]);
+        else if (m > 256) lds_st(line + c16_pad(m - 128), X[t]);
+      }
+      if (rc == 0) lds_st(line + c16_pad(128), zero);
+    }
+    __syncthreads();
+    // inverse 256
+    if (rc < 16) {
+#pragma unroll
+      for (int t = 0; t < 16; ++t) X[t] = lds_ld(line + c16_pad(rc + 16 * t));
+    }
+    __syncthreads();
+    if (rc < 16) {
+      c16_pair_a(X, tw256, true);
+#pragma unroll
+      for (int c = 0; c < 16; ++c) lds_st(line + c16_pad(16 * rc + c), X[c]);
+    }
+    __syncthreads();
+    if (rc < 16) {
+#pragma unroll
+      for (int t = 0; t < 16; ++t) X[t] = lds_ld(line + c16_pad(rc + 16 * t));
+      c16_pair_b(X, tw256, rc, true);
+      if (valid) {
+#pragma unroll
+        for (int t = 0; t < 16; ++t) dst[(long)(rc + 16 * t) * rs] = X[t];
+      }
+    }
+  }
+}
+
 template <typename T, int CP>
 __global__ void __launch_bounds__(LES_BLOCK) k_cols_resample(ColResArgs<T> a, FftPlan p1, FftPlan p2,
                                                              const Cplx<T>* __restrict__ tw) {

@@ -56,8 +56,29 @@ __global__ void k_set_planes(SetPlaneArgs s) {
 #ifndef LES_COLRES_TILE_BYTES
 #define LES_COLRES_TILE_BYTES 64
 #endif
+// FP32 256 <-> 384: register-resident kernel (k_colres_c16); LES_COLRES16=0
+// selects the LDS-engine kernel (k_cols_resample) instead.
+#ifndef LES_COLRES16
+#define LES_COLRES16 1
+#endif
 template <typename T>
 static void launch_cols_resample(Les& L, ColResArgs<T>& a, int nf, const FftPlan& p1, const FftPlan& p2) {
+  if constexpr (sizeof(T) == 4 && LES_COLRES16) {
+   if ((a.h1 == 256 && a.h2 == 384) || (a.h1 == 384 && a.h2 == 256)) {
+    // register-resident 256 <-> 384 (k_colres_c16)
+    a.ncp = (a.W / 2 + 15) / 16;
+    a.ls = LES_C24_LS;
+    const size_t sh = (size_t)16 * LES_C24_LS * sizeof(Cplx<T>);
+    const Cplx<T>* t256 = twpool<T>(L) + (p1.n == 256 ? p1.twf : p2.twf);
+    const Cplx<T>* t384 = twpool<T>(L) + (p1.n == 384 ? p1.twf : p2.twf);
+    const dim3 g((unsigned)(a.ncp * a.nplanes), nf);
+    if (a.h1 == 256)
+      hipLaunchKernelGGL((k_colres_c16<T, true>), g, dim3(384), sh, L.stream, a, t256, t384);
+    else
+      hipLaunchKernelGGL((k_colres_c16<T, false>), g, dim3(384), sh, L.stream, a, t256, t384);
+    return;
+   }
+  }
   constexpr int CP = LES_COLRES_TILE_BYTES / 2 / (int)sizeof(T);
   const int nm = a.h1 > a.h2 ? a.h1 : a.h2;
   a.ls = lds_stride(nm, CP, (int)sizeof(Cplx<T>));
