@@ -130,7 +130,8 @@ def report(args):
             c = sum(t[1] for t in tab if re.search(pat, t[0]))
             tot = sum(t[2] for t in tab if re.search(pat, t[0]))
             if c:
-                rows.append(f"| {f} | `{pat}` | {fam[f]['avg_ms']:.4f} | {tot / c / 1e6:.4f} | {c} |")
+                shown = pat.replace("|", "\\|")  # a literal bar inside a markdown table cell
+                rows.append(f"| {f} | `{shown}` | {fam[f]['avg_ms']:.4f} | {tot / c / 1e6:.4f} | {c} |")
     bb = json.loads(pathlib.Path(args.bench_json).read_text().splitlines()[-1])
     steps = [f"| {k} | {v['ms_per_step']:.3f} | {v['mpts']:.0f} |" for k, v in bb["variants"].items() if "mpts" in v]
     r = bb["roofline"]
