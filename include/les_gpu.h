@@ -32,7 +32,7 @@ extern "C" {
  * (src/parameters_IO.f90:240-298); replaces the namelist globals. */
 typedef struct les_params {
   int nx, ny, nz;
-  int sgs_model;   /* 0 cst_Smag, 2 dyn_LagScaleDep (src/compute_sgs.f90:149-175) */
+  int sgs_model;   /* 0 cst_Smag, 1 dyn_Smag, 2 dyn_LagScaleDep (src/compute_sgs.f90:149-175) */
   int cs_count;    /* LASD update period (src/compute_LagScaleDep.f90:237) */
   int dyn_init;    /* first LASD step */
   int lagran_init; /* restart with F_* present (src/parameters_IO.f90:77) */
@@ -117,7 +117,10 @@ int les_grad_velocity_filt(int f, int dfdx, int dfdy);
 int les_ddz_uv(int dfdz, int f);
 int les_ddz_w(int dfdz, int f);
 /* sgs_stress(jt, jtt) (src/compute_sgs.f90:105-247), includes
- * compute_LagScaleDep_coeff (src/compute_LagScaleDep.f90:204-298). */
+ * compute_DynSmag_coeff (src/compute_DynSmag.f90:61-245; updates when jt is a
+ * multiple of cs_count) and compute_LagScaleDep_coeff
+ * (src/compute_LagScaleDep.f90:204-298; on jtt).  les_step passes the number
+ * of steps since les_init as jt. */
 int les_sgs_stress(int jt, int jtt);
 /* compute_dealias_convec(cx, cy, cz) (src/compute_convective_padd.f90:115-262). */
 int les_compute_dealias_convec(int cx, int cy, int cz);

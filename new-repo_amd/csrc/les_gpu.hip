@@ -358,18 +358,27 @@ template <typename T>
 static void lasd_update(Les& L, int jtt, int nlev);
 template <typename T>
 static void learned_update(Les& L, int jtt);
-
 template <typename T>
-static void sgs_stress(Les& L, int jtt) {
+static void dynsmag_update(Les& L);
+
+// sgs_stress (src/compute_sgs.f90:105-247).  Model 0 cst_Smag, 1 dyn_Smag
+// (updated when the local step count jt is a multiple of cs_count,
+// src/compute_DynSmag.f90:70-80), 2 dyn_LagScaleDep (on the global count jtt,
+// src/compute_LagScaleDep.f90:237); both dynamic models hold Cs^2 = 0.03
+// while jtt < dyn_init.
+template <typename T>
+static void sgs_stress(Les& L, int jt, int jtt) {
   SgsArgs<T> a = sgs_args<T>(L);
   bool lasd = L.P.sgs_model == 2;
+  bool dyns = L.P.sgs_model == 1;
   bool acc = lasd && jtt >= L.P.dyn_init;
   bool update = lasd && jtt >= L.P.dyn_init && (jtt % L.P.cs_count) == 0;
+  bool dyn_update = dyns && jtt >= L.P.dyn_init && (jt % L.P.cs_count) == 0;
   if (L.P.sgs_model == 0) {
     CsFillArgs<T> c;
     c.cs = F<T>(L, LES_CS_OPT2); c.val = (T)(L.P.cs0 * L.P.cs0); c.np = L.np; c.nz = L.nz;
     hipLaunchKernelGGL((k_cs_fill<T>), dim3(nblk(L.np * (L.nz + 1))), dim3(LES_BLOCK), 0, L.stream, c);
-  } else if (lasd && jtt < L.P.dyn_init) {
+  } else if ((lasd || dyns) && jtt < L.P.dyn_init) {
     CsFillArgs<T> c;
     c.cs = F<T>(L, LES_CS_OPT2); c.val = (T)0.03; c.np = L.np; c.nz = L.nz;
     hipLaunchKernelGGL((k_cs_fill<T>), dim3(nblk(L.np * (L.nz + 1))), dim3(LES_BLOCK), 0, L.stream, c);
@@ -387,6 +396,12 @@ static void sgs_stress(Les& L, int jtt) {
       lasd_update<T>(L, jtt, L.nz + 1);
     }
     a.accumulate = 0;
+  } else if (dyn_update) {
+    a.write_sij = 1;
+    kt_begin(L, KT_POINTWISE);
+    hipLaunchKernelGGL((k_sij<T>), dim3(nblk(L.np * (L.nz + 1))), dim3(LES_BLOCK), 0, L.stream, a);
+    kt_end(L, KT_POINTWISE, (double)L.np * (L.nz + 1) * 16.0 * sizeof(T));
+    dynsmag_update<T>(L);
   } else {
     a.accumulate = acc ? 1 : 0;
   }
@@ -518,12 +533,13 @@ static void ab2(Les& L, double tadv1, double tadv2) {
 template <typename T>
 static int les_step_t(Les& L) {
   L.jt_total += 1;
+  L.jt += 1;
   const int jtt = L.jt_total;
   if (L.timing) hipEventRecord(L.ph_ev[0], L.stream);
   wall_law<T>(L);
   compute_vel_grad<T>(L);
   if (L.timing) hipEventRecord(L.ph_ev[1], L.stream);
-  sgs_stress<T>(L, jtt);
+  sgs_stress<T>(L, L.jt, jtt);
   if (L.timing) hipEventRecord(L.ph_ev[2], L.stream);
   compute_rhs_adv<T>(L);
   if (L.timing) hipEventRecord(L.ph_ev[3], L.stream);
@@ -612,7 +628,11 @@ int les_init(const les_params* p) {
     if (m != 1 || n % 4 || n < 8) { les_set_error("les_init: nx, ny must be 2^a 3^b, multiples of 4, >= 8"); return 10; }
   }
   if (P.nz < 3) { les_set_error("les_init: nz must be >= 3"); return 10; }
-  if (P.sgs_model != 0 && P.sgs_model != 2) { les_set_error("Invalid SGS model"); return 1; }
+  if (P.sgs_model < 0 || P.sgs_model > 2) { les_set_error("Invalid SGS model"); return 1; }
+  if (P.precision == 2 && P.sgs_model != 2) {
+    les_set_error("the learned SGS surrogate replaces dyn_LagScaleDep only");
+    return 1;
+  }
   if (P.cs_count < 1) { les_set_error("cs_count must be >= 1"); return 1; }
   if (P.wall_model != 0 && P.wall_model != 1) { les_set_error("Invalid wall law type"); return 1; }
   if (P.precision < 0 || P.precision > 2) { les_set_error("precision must be 0, 1 or 2"); return 1; }
@@ -634,6 +654,7 @@ int les_init(const les_params* p) {
     if (z / L->delta < P.learned_zmin) L->nband = k;
   }
   L->jt_total = 0;
+  L->jt = 0;
   L->F_coeff_init = P.lagran_init != 0;
   L->mixed = P.precision != 0;
   if (hipStreamCreate(&L->stream) != hipSuccess) { delete L; les_set_error("stream"); return 5; }
@@ -945,9 +966,8 @@ int les_grad_velocity_filt(int f, int dfdx, int dfdy) {
   return 0;
 }
 int les_sgs_stress(int jt, int jtt) {
-  (void)jt;
   if (!g_les) return 1;
-  if (g_les->mixed) sgs_stress<float>(*g_les, jtt); else sgs_stress<double>(*g_les, jtt);
+  if (g_les->mixed) sgs_stress<float>(*g_les, jt, jtt); else sgs_stress<double>(*g_les, jt, jtt);
   LES_CHECK(hipGetLastError());
   return 0;
 }

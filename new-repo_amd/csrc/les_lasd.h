@@ -240,3 +240,142 @@ static void lasd_update(Les& L, int jtt, int nlev) {
   hipMemsetAsync(L.f[LES_V_LAG], 0, tb, L.stream);
   hipMemsetAsync(L.f[LES_W_LAG], 0, tb, L.stream);
 }
+
+// ---------------------------------------------------------------- plane-averaged dynamic model
+// compute_dynamic_mum's plane averages and coefficient
+// (src/compute_DynSmag.f90:225-243 with compute_planar_average_wghost,
+// src/utility_tools.f90:100-132): for level k, the sums of L:M and M:M over
+// the plane in storage order (i fastest, then j) as one running FP64 sum
+// each, the order of the reference's loop, divided by nx*ny; CS_Z = num/den
+// (0 where den = 0), clipped at 0 below, written over the whole level.
+// One block per level.  All threads stage chunks of both planes in LDS with
+// coalesced loads, double-buffered so the next chunk is in flight while the
+// current one is summed; lane 0 of wave 0 adds the L:M chunk and lane 0 of
+// wave 1 the M:M chunk, in order, so the sums equal the sequential ones bit
+// for bit.  The add chain (np dependent FP64 adds per level) bounds the
+// kernel; it runs once per cs_count steps.
+#define LES_PS_CHUNK 2048
+template <typename T>
+struct DynPlaneArgs {
+  const T* LM;
+  const T* MM;
+  T* cs;
+  long np;
+  double npl;  // nx*ny as a real
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_dynsmag_plane(DynPlaneArgs<T> a) {
+  constexpr int PER = LES_PS_CHUNK / 256;
+  __shared__ double buf[2][2][LES_PS_CHUNK];
+  __shared__ double res[2];
+  const int t = threadIdx.x, wv = t >> 6, lane = t & 63;
+  const long base = (long)(blockIdx.x + 1) * a.np;
+  const T* s0 = a.LM + base;
+  const T* s1 = a.MM + base;
+  const long nch = (a.np + LES_PS_CHUNK - 1) / LES_PS_CHUNK;
+  double r0[PER], r1[PER];
+#pragma unroll
+  for (int m = 0; m < PER; ++m) {
+    const long i = (long)m * 256 + t;
+    r0[m] = i < a.np ? (double)s0[i] : 0.0;
+    r1[m] = i < a.np ? (double)s1[i] : 0.0;
+  }
+#pragma unroll
+  for (int m = 0; m < PER; ++m) { buf[0][0][m * 256 + t] = r0[m]; buf[0][1][m * 256 + t] = r1[m]; }
+  __syncthreads();
+  double s = 0.0;
+  for (long c = 0; c < nch; ++c) {
+    const bool more = c + 1 < nch;
+    if (more) {
+#pragma unroll
+      for (int m = 0; m < PER; ++m) {
+        const long i = (c + 1) * LES_PS_CHUNK + (long)m * 256 + t;
+        r0[m] = i < a.np ? (double)s0[i] : 0.0;
+        r1[m] = i < a.np ? (double)s1[i] : 0.0;
+      }
+    }
+    if (lane == 0 && wv < 2) {
+      const double* p = buf[c & 1][wv];
+      const long rem = a.np - c * LES_PS_CHUNK;
+      const int cnt = rem < LES_PS_CHUNK ? (int)rem : LES_PS_CHUNK;
+#pragma unroll 16
+      for (int i = 0; i < cnt; ++i) s = s + p[i];
+    }
+    if (more) {
+      const int b = (int)((c + 1) & 1);
+#pragma unroll
+      for (int m = 0; m < PER; ++m) { buf[b][0][m * 256 + t] = r0[m]; buf[b][1][m * 256 + t] = r1[m]; }
+    }
+    __syncthreads();
+  }
+  if (lane == 0 && wv < 2) res[wv] = s;
+  __syncthreads();
+  const double num = res[0] / a.npl, den = res[1] / a.npl;
+  double cz = den != 0.0 ? num / den : 0.0;
+  cz = cz > 0.0 ? cz : 0.0;
+  const T v = (T)cz;
+  T* out = a.cs + base;
+  for (long q = t; q < a.np; q += 256) out[q] = v;
+}
+
+// compute_DynSmag_coeff update (src/compute_DynSmag.f90:76-80) and
+// compute_dynamic_mum (:96-243): the Germano identity at the 2-delta test
+// filter only.  Inputs, filtering and the pointwise L:M, M:M are those of the
+// Lagrangian model's 2-delta half (the reference forms them identically);
+// the plane average replaces the Lagrangian relaxation.
+template <typename T>
+static void dynsmag_update(Les& L) {
+  const int nlev = L.nz + 1;
+  const long npt = L.np * nlev;
+  T* in[21];
+  for (int i = 0; i < 9; ++i) in[i] = (T*)L.wk_real[i];
+  in[9] = F<T>(L, LES_S11); in[10] = F<T>(L, LES_S12); in[11] = F<T>(L, LES_S13);
+  in[12] = F<T>(L, LES_S22); in[13] = F<T>(L, LES_S23); in[14] = F<T>(L, LES_S33);
+  for (int i = 15; i < 21; ++i) in[i] = (T*)L.wk_real[i - 6];
+  LasdInArgs<T> ia;
+  for (int i = 0; i < 9; ++i) ia.o[i] = in[i];
+  for (int i = 0; i < 6; ++i) ia.o[9 + i] = in[15 + i];
+  ia.u = D(L, LES_U); ia.v = D(L, LES_V); ia.w = D(L, LES_W);
+  for (int i = 0; i < 6; ++i) ia.s[i] = in[9 + i];
+  ia.smod = (T*)L.wk_real[63];
+  ia.np = L.np; ia.nz = L.nz; ia.nlev = nlev;
+  kt_begin(L, KT_POINTWISE);
+  hipLaunchKernelGGL((k_lasd_inputs<T>), dim3(nblk(npt)), dim3(LES_BLOCK), 0, L.stream, ia);
+  kt_end(L, KT_POINTWISE, (double)npt * (3 * 8.0 + 22.0 * sizeof(T)));
+  const T* g2 = sizeof(T) == 8 ? (const T*)L.g2_64 : (const T*)L.g2_32;
+  T* fl[21];
+  for (int f = 0; f < 21; ++f) fl[f] = (T*)L.wk_real[15 + f];
+  for (int f0 = 0; f0 < 21; f0 += 3) {
+    const T* src[3];
+    Cplx<T>* sp[3];
+    const Cplx<T>* cs[3];
+    T* outs[3];
+    for (int t = 0; t < 3; ++t) {
+      src[t] = in[f0 + t];
+      sp[t] = (Cplx<T>*)L.wk_spec[t];
+      cs[t] = (const Cplx<T>*)L.wk_spec[3 + t];
+      outs[t] = fl[f0 + t];
+    }
+    rows_r2c<T, T>(L, src, sp, 3, 1, nlev, (T)1);
+    for (int t = 0; t < 3; ++t)
+      cols<T>(L, COP_FILT1, sp[t], nullptr, (Cplx<T>*)L.wk_spec[3 + t], nullptr, nullptr, 1, 1, 1, 1, nlev, g2);
+    rows_c2r<T, T>(L, cs, outs, 3, 1, nlev);
+  }
+  GermanoArgs<T> ga;
+  for (int f = 0; f < 21; ++f) ga.fl[f] = fl[f];
+  ga.AB = (T*)L.wk_real[57];
+  ga.BB = (T*)L.wk_real[58];
+  ga.tf2 = (T)4.0;
+  ga.cst = (T)(2.0 * (L.delta * L.delta));
+  ga.np = L.np; ga.nz = L.nz; ga.nlev = nlev;
+  kt_begin(L, KT_POINTWISE);
+  hipLaunchKernelGGL((k_germano<T>), dim3(nblk(npt)), dim3(LES_BLOCK), 0, L.stream, ga);
+  kt_end(L, KT_POINTWISE, (double)npt * 23.0 * sizeof(T));
+  DynPlaneArgs<T> pa;
+  pa.LM = ga.AB; pa.MM = ga.BB; pa.cs = F<T>(L, LES_CS_OPT2);
+  pa.np = L.np; pa.npl = (double)(L.nx * L.ny);
+  kt_begin(L, KT_POINTWISE);
+  hipLaunchKernelGGL((k_dynsmag_plane<T>), dim3(nlev), dim3(256), 0, L.stream, pa);
+  kt_end(L, KT_POINTWISE, (double)npt * 3.0 * sizeof(T));
+}

@@ -21,6 +21,7 @@ struct Les {
   long nbp;              // nx2*ny2
   double dx, dy, dz, delta;
   int jt_total;
+  int jt;                // steps since les_init (the reference's local count)
   int F_coeff_init;
   hipStream_t stream;
   int mixed;             // flow kernels in FP32

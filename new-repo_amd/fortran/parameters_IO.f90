@@ -89,6 +89,8 @@ contains
     cparams%nx = nx; cparams%ny = ny; cparams%nz = nz
     if (trim(sgs_model) == 'dyn_LagScaleDep') then
        cparams%sgs_model = 2
+    else if (trim(sgs_model) == 'dyn_Smag') then
+       cparams%sgs_model = 1
     else if (trim(sgs_model) == 'cst_Smag') then
        cparams%sgs_model = 0
     else

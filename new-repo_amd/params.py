@@ -19,6 +19,7 @@ import numpy as np
 PI = 3.1415926535897932384626433
 
 SGS_CST_SMAG = 0
+SGS_DYN_SMAG = 1
 SGS_LASD = 2
 PREC_FP64 = 0
 PREC_MIXED = 1
@@ -84,8 +85,10 @@ class LesParams:
                 raise ValueError(f"horizontal size {n}: the 3/2-padded transform length is limited to 1024")
         if self.nz < 3:
             raise ValueError("nz must be >= 3")
-        if self.sgs_model not in (SGS_CST_SMAG, SGS_LASD):
-            raise ValueError("sgs_model must be cst_Smag or dyn_LagScaleDep")
+        if self.sgs_model not in (SGS_CST_SMAG, SGS_DYN_SMAG, SGS_LASD):
+            raise ValueError("sgs_model must be cst_Smag, dyn_Smag or dyn_LagScaleDep")
+        if self.precision == PREC_LEARNED and self.sgs_model != SGS_LASD:
+            raise ValueError("the learned SGS surrogate replaces dyn_LagScaleDep only")
         if self.cs_count < 1:
             raise ValueError("cs_count must be >= 1")
 
@@ -150,6 +153,8 @@ def params_from_namelist(text: str, precision: int = PREC_FP64) -> LesParams:
     model = str(sgs.get("sgs_model", "cst_Smag"))
     if model == "dyn_LagScaleDep":
         p.sgs_model = SGS_LASD
+    elif model == "dyn_Smag":
+        p.sgs_model = SGS_DYN_SMAG
     elif model == "cst_Smag":
         p.sgs_model = SGS_CST_SMAG
     else:

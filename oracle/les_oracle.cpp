@@ -54,7 +54,7 @@ static const double KVONK = 0.4;
 
 struct Params {
   int nx, ny, nz;
-  int sgs_model;    // 0 constant Smagorinsky, 2 Lagrangian scale-dependent
+  int sgs_model;    // 0 constant Smagorinsky, 1 plane-averaged dynamic Smagorinsky, 2 Lagrangian scale-dependent
   int cs_count, dyn_init, lagran_init;
   int wall_model;   // 0 wall_filter (2-delta filtered first level), 1 wall_inst
   int ubc_u_neu, ubc_v_neu;  // top BC type for u, v (1 Neumann, 0 Dirichlet)
@@ -81,6 +81,7 @@ struct Oracle {
   std::vector<double> lbc_u, lbc_v, bottom_wk, top_wk, ustar;
   std::vector<double> s11, s12, s13, s22, s23, s33;
   int jt_total = 0;
+  int jt = 0;  // steps since initialisation (the reference's jt, reset at every start)
   bool F_coeff_init = false;
   bool lagran_flag = false;
   // running statistics: 26 sums of (nx, ny, nz) and the u* sum
@@ -350,9 +351,13 @@ static void fwd_spec(const Oracle& o, const std::vector<double>& f, std::vector<
   for (int k = 1; k <= o.nz + 1; ++k) fft2_fwd(o, &f[o.I(0, 0, k)], &spec[ns * k], 1.0);
 }
 
-// compute_dynamic_mum for both filter widths (src/compute_LagScaleDep.f90:485-596)
+// compute_dynamic_mum for both filter widths (src/compute_LagScaleDep.f90:485-596);
+// with nscale = 1 only the 2-delta part (LM, MM), which is the pointwise half of
+// compute_dynamic_mum of the plane-averaged model (src/compute_DynSmag.f90:96-221:
+// same u_ft/v_ft/w_ft at the first node, same L_ij, M_ij = 2 delta^2
+// (filt(|S| S_ij) - 4 |S~| S~_ij), same L:M and M:M sums).
 static void dynamic_mum(Oracle& o, const std::vector<double>& Smod, std::vector<double>& LM, std::vector<double>& MM,
-                        std::vector<double>& QN, std::vector<double>& NN) {
+                        std::vector<double>& QN, std::vector<double>& NN, int nscale = 2) {
   int nz = o.nz;
   size_t nf = o.nf;
   // 21 inputs: u_ft v_ft w_ft, A11 A12 A13 A22 A23 A33, S11..S33, S*Sij
@@ -382,7 +387,7 @@ static void dynamic_mum(Oracle& o, const std::vector<double>& Smod, std::vector<
   std::vector<std::vector<Cx>> spec(21);
   for (int f = 0; f < 21; ++f) fwd_spec(o, in[f], spec[f]);
   double cst = 2.0 * (o.delta * o.delta);
-  for (int sc = 0; sc < 2; ++sc) {
+  for (int sc = 0; sc < nscale; ++sc) {
     const std::vector<double>& G = sc == 0 ? o.G2 : o.G4;
     double tf2 = sc == 0 ? 4.0 : 16.0;
     std::vector<std::vector<double>> fl(21, std::vector<double>(nf, 0.0));
@@ -573,8 +578,42 @@ static void lasd(Oracle& o, int jtt) {
   }
 }
 
+// compute_DynSmag_coeff (src/compute_DynSmag.f90:61-82) with the plane
+// averages of compute_dynamic_mum (:225-243): num(k), den(k) are the level sums
+// of L:M and M:M in i-fastest, then j order divided by nx*ny
+// (compute_planar_average_wghost, src/utility_tools.f90:100-132), CS_Z(k) =
+// num/den (0 where den = 0), clipped at 0 below, and Cs^2 = CS_Z(k) on the
+// whole level.  The update runs on the local step count jt, the reset to 0.03
+// on the global count jtt.
+static void dynsmag(Oracle& o, int jt, int jtt) {
+  int nz = o.nz;
+  if (jtt < o.P.dyn_init) {
+    for (int k = 1; k <= nz + 1; ++k)
+      for (size_t q = 0; q < o.np; ++q) o.cs_opt2[o.I(0, 0, k) + q] = 0.03;
+  } else if (jt % o.P.cs_count == 0) {
+    std::vector<double> Smod(o.nf, 0.0), LM(o.nf, 0.0), MM(o.nf, 0.0);
+    for (int k = 1; k <= nz + 1; ++k)
+      for (size_t q = 0; q < o.np; ++q) {
+        size_t a = o.I(0, 0, k) + q;
+        Smod[a] = smag(o.s11[a], o.s12[a], o.s13[a], o.s22[a], o.s23[a], o.s33[a]);
+      }
+    dynamic_mum(o, Smod, LM, MM, LM, MM, 1);
+    const double npl = (double)(o.nx * o.ny);
+    for (int k = 1; k <= nz + 1; ++k) {
+      double num = 0.0, den = 0.0;
+      for (size_t q = 0; q < o.np; ++q) num = num + LM[o.I(0, 0, k) + q];
+      for (size_t q = 0; q < o.np; ++q) den = den + MM[o.I(0, 0, k) + q];
+      num = num / npl;
+      den = den / npl;
+      double c = den != 0.0 ? num / den : 0.0;
+      c = c > 0.0 ? c : 0.0;
+      for (size_t q = 0; q < o.np; ++q) o.cs_opt2[o.I(0, 0, k) + q] = c;
+    }
+  }
+}
+
 // sgs_stress (src/compute_sgs.f90:105-247)
-static void sgs_stress(Oracle& o, int jtt) {
+static void sgs_stress(Oracle& o, int jt, int jtt) {
   int nz = o.nz;
   compute_sij(o);
   std::vector<double> l(nz + 2, o.delta);
@@ -588,6 +627,8 @@ static void sgs_stress(Oracle& o, int jtt) {
     }
     for (int k = 1; k <= nz + 1; ++k)
       for (size_t q = 0; q < o.np; ++q) o.cs_opt2[o.I(0, 0, k) + q] = o.P.cs0 * o.P.cs0;
+  } else if (o.P.sgs_model == 1) {
+    dynsmag(o, jt, jtt);
   } else {
     lasd(o, jtt);
   }
@@ -908,10 +949,11 @@ static void pressure(Oracle& o, double tadv1) {
 
 static void step(Oracle& o) {
   o.jt_total += 1;
+  o.jt += 1;
   int jtt = o.jt_total, nz = o.nz;
   wall_law(o);
   compute_vel_grad(o);
-  sgs_stress(o, jtt);
+  sgs_stress(o, o.jt, jtt);
   compute_rhs(o);
   double tadv1 = (jtt == 1) ? 1.0 : 1.5;
   double tadv2 = (jtt == 1) ? 0.0 : 1.0 - tadv1;
@@ -1141,6 +1183,7 @@ int orc_init(const int* ip, const double* rp) {
   o.lbc_u.assign(o.np, 0.0); o.lbc_v.assign(o.np, 0.0);
   o.bottom_wk.assign(o.np, 0.0); o.top_wk.assign(o.np, 0.0); o.ustar.assign(o.np, 0.0);
   o.jt_total = 0;
+  o.jt = 0;
   o.F_coeff_init = P.lagran_init != 0;
   o.lagran_flag = P.lagran_init != 0;
   return 0;
@@ -1279,7 +1322,7 @@ int orc_op(int which, double arg) {
   switch (which) {
     case 0: wall_law(o); break;
     case 1: compute_vel_grad(o); break;
-    case 2: sgs_stress(o, (int)arg); break;
+    case 2: sgs_stress(o, (int)arg, (int)arg); break;
     case 3: compute_rhs(o); break;
     case 4: pressure(o, arg); break;
     case 5: compute_sij(o); break;
@@ -1292,6 +1335,9 @@ int orc_op(int which, double arg) {
 }
 
 int orc_set_jt_total(int jt) { if (!g) return 1; g->jt_total = jt; return 0; }
+int orc_jt(void) { return g ? g->jt : -1; }
+// sgs_stress(jt, jtt) with the two step counts given separately
+int orc_sgs_stress(int jt, int jtt) { if (!g) return 1; sgs_stress(*g, jt, jtt); return 0; }
 
 // running statistics and diagnostics
 int orc_stat_reset(void) { if (!g) return 1; stat_reset(*g); return 0; }

@@ -42,7 +42,8 @@ def lib():
             "orc_trunc_plane": [ctypes.c_int, ctypes.c_int, d, d],
             "orc_tridiag": [ctypes.c_int, ctypes.c_int, d, d], "orc_table": [ctypes.c_int, d],
             "orc_op": [ctypes.c_int, ctypes.c_double], "orc_set_jt_total": [ctypes.c_int],
-            "orc_set_threads": [ctypes.c_int], "orc_jt_total": [],
+            "orc_set_threads": [ctypes.c_int], "orc_jt_total": [], "orc_jt": [],
+            "orc_sgs_stress": [ctypes.c_int, ctypes.c_int],
             "orc_stat_reset": [], "orc_stat_accumulate": [], "orc_stat_get": [d, d, i],
             "orc_stat_profiles": [d], "orc_diagnostics": [d],
         }.items():
@@ -170,6 +171,13 @@ class Oracle:
 
     def jt_total(self):
         return lib().orc_jt_total()
+
+    def jt(self):
+        """Steps taken since initialisation (the reference's local count jt)."""
+        return lib().orc_jt()
+
+    def sgs_stress(self, jt, jtt):
+        lib().orc_sgs_stress(int(jt), int(jtt))
 
     # ---- running statistics and diagnostics
     STATS = ["u", "v", "w", "p", "u2", "v2", "w2", "u3", "v3", "w3", "u4", "v4", "w4", "uv", "uw", "vw",

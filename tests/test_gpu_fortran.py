@@ -22,15 +22,15 @@ NLST = """&simulation_param  p_on_y=1 p_on_z=1 nx={nx} ny={ny} nz={nz} nt={nt} d
                    tke_flag=.false. start_tke=0 tke_period=100 checkpoint={ckpt} scheckpoint=100000 /
 &physical_param    XY_pi=.true. lx=2.0d0 ly=2.0d0 lz=1.0d0 u_scale=1.0d0 zi=1.0d0 lbc_dz0=1.0d-4 Re=1.0d5
                    pfx=1.0d0 pfy=0.0d0 coriolis_flag=.false. ug_dim=0.0d0 vg_dim=0.0d0 geostr_rot=.false. geostr_start=100000 /
-&sgs_param         Cs0=0.16d0 sgs_model='dyn_LagScaleDep' cs_count=5 dyn_init=1 /
+&sgs_param         Cs0=0.16d0 sgs_model='{sgs}' cs_count=5 dyn_init=1 /
 &momentum_bc       lbc_model='std' lbc_special='wall_law' ubc_model='std' ubc_special='std' wall_model='wall_filter' surface_files=.false. /
 &momentum_ic       ic_type='files' u0=0.0d0 v0=0.0d0 w0=0.0d0 u_star=1.0d0 dudz0=0.0d0 z_turb=0.5d0 vnf=1.0d0 /
 """
 
 
-def _run(d, nx, ny, nz, nt, restart=False, ckpt=100000, stats=None, step_out=None):
+def _run(d, nx, ny, nz, nt, restart=False, ckpt=100000, stats=None, step_out=None, sgs="dyn_LagScaleDep"):
     stat, start, period = (".true.", *stats) if stats else (".false.", 0, 100)
-    (d / "parameters.nlst").write_text(NLST.format(nx=nx, ny=ny, nz=nz, nt=nt, ckpt=ckpt,
+    (d / "parameters.nlst").write_text(NLST.format(nx=nx, ny=ny, nz=nz, nt=nt, ckpt=ckpt, sgs=sgs,
                                                    restart=".true." if restart else ".false.",
                                                    stat=stat, start=start, period=period,
                                                    output=".true." if step_out else ".false.",
@@ -122,3 +122,25 @@ def test_fortran_driver_statistics_and_diagnostics_files(tmp_path):
         for col, key in ((2, "cfl"), (3, "mke"), (4, "u_star"), (5, "divu")):
             # E16.9: 9 significant digits, relative rounding up to 5e-9
             assert r[col] == pytest.approx(diag[jt][key], rel=6e-9, abs=1e-18), (jt, key)
+
+
+def test_fortran_driver_dyn_smag_matches_oracle(tmp_path):
+    if not DRIVER.exists():
+        pytest.skip("flang driver not built")
+    P, IC, O = load_params(), load_ic(), load_oracle()
+    nx, ny, nz = 16, 16, 8
+    prm = P.LesParams(nx=nx, ny=ny, nz=nz, sgs_model=P.SGS_DYN_SMAG)
+    u, v, w = IC.log_law_ic(nx, ny, nz, prm.lz, prm.z0, 0.1, seed=29)
+    IC.write_vel_files(tmp_path, u, v, w)
+    print("fortran driver: dyn_Smag, 12 steps", flush=True)
+    _run(tmp_path, nx, ny, nz, 12, sgs="dyn_Smag")
+    # without the Lagrangian model the checkpoint holds u v w rhs(3) cs_opt2
+    ck = _read_ckpt(tmp_path / "checkpoint_000000012", nx, ny, nz, 7)
+    o = O.Oracle(prm)
+    o.set_ic(u, v, w)
+    o.step(12)  # Cs^2 updates at jt = 5 and 10
+    for i, n in enumerate(["u", "v", "w", "rhsx", "rhsy", "rhsz", "cs_opt2"]):
+        ref = o.get(n)[1:nz + 2]
+        lev = slice(0, nz) if n.startswith("rhs") else slice(0, nz + 1)
+        assert np.array_equal(ck[i][lev], ref[lev]), n
+    assert ck[6][1:nz + 1].max() > 0
