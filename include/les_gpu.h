@@ -42,6 +42,8 @@ typedef struct les_params {
   int coriolis;    /* coriolis_flag */
   int precision;   /* 0 FP64, 1 mixed (FP32 flow + FP64 pressure/averages/AB2), 2 mixed + learned SGS */
   int judge_every; /* learned SGS: steps between plain-LASD judge updates */
+  int dealias;     /* convection dealiasing, the reference's compile-time DSLG
+                      (src/Makefile:17-40): 0 PADD, 1 TRUNC, 2 FSMTHG, 3 NODLSG */
   double lx, ly, lz, dt, z0, pfx, pfy, cs0, coriol, ug, vg, ubc_u, ubc_v, ubc_w;
   double learned_zmin; /* learned SGS: levels with z / delta < learned_zmin (the
                           near-wall band) keep the plain procedure on every update */

@@ -46,7 +46,7 @@ class LesError(RuntimeError):
 class _Params(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int) for n in
                 ("nx", "ny", "nz", "sgs_model", "cs_count", "dyn_init", "lagran_init", "wall_model",
-                 "ubc_u_neu", "ubc_v_neu", "halo_mode", "coriolis", "precision", "judge_every")] + \
+                 "ubc_u_neu", "ubc_v_neu", "halo_mode", "coriolis", "precision", "judge_every", "dealias")] + \
                [(n, ctypes.c_double) for n in
                 ("lx", "ly", "lz", "dt", "z0", "pfx", "pfy", "cs0", "coriol", "ug", "vg",
                  "ubc_u", "ubc_v", "ubc_w", "learned_zmin")]

@@ -634,6 +634,7 @@ int les_init(const les_params* p) {
     return 1;
   }
   if (P.cs_count < 1) { les_set_error("cs_count must be >= 1"); return 1; }
+  if (P.dealias < 0 || P.dealias > 3) { les_set_error("dealias must be 0 PADD, 1 TRUNC, 2 FSMTHG or 3 NODLSG"); return 1; }
   if (P.wall_model != 0 && P.wall_model != 1) { les_set_error("Invalid wall law type"); return 1; }
   if (P.precision < 0 || P.precision > 2) { les_set_error("precision must be 0, 1 or 2"); return 1; }
   Les* L = new Les();
@@ -717,6 +718,48 @@ int les_init(const les_params* p) {
     for (long q = 0; q < L->ns; ++q) G[q] = kk2[q] >= kc2 ? 0.0 : 1.0 / (double)(nx * ny);
     for (int j = 0; j < ny; ++j) G[(lhx - 1) + (long)lhx * j] = 0.0;
     for (int i = 0; i < lhx; ++i) G[i + (long)lhx * (ny / 2)] = 0.0;
+  }
+  // Dealiasing multipliers of TRUNC and FSMTHG on the half spectrum.  The
+  // reference filters each real line along x, then along y, with 1D real
+  // transforms that zero or scale entry m of the line's half spectrum; the
+  // two stages together multiply the 2D spectrum by Fx(i) Fy(|ky|).
+  //  TRUNC sources: scr(tt:lh) = 0, tt = ceiling(n/3): modes m >= tt-1
+  //    (src/compute_convective_trunc.f90:279-304); products: tt = ceiling(2 lh/3)
+  //    (:322-348), lh = n/2 + 1;
+  //  FSMTHG: F(m) = exp(-36 (m/(lh-1))^36), F(lh-1) = 0, sources and products
+  //    (src/compute_convective_Fsmthg.f90:82-100).
+  if (P.dealias == 1 || P.dealias == 2) {
+    const int lhy = ny / 2 + 1;
+    std::vector<double> gab[2] = {std::vector<double>(L->ns), std::vector<double>(L->ns)};
+    for (int t = 0; t < 2; ++t) {
+      std::vector<double> fx(lhx), fy(lhy);
+      if (P.dealias == 1) {
+        const int ttx = t == 0 ? (nx + 2) / 3 : (2 * lhx + 2) / 3;
+        const int tty = t == 0 ? (ny + 2) / 3 : (2 * lhy + 2) / 3;
+        for (int m = 0; m < lhx; ++m) fx[m] = m <= ttx - 2 ? 1.0 : 0.0;
+        for (int m = 0; m < lhy; ++m) fy[m] = m <= tty - 2 ? 1.0 : 0.0;
+      } else {
+        for (int m = 0; m < lhx; ++m) fx[m] = std::exp(-36.0 * std::pow((double)m / (double)(lhx - 1), 36.0));
+        for (int m = 0; m < lhy; ++m) fy[m] = std::exp(-36.0 * std::pow((double)m / (double)(lhy - 1), 36.0));
+        fx[lhx - 1] = 0.0;
+        fy[lhy - 1] = 0.0;
+      }
+      for (int j = 0; j < ny; ++j) {
+        const int my = j <= ny / 2 ? j : ny - j;
+        for (int i = 0; i < lhx; ++i) gab[t][i + (long)lhx * j] = fx[i] * fy[my];
+      }
+    }
+    std::vector<float> g32(L->ns);
+    double** d64g[2] = {&L->ga_64, &L->gb_64};
+    float** d32g[2] = {&L->ga_32, &L->gb_32};
+    for (int t = 0; t < 2; ++t) {
+      *d64g[t] = (double*)dalloc(*L, sizeof(double) * L->ns);
+      *d32g[t] = (float*)dalloc(*L, sizeof(float) * L->ns);
+      if (!*d64g[t] || !*d32g[t]) { les_finalize(); les_set_error("out of memory"); return 3; }
+      hipMemcpy(*d64g[t], gab[t].data(), sizeof(double) * L->ns, hipMemcpyHostToDevice);
+      for (long q = 0; q < L->ns; ++q) g32[q] = (float)gab[t][q];
+      hipMemcpy(*d32g[t], g32.data(), sizeof(float) * L->ns, hipMemcpyHostToDevice);
+    }
   }
   std::vector<float> f32(L->ns);
   double** d64[4] = {&L->kkx64, &L->kky64, &L->g2_64, &L->g4_64};

@@ -5,17 +5,19 @@
 
 // Products on the padded grid (src/compute_convective_padd.f90:179-249);
 // velocities on the padded grid are the negated ones, as in the reference.
-template <typename T>
+// The nx x ny variants (TRUNC, FSMTHG, NODLSG) form the same products with
+// nbp = nx ny and c2 = 1/(nx ny) (src/compute_convective_trunc.f90:177-244).
+template <typename T, typename TO>
 struct ProdArgs {
   const T *ub, *vb, *wb, *o1, *o2, *o3;
-  T *px, *py, *pz;
+  TO *px, *py, *pz;
   long nbp;
   int nz;
   T c2;
 };
 
-template <typename T>
-__global__ void k_conv_prod(ProdArgs<T> a) {
+template <typename T, typename TO>
+__global__ void k_conv_prod(ProdArgs<T, TO> a) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= a.nbp * a.nz) return;
   const int k = 1 + (int)(t / a.nbp);
@@ -32,9 +34,29 @@ __global__ void k_conv_prod(ProdArgs<T> a) {
     y = c2 * (a.ub[c] * a.o3[c] + h * (a.wb[c] * (-a.o1[c]) + a.wb[up] * (-a.o1[up])));
     z = c2 * h * ((a.ub[c] + a.ub[dn]) * (-a.o2[c]) + (a.vb[c] + a.vb[dn]) * a.o1[c]);
   }
-  a.px[c] = x;
-  a.py[c] = y;
-  a.pz[c] = z;
+  a.px[c] = (TO)x;
+  a.py[c] = (TO)y;
+  a.pz[c] = (TO)z;
+}
+
+// The six convection sources of conv_src on levels 1..nz+1 of the nx x ny
+// grid (field = blockIdx.y).
+template <typename T>
+struct ConvInArgs {
+  ConvSrc<T> cs;
+  T* dst[6];
+  T c;
+  long np;
+  int nlev;
+};
+
+template <typename T>
+__global__ void k_conv_sources(ConvInArgs<T> a) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.np * a.nlev) return;
+  const int k = 1 + (int)(t / a.np);
+  const long q = t % a.np;
+  a.dst[blockIdx.y][(long)k * a.np + q] = conv_src<T, T>(a.cs, blockIdx.y, k, q, a.np, a.c);
 }
 
 struct SetPlaneArgs {
@@ -88,8 +110,95 @@ static void launch_cols_resample(Les& L, ColResArgs<T>& a, int nf, const FftPlan
                      p1, p2, twpool<T>(L));
 }
 
+// Real multiplier g on the 2D spectrum of planes 1..nlev of nf real fields:
+// forward transforms, g, inverse transforms (unnormalised, as the test
+// filter), three fields at a time; dst may alias src.
+template <typename T, typename TO>
+static void spectral_multiply(Les& L, T* const* src, TO* const* dst, int nf, int nlev, const T* g) {
+  for (int f0 = 0; f0 < nf; f0 += 3) {
+    const int m = nf - f0 < 3 ? nf - f0 : 3;
+    const T* s[3];
+    Cplx<T>* sp[3];
+    const Cplx<T>* cs[3];
+    TO* o[3];
+    for (int t = 0; t < m; ++t) {
+      s[t] = src[f0 + t];
+      sp[t] = (Cplx<T>*)L.wk_spec[t];
+      cs[t] = (const Cplx<T>*)L.wk_spec[3 + t];
+      o[t] = dst[f0 + t];
+    }
+    rows_r2c<T, T>(L, s, sp, m, 1, nlev, (T)1);
+    for (int t = 0; t < m; ++t)
+      cols<T>(L, COP_FILT1, sp[t], nullptr, (Cplx<T>*)L.wk_spec[3 + t], nullptr, nullptr, 1, 1, 1, 1, nlev, g);
+    rows_c2r<T, TO>(L, cs, o, m, 1, nlev);
+  }
+}
+
+// compute_dealias_convec of the other dealiasing options, chosen at compile
+// time in the reference (DSLG, src/Makefile:17-40): 1 TRUNC
+// (src/compute_convective_trunc.f90:115-247), 2 FSMTHG
+// (src/compute_convective_Fsmthg.f90:141-264), 3 NODLSG
+// (src/compute_convective_nodlsg.f90:88-205).  Sources and products stay on
+// the nx x ny grid with const = 1/(nx ny) in both places.  TRUNC and FSMTHG
+// multiply the spectra of the sources (levels 1..nz+1; level 0 enters no
+// product) by ga and of the products (levels 1..nz) by gb; NODLSG forms the
+// products of the scaled sources directly, so its result carries the factor
+// const^3 of the reference's code.  The sources use the padded path's
+// buffers (wk_big, plane stride nx ny), the products wk_prod.
+template <typename T>
+static void convection_nxny(Les& L, double* cx, double* cy, double* cz) {
+  const int nz = L.nz;
+  const long np = L.np;
+  const T c = (T)(1.0 / (double)(L.nx * L.ny));
+  const bool filt = L.P.dealias != 3;
+  T* srcf[6];
+  for (int f = 0; f < 6; ++f) srcf[f] = (T*)L.wk_big[f];
+  {
+    ConvInArgs<T> a;
+    a.cs.u = D(L, LES_U); a.cs.v = D(L, LES_V); a.cs.w = D(L, LES_W);
+    a.cs.dudy = F<T>(L, LES_DUDY); a.cs.dudz = F<T>(L, LES_DUDZ);
+    a.cs.dvdx = F<T>(L, LES_DVDX); a.cs.dvdz = F<T>(L, LES_DVDZ);
+    a.cs.dwdx = F<T>(L, LES_DWDX); a.cs.dwdy = F<T>(L, LES_DWDY);
+    for (int f = 0; f < 6; ++f) a.dst[f] = srcf[f];
+    a.c = c; a.np = np; a.nlev = nz + 1;
+    kt_begin(L, KT_POINTWISE);
+    hipLaunchKernelGGL((k_conv_sources<T>), dim3(nblk(np * (nz + 1)), 6), dim3(LES_BLOCK), 0, L.stream, a);
+    kt_end(L, KT_POINTWISE, (double)np * (nz + 1) * (3.0 * 8 + 8.0 * sizeof(T) + 6.0 * sizeof(T)));
+  }
+  const T* ga = sizeof(T) == 8 ? (const T*)L.ga_64 : (const T*)L.ga_32;
+  const T* gb = sizeof(T) == 8 ? (const T*)L.gb_64 : (const T*)L.gb_32;
+  if (filt) spectral_multiply<T, T>(L, srcf, srcf, 6, nz + 1, ga);
+  T* pr[3] = {(T*)L.wk_prod[0], (T*)L.wk_prod[1], (T*)L.wk_prod[2]};
+  if (filt) {
+    ProdArgs<T, T> a;
+    a.ub = srcf[0]; a.vb = srcf[1]; a.wb = srcf[2]; a.o1 = srcf[3]; a.o2 = srcf[4]; a.o3 = srcf[5];
+    a.px = pr[0]; a.py = pr[1]; a.pz = pr[2];
+    a.nbp = np; a.nz = nz; a.c2 = c;
+    kt_begin(L, KT_POINTWISE);
+    hipLaunchKernelGGL((k_conv_prod<T, T>), dim3(nblk(np * nz)), dim3(LES_BLOCK), 0, L.stream, a);
+    kt_end(L, KT_POINTWISE, 9.0 * np * nz * sizeof(T));
+    double* out[3] = {cx, cy, cz};
+    spectral_multiply<T, double>(L, pr, out, 3, nz, gb);
+  } else {
+    ProdArgs<T, double> a;
+    a.ub = srcf[0]; a.vb = srcf[1]; a.wb = srcf[2]; a.o1 = srcf[3]; a.o2 = srcf[4]; a.o3 = srcf[5];
+    a.px = cx; a.py = cy; a.pz = cz;
+    a.nbp = np; a.nz = nz; a.c2 = c;
+    kt_begin(L, KT_POINTWISE);
+    hipLaunchKernelGGL((k_conv_prod<T, double>), dim3(nblk(np * nz)), dim3(LES_BLOCK), 0, L.stream, a);
+    kt_end(L, KT_POINTWISE, np * nz * (6.0 * sizeof(T) + 3.0 * 8));
+  }
+  SetPlaneArgs sp;
+  sp.a = cx; sp.b = cy; sp.c = cz; sp.np = np; sp.k0 = 0; sp.k1 = nz + 1; sp.val = BOGUS;
+  hipLaunchKernelGGL(k_set_planes, dim3(nblk(np)), dim3(LES_BLOCK), 0, L.stream, sp);
+}
+
 template <typename T>
 static void convection(Les& L, double* cx, double* cy, double* cz) {
+  if (L.P.dealias != 0) {
+    convection_nxny<T>(L, cx, cy, cz);
+    return;
+  }
   const int nx = L.nx, ny = L.ny, nz = L.nz, nx2 = L.nx2, ny2 = L.ny2;
   const T c = (T)(1.0 / (double)(nx * ny));
   // pad in x: planes 1..nz+1 of u, v, w and the vorticities
@@ -124,14 +233,14 @@ static void convection(Les& L, double* cx, double* cy, double* cz) {
   }
   // products
   {
-    ProdArgs<T> a;
+    ProdArgs<T, T> a;
     a.ub = (const T*)L.wk_big[0]; a.vb = (const T*)L.wk_big[1]; a.wb = (const T*)L.wk_big[2];
     a.o1 = (const T*)L.wk_big[3]; a.o2 = (const T*)L.wk_big[4]; a.o3 = (const T*)L.wk_big[5];
     a.px = (T*)L.wk_prod[0]; a.py = (T*)L.wk_prod[1]; a.pz = (T*)L.wk_prod[2];
     a.nbp = L.nbp; a.nz = nz;
     a.c2 = (T)(1.0 / (double)(nx2 * ny2));
     kt_begin(L, KT_POINTWISE);
-    hipLaunchKernelGGL((k_conv_prod<T>), dim3(nblk(L.nbp * nz)), dim3(LES_BLOCK), 0, L.stream, a);
+    hipLaunchKernelGGL((k_conv_prod<T, T>), dim3(nblk(L.nbp * nz)), dim3(LES_BLOCK), 0, L.stream, a);
     kt_end(L, KT_POINTWISE, 9.0 * L.nbp * nz * sizeof(T));
   }
   // truncate in y

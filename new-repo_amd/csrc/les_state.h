@@ -34,6 +34,9 @@ struct Les {
   // tables (double and float copies)
   double *kkx64, *kky64, *g2_64, *g4_64;
   float *kkx32, *kky32, *g2_32, *g4_32;
+  // TRUNC / FSMTHG multipliers of the convection sources (a) and products (b)
+  double *ga_64, *gb_64;
+  float *ga_32, *gb_32;
   double* logterm;  // log(0.5 dz / z0) per surface point
   double* lvec;     // mixing length per level (nz+2)
 

@@ -4,8 +4,9 @@
 !! Reads the reference's parameters.nlst (same six namelist groups and
 !! variable names, src/parameters_IO.f90:151-166) and nondimensionalises it
 !! the same way (src/parameters_IO.f90:240-298).  An optional seventh group
-!! &les_gpu_param (precision, halo_mode, judge_every, learned_zmin) selects the device
-!! variant.  read_parameters fills les_params_c for les_init.
+!! &les_gpu_param (precision, halo_mode, judge_every, learned_zmin, dealias) selects
+!! the device variant; dealias = 'PADD' | 'TRUNC' | 'FSMTHG' | 'NODLSG' replaces the
+!! reference's compile-time DSLG choice (src/Makefile:17-40).  read_parameters fills les_params_c for les_init.
 !------------------------------------------------------------------------------
 module parameters_IO
   use iso_c_binding
@@ -46,6 +47,7 @@ module parameters_IO
   ! les_gpu_param (device variant)
   integer, public :: precision = 0, halo_mode = 0, judge_every = 50
   real(rprec), public :: learned_zmin = 4._rprec
+  character(8), public :: dealias = 'PADD'
   ! derived
   real(rprec), public :: dx, dy, dz
   logical, public :: Lagran_init = .false.
@@ -66,7 +68,7 @@ contains
     namelist / sgs_param / Cs0, sgs_model, cs_count, dyn_init
     namelist / momentum_bc / lbc_model, lbc_special, ubc_model, ubc_special, wall_model, surface_files
     namelist / momentum_ic / ic_type, u0, v0, w0, u_star, dudz0, z_turb, vnf
-    namelist / les_gpu_param / precision, halo_mode, judge_every, learned_zmin
+    namelist / les_gpu_param / precision, halo_mode, judge_every, learned_zmin, dealias
 
     open(88, file='parameters.nlst', status='OLD', action='READ', delim='APOSTROPHE', iostat=ios)
     if (ios /= 0) call abort22('Could not open parameters.nlst for reading')
@@ -113,6 +115,15 @@ contains
     end if
     cparams%ubc_u_neu = 1; cparams%ubc_v_neu = 1
     cparams%halo_mode = halo_mode; cparams%precision = precision; cparams%judge_every = judge_every
+    select case (trim(dealias))
+    case ('PADD');   cparams%dealias = 0
+    case ('TRUNC');  cparams%dealias = 1
+    case ('FSMTHG'); cparams%dealias = 2
+    case ('NODLSG'); cparams%dealias = 3
+    case default
+       write(*,*) 'ABORT - Invalid dealiasing option ', trim(dealias)
+       error stop 1
+    end select
     cparams%learned_zmin = learned_zmin
     cparams%coriolis = 0; cparams%coriol = 0._rprec; cparams%ug = 0._rprec; cparams%vg = 0._rprec
     if (coriolis_flag) then

@@ -21,6 +21,8 @@ PI = 3.1415926535897932384626433
 SGS_CST_SMAG = 0
 SGS_DYN_SMAG = 1
 SGS_LASD = 2
+# convection dealiasing, the reference's compile-time DSLG choice (src/Makefile:17-40)
+DEALIAS = {"PADD": 0, "TRUNC": 1, "FSMTHG": 2, "NODLSG": 3}
 PREC_FP64 = 0
 PREC_MIXED = 1
 PREC_LEARNED = 2
@@ -52,6 +54,7 @@ class LesParams:
     ubc_w: float = 0.0
     halo_mode: int = 0          # 0 one-rank reference halo, 1 periodic
     coriolis: int = 0
+    dealias: int = 0            # 0 PADD (3/2 rule), 1 TRUNC (2/3 rule), 2 FSMTHG, 3 NODLSG
     coriol: float = 0.0
     ug: float = 0.0
     vg: float = 0.0
@@ -62,7 +65,7 @@ class LesParams:
     # ---- vectors shared with the CPU oracle (orc_init argument order)
     def int_vector(self) -> List[int]:
         return [self.nx, self.ny, self.nz, self.sgs_model, self.cs_count, self.dyn_init, self.lagran_init,
-                self.wall_model, self.ubc_u_neu, self.ubc_v_neu, self.halo_mode, self.coriolis]
+                self.wall_model, self.ubc_u_neu, self.ubc_v_neu, self.halo_mode, self.coriolis, self.dealias]
 
     def real_vector(self) -> List[float]:
         return [self.lx, self.ly, self.lz, self.dt, self.z0, self.pfx, self.pfy, self.cs0, self.coriol,
@@ -91,6 +94,8 @@ class LesParams:
             raise ValueError("the learned SGS surrogate replaces dyn_LagScaleDep only")
         if self.cs_count < 1:
             raise ValueError("cs_count must be >= 1")
+        if self.dealias not in DEALIAS.values():
+            raise ValueError("dealias must be 0 PADD, 1 TRUNC, 2 FSMTHG or 3 NODLSG")
 
 
 # ---------------------------------------------------------------- namelist
@@ -169,6 +174,10 @@ def params_from_namelist(text: str, precision: int = PREC_FP64) -> LesParams:
         raise ValueError(f"Invalid wall law type {wm!r}")
     p.wall_model = 0 if wm == "wall_filter" else 1
     p.precision = precision
+    dslg = str(nl.get("les_gpu_param", {}).get("dealias", "PADD")).upper()
+    if dslg not in DEALIAS:
+        raise ValueError(f"Invalid dealiasing option {dslg!r}")
+    p.dealias = DEALIAS[dslg]
     p.validate()
     return p
 

@@ -60,6 +60,7 @@ struct Params {
   int ubc_u_neu, ubc_v_neu;  // top BC type for u, v (1 Neumann, 0 Dirichlet)
   int halo_mode;    // 0 one-rank reference halo, 1 periodic
   int coriolis;
+  int dealias;      // convection: 0 PADD (3/2 rule), 1 TRUNC, 2 FSMTHG, 3 NODLSG
   double lx, ly, lz, dt, z0, pfx, pfy, cs0, coriol, ug, vg;
   double ubc_u, ubc_v, ubc_w;
 };
@@ -72,6 +73,7 @@ struct Oracle {
   double dx, dy, dz, delta;
   Plan px, py, px2, py2;
   std::vector<double> kkx, kky, kk2, G2, G4;
+  std::vector<double> GA, GB;   // TRUNC / FSMTHG multipliers of the sources and of the products
   std::vector<double> logterm;  // log(0.5 dz / z0) per surface point
   // fields (nx, ny, 0:nz+1), x fastest
   std::vector<double> u, v, w, dudx, dudy, dudz, dvdx, dvdy, dvdz, dwdx, dwdy, dwdz;
@@ -181,6 +183,38 @@ static void build_tables(Oracle& o) {
       if (o.kk2[q] >= kc2) G[q] = 0.0;
     for (int j = 0; j < ny; ++j) G[(lhx - 1) + (size_t)lhx * j] = 0.0;
     for (int i = 0; i < lhx; ++i) G[i + (size_t)lhx * (ny / 2)] = 0.0;
+  }
+  // Dealiasing multipliers of TRUNC and FSMTHG on the half spectrum.  The
+  // reference filters each real line along x, then along y, with 1D real
+  // transforms, zeroing or scaling entry m of the half spectrum of the line;
+  // the two stages together multiply the 2D spectrum by Fx(i) Fy(|ky|).
+  //  TRUNC sources: scr(tt:lh) = 0 with tt = ceiling(n/3), i.e. modes m >= tt-1
+  //    (src/compute_convective_trunc.f90:279-304);
+  //  TRUNC products: tt = ceiling(2 lh / 3) (:322-348), lh = n/2 + 1;
+  //  FSMTHG sources and products: F(m) = exp(-36 (m / (lh-1))^36), F(lh-1) = 0
+  //    (src/compute_convective_Fsmthg.f90:82-100, 281-299, 339-359).
+  if (o.P.dealias == 1 || o.P.dealias == 2) {
+    int lhy = ny / 2 + 1;
+    for (int t = 0; t < 2; ++t) {
+      std::vector<double>& G = t == 0 ? o.GA : o.GB;
+      G.assign((size_t)lhx * ny, 0.0);
+      std::vector<double> fx(lhx), fy(lhy);
+      if (o.P.dealias == 1) {
+        int ttx = t == 0 ? (nx + 2) / 3 : (2 * lhx + 2) / 3;
+        int tty = t == 0 ? (ny + 2) / 3 : (2 * lhy + 2) / 3;
+        for (int m = 0; m < lhx; ++m) fx[m] = m <= ttx - 2 ? 1.0 : 0.0;
+        for (int m = 0; m < lhy; ++m) fy[m] = m <= tty - 2 ? 1.0 : 0.0;
+      } else {
+        for (int m = 0; m < lhx; ++m) fx[m] = std::exp(-36.0 * std::pow((double)m / (double)(lhx - 1), 36.0));
+        for (int m = 0; m < lhy; ++m) fy[m] = std::exp(-36.0 * std::pow((double)m / (double)(lhy - 1), 36.0));
+        fx[lhx - 1] = 0.0;
+        fy[lhy - 1] = 0.0;
+      }
+      for (int j = 0; j < ny; ++j) {
+        int my = j <= ny / 2 ? j : ny - j;
+        for (int i = 0; i < lhx; ++i) G[i + (size_t)lhx * j] = fx[i] * fy[my];
+      }
+    }
   }
 }
 
@@ -713,7 +747,13 @@ static void trunc_plane(const Oracle& o, const double* in, double* out) {
   }
 }
 
+static void convection_nxny(Oracle& o, std::vector<double>& cx, std::vector<double>& cy, std::vector<double>& cz);
+
 static void convection(Oracle& o, std::vector<double>& cx, std::vector<double>& cy, std::vector<double>& cz) {
+  if (o.P.dealias != 0) {
+    convection_nxny(o, cx, cy, cz);
+    return;
+  }
   int nx = o.nx, ny = o.ny, nz = o.nz, nx2 = 3 * nx / 2, ny2 = 3 * ny / 2;
   size_t nb = (size_t)nx2 * ny2;
   double c = 1.0 / (double)(nx * ny);
@@ -773,6 +813,78 @@ static void convection(Oracle& o, std::vector<double>& cx, std::vector<double>& 
         std::vector<double>& dst = t == 0 ? cx : (t == 1 ? cy : cz);
         std::memcpy(&dst[o.I(0, 0, k)], res.data(), sizeof(double) * o.np);
       }
+    }
+  }
+  for (size_t q = 0; q < o.np; ++q) {
+    cx[q] = BOGUS; cy[q] = BOGUS; cz[q] = BOGUS;
+    cx[o.I(0, 0, nz + 1) + q] = BOGUS; cy[o.I(0, 0, nz + 1) + q] = BOGUS; cz[o.I(0, 0, nz + 1) + q] = BOGUS;
+  }
+}
+
+// compute_dealias_convec of the other dealiasing options, chosen at compile
+// time in the reference (DSLG, src/Makefile:17-40): 1 TRUNC
+// (src/compute_convective_trunc.f90:115-247), 2 FSMTHG
+// (src/compute_convective_Fsmthg.f90:141-264), 3 NODLSG
+// (src/compute_convective_nodlsg.f90:88-205).  The six sources (the same as
+// the padded version's, levels 1..nz+1; level 0 enters no product) and the
+// products stay on the nx x ny grid with const = 1/(nx ny) in both places;
+// TRUNC and FSMTHG filter the sources (levels 1..nz+1) with GA and the
+// products (levels 1..nz) with GB through the unnormalised 2D transforms.
+static void convection_nxny(Oracle& o, std::vector<double>& cx, std::vector<double>& cy, std::vector<double>& cz) {
+  int nz = o.nz;
+  double c = 1.0 / (double)(o.nx * o.ny);
+  bool filt = o.P.dealias != 3;
+  std::vector<std::vector<double>> src(6, std::vector<double>(o.nf, 0.0));
+#pragma omp parallel for schedule(static)
+  for (int k = 1; k <= nz + 1; ++k)
+    for (size_t q = 0; q < o.np; ++q) {
+      size_t a = o.I(0, 0, k) + q, a2 = o.I(0, 0, 2) + q;
+      src[0][a] = -(c * o.u[a]);
+      src[1][a] = -(c * o.v[a]);
+      src[2][a] = -(c * o.w[a]);
+      if (k == 1) {
+        src[3][a] = c * (0.5 * (o.dwdy[a] + o.dwdy[a2]) - o.dvdz[a]);
+        src[4][a] = c * (o.dudz[a] - 0.5 * (o.dwdx[a] + o.dwdx[a2]));
+      } else {
+        src[3][a] = c * (o.dwdy[a] - o.dvdz[a]);
+        src[4][a] = c * (o.dudz[a] - o.dwdx[a]);
+      }
+      src[5][a] = c * (o.dvdx[a] - o.dudy[a]);
+    }
+  if (filt) {
+    std::vector<Cx> spec;
+    for (int f = 0; f < 6; ++f) {
+      fwd_spec(o, src[f], spec);
+      filter_from_spec(o, spec, o.GA, src[f]);
+    }
+  }
+  std::vector<double>* dst[3] = {&cx, &cy, &cz};
+  std::vector<std::vector<double>> prod(3, std::vector<double>(o.nf, 0.0));
+#pragma omp parallel for schedule(static)
+  for (int k = 1; k <= nz; ++k)
+    for (size_t q = 0; q < o.np; ++q) {
+      size_t a = o.I(0, 0, k) + q, up = o.I(0, 0, k + 1) + q, dn = o.I(0, 0, k - 1) + q;
+      const std::vector<double>&ub = src[0], &vb = src[1], &wb = src[2], &o1 = src[3], &o2 = src[4], &o3 = src[5];
+      double x, y, z;
+      if (k == 1) {
+        x = c * (-(vb[a] * o3[a]) + 0.5 * wb[up] * o2[up]);
+        y = c * (ub[a] * o3[a] + 0.5 * wb[up] * (-o1[up]));
+        z = 0.0;
+      } else {
+        x = c * (-(vb[a] * o3[a]) + 0.5 * (wb[up] * o2[up] + wb[a] * o2[a]));
+        y = c * (ub[a] * o3[a] + 0.5 * (wb[a] * (-o1[a]) + wb[up] * (-o1[up])));
+        z = c * 0.5 * ((ub[a] + ub[dn]) * (-o2[a]) + (vb[a] + vb[dn]) * o1[a]);
+      }
+      prod[0][a] = x; prod[1][a] = y; prod[2][a] = z;
+    }
+  for (int t = 0; t < 3; ++t) {
+    if (filt) {
+      std::vector<Cx> spec;
+      fwd_spec(o, prod[t], spec);
+      filter_from_spec(o, spec, o.GB, *dst[t]);
+    } else {
+      for (int k = 1; k <= nz; ++k)
+        std::memcpy(&(*dst[t])[o.I(0, 0, k)], &prod[t][o.I(0, 0, k)], sizeof(double) * o.np);
     }
   }
   for (size_t q = 0; q < o.np; ++q) {
@@ -1153,7 +1265,7 @@ static std::vector<double>* field(Oracle& o, int id) {
   return nullptr;
 }
 
-// ip: nx ny nz sgs_model cs_count dyn_init lagran_init wall_model ubc_u_neu ubc_v_neu halo_mode coriolis
+// ip: nx ny nz sgs_model cs_count dyn_init lagran_init wall_model ubc_u_neu ubc_v_neu halo_mode coriolis dealias
 // rp: lx ly lz dt z0 pfx pfy cs0 coriol ug vg ubc_u ubc_v ubc_w
 int orc_init(const int* ip, const double* rp) {
   delete g;
@@ -1162,7 +1274,8 @@ int orc_init(const int* ip, const double* rp) {
   Params& P = o.P;
   P.nx = ip[0]; P.ny = ip[1]; P.nz = ip[2]; P.sgs_model = ip[3]; P.cs_count = ip[4]; P.dyn_init = ip[5];
   P.lagran_init = ip[6]; P.wall_model = ip[7]; P.ubc_u_neu = ip[8]; P.ubc_v_neu = ip[9]; P.halo_mode = ip[10];
-  P.coriolis = ip[11];
+  P.coriolis = ip[11]; P.dealias = ip[12];
+  if (P.dealias < 0 || P.dealias > 3) return 1;
   P.lx = rp[0]; P.ly = rp[1]; P.lz = rp[2]; P.dt = rp[3]; P.z0 = rp[4]; P.pfx = rp[5]; P.pfy = rp[6];
   P.cs0 = rp[7]; P.coriol = rp[8]; P.ug = rp[9]; P.vg = rp[10]; P.ubc_u = rp[11]; P.ubc_v = rp[12]; P.ubc_w = rp[13];
   o.nx = P.nx; o.ny = P.ny; o.nz = P.nz; o.nzg = P.nz + 2; o.lhx = P.nx / 2 + 1;
@@ -1307,10 +1420,13 @@ int orc_tridiag(int i, int j, const double* r, double* x) {
   return 0;
 }
 
-// spectral tables of the current grid: which 0 kkx, 1 kky, 2 kk2, 3 G(2 delta), 4 G(4 delta)
+// spectral tables of the current grid: which 0 kkx, 1 kky, 2 kk2, 3 G(2 delta),
+// 4 G(4 delta), 5 dealiasing multiplier of the sources, 6 of the products
 int orc_table(int which, double* out) {
   if (!g) return 1;
-  std::vector<double>* t = which == 0 ? &g->kkx : which == 1 ? &g->kky : which == 2 ? &g->kk2 : which == 3 ? &g->G2 : &g->G4;
+  std::vector<double>* t = which == 0 ? &g->kkx : which == 1 ? &g->kky : which == 2 ? &g->kk2 : which == 3 ? &g->G2
+                         : which == 4 ? &g->G4 : which == 5 ? &g->GA : &g->GB;
+  if (t->empty()) return 2;
   std::memcpy(out, t->data(), sizeof(double) * t->size());
   return 0;
 }

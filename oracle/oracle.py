@@ -149,9 +149,10 @@ class Oracle:
         lib().orc_set_2d(idx, _dp(a))
 
     def table(self, which):
-        idx = {"kkx": 0, "kky": 1, "kk2": 2, "G2": 3, "G4": 4}[which]
+        idx = {"kkx": 0, "kky": 1, "kk2": 2, "G2": 3, "G4": 4, "GA": 5, "GB": 6}[which]
         out = np.zeros((self.prm.ny, self.prm.nx // 2 + 1))
-        lib().orc_table(idx, _dp(out))
+        if lib().orc_table(idx, _dp(out)):
+            raise KeyError(which)
         return out
 
     def tridiag(self, i, j, r):

@@ -46,8 +46,8 @@ def test_params_struct_layout_matches_c():
 #include <stddef.h>
 #include "{HEADER}"
 int main(void) {{
-  printf("%zu %zu %zu %zu\\n", sizeof(les_params), offsetof(les_params, judge_every),
-         offsetof(les_params, lx), offsetof(les_params, ubc_w));
+  printf("%zu %zu %zu %zu %zu\\n", sizeof(les_params), offsetof(les_params, judge_every),
+         offsetof(les_params, dealias), offsetof(les_params, lx), offsetof(les_params, ubc_w));
   return 0;
 }}
 """
@@ -58,7 +58,7 @@ int main(void) {{
         subprocess.run(["gcc", str(c), "-o", str(exe)], check=True)
         vals = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split()]
     P = pkg._Params
-    assert vals == [ctypes.sizeof(P), P.judge_every.offset, P.lx.offset, P.ubc_w.offset]
+    assert vals == [ctypes.sizeof(P), P.judge_every.offset, P.dealias.offset, P.lx.offset, P.ubc_w.offset]
 
 
 def test_field_ids_match_header_enum():

@@ -25,12 +25,12 @@ NLST = """&simulation_param  p_on_y=1 p_on_z=1 nx={nx} ny={ny} nz={nz} nt={nt} d
 &sgs_param         Cs0=0.16d0 sgs_model='{sgs}' cs_count=5 dyn_init=1 /
 &momentum_bc       lbc_model='std' lbc_special='wall_law' ubc_model='std' ubc_special='std' wall_model='wall_filter' surface_files=.false. /
 &momentum_ic       ic_type='files' u0=0.0d0 v0=0.0d0 w0=0.0d0 u_star=1.0d0 dudz0=0.0d0 z_turb=0.5d0 vnf=1.0d0 /
-"""
+{extra}"""
 
 
-def _run(d, nx, ny, nz, nt, restart=False, ckpt=100000, stats=None, step_out=None, sgs="dyn_LagScaleDep"):
+def _run(d, nx, ny, nz, nt, restart=False, ckpt=100000, stats=None, step_out=None, sgs="dyn_LagScaleDep", extra=""):
     stat, start, period = (".true.", *stats) if stats else (".false.", 0, 100)
-    (d / "parameters.nlst").write_text(NLST.format(nx=nx, ny=ny, nz=nz, nt=nt, ckpt=ckpt, sgs=sgs,
+    (d / "parameters.nlst").write_text(NLST.format(nx=nx, ny=ny, nz=nz, nt=nt, ckpt=ckpt, sgs=sgs, extra=extra,
                                                    restart=".true." if restart else ".false.",
                                                    stat=stat, start=start, period=period,
                                                    output=".true." if step_out else ".false.",
@@ -124,16 +124,16 @@ def test_fortran_driver_statistics_and_diagnostics_files(tmp_path):
             assert r[col] == pytest.approx(diag[jt][key], rel=6e-9, abs=1e-18), (jt, key)
 
 
-def test_fortran_driver_dyn_smag_matches_oracle(tmp_path):
+def test_fortran_driver_dyn_smag_fsmthg_matches_oracle(tmp_path):
     if not DRIVER.exists():
         pytest.skip("flang driver not built")
     P, IC, O = load_params(), load_ic(), load_oracle()
     nx, ny, nz = 16, 16, 8
-    prm = P.LesParams(nx=nx, ny=ny, nz=nz, sgs_model=P.SGS_DYN_SMAG)
+    prm = P.LesParams(nx=nx, ny=ny, nz=nz, sgs_model=P.SGS_DYN_SMAG, dealias=P.DEALIAS["FSMTHG"])
     u, v, w = IC.log_law_ic(nx, ny, nz, prm.lz, prm.z0, 0.1, seed=29)
     IC.write_vel_files(tmp_path, u, v, w)
-    print("fortran driver: dyn_Smag, 12 steps", flush=True)
-    _run(tmp_path, nx, ny, nz, 12, sgs="dyn_Smag")
+    print("fortran driver: dyn_Smag with FSMTHG dealiasing, 12 steps", flush=True)
+    _run(tmp_path, nx, ny, nz, 12, sgs="dyn_Smag", extra="&les_gpu_param dealias='FSMTHG' /\n")
     # without the Lagrangian model the checkpoint holds u v w rhs(3) cs_opt2
     ck = _read_ckpt(tmp_path / "checkpoint_000000012", nx, ny, nz, 7)
     o = O.Oracle(prm)
