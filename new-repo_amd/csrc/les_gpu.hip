@@ -424,46 +424,61 @@ static void sgs_stress(Les& L, int jt, int jtt) {
 // formed directly in FP64 by removing the projection on the kx = nx/2 and
 // ky = ny/2 modes: f -= (-1)^i A_j + (-1)^j B_i - (-1)^(i+j) C, with A, B, C
 // the normalised alternating sums along x, y and both.
-// Pass 1: one block per band of 4 rows of a plane (one row per wavefront):
-// A_j = sum_i (-1)^i f_ij / nx by wavefront reduction, and the band's
-// alternating column partial sums P_i = sum_{j in band} (-1)^j f_ij.  Every
-// sum has a fixed order, so results are reproducible.  blockIdx.y selects
-// the field (u, v, w in one launch).
+// Pass 1: one block per band of 4 RPW rows of a plane, RPW consecutive rows
+// per wavefront (RPW = 4 when ny is a multiple of 16, else 1): A_j =
+// sum_i (-1)^i f_ij / nx by wavefront reduction, and the band's alternating
+// column partial sums P_i = sum_{j in band} (-1)^j f_ij (each wave's rows in
+// order, then the four waves).  Every sum has a fixed order, so results are
+// reproducible.  blockIdx.y selects the field (u, v, w in one launch).
 struct NyqArgs {
   double* f[3];
   double *A, *B, *C, *P;  // per field: A ny*nzg, B nx*nzg, C nzg, P (ny/4)*nx*nzg
   int nx, ny, nzg;
 };
 
+template <int RPW>
 __global__ void __launch_bounds__(LES_BLOCK) k_nyq_reduce(NyqArgs a) {
   __shared__ double part[LES_BLOCK / LES_WAVE][LES_MAX_N];
   const int nx = a.nx, ny = a.ny, fld = blockIdx.y;
   const int lane = threadIdx.x & (LES_WAVE - 1), wave = threadIdx.x / LES_WAVE;
-  const long row = (long)blockIdx.x * (LES_BLOCK / LES_WAVE) + wave;  // global row (k * ny + j)
-  const int j = (int)(row % ny);
-  const double* __restrict__ fr = a.f[fld] + row * nx;
-  const double sj = (j & 1) ? -1.0 : 1.0;
-  double s = 0.0;
-  for (int i = lane; i < nx; i += LES_WAVE) {
-    const double v = fr[i];
-    s += (i & 1) ? -v : v;
-    part[wave][i] = sj * v;
+  const long row0 = ((long)blockIdx.x * (LES_BLOCK / LES_WAVE) + wave) * RPW;  // global row k * ny + j
+  const double* __restrict__ fr = a.f[fld] + row0 * nx;
+  double s[RPW], sj[RPW];
+#pragma unroll
+  for (int r = 0; r < RPW; ++r) {
+    s[r] = 0.0;
+    sj[r] = ((row0 + r) % ny) & 1 ? -1.0 : 1.0;
   }
-  for (int o = LES_WAVE / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, LES_WAVE);
-  if (lane == 0) a.A[(long)fld * ny * a.nzg + row] = s / (double)nx;
+  for (int i = lane; i < nx; i += LES_WAVE) {
+    double v[RPW];
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) v[r] = fr[(long)r * nx + i];
+    double acc = sj[0] * v[0];
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+      s[r] += (i & 1) ? -v[r] : v[r];
+      if (r > 0) acc += sj[r] * v[r];
+    }
+    part[wave][i] = acc;
+  }
+#pragma unroll
+  for (int r = 0; r < RPW; ++r) {
+    for (int o = LES_WAVE / 2; o > 0; o >>= 1) s[r] += __shfl_xor(s[r], o, LES_WAVE);
+    if (lane == 0) a.A[(long)fld * ny * a.nzg + row0 + r] = s[r] / (double)nx;
+  }
   __syncthreads();
   double* __restrict__ P = a.P + (long)fld * (ny / 4) * nx * a.nzg;
   for (int i = threadIdx.x; i < nx; i += blockDim.x)
     P[(long)blockIdx.x * nx + i] = ((part[0][i] + part[1][i]) + part[2][i]) + part[3][i];
 }
 
-// Pass 2, one block per plane and field: B_i = sum of the ny/4 band
-// partials / ny per column; C = sum_j (-1)^j A_j / ny, each thread summing
-// its rows j in order, then a fixed tree.
-__global__ void __launch_bounds__(LES_BLOCK) k_nyq_bc(NyqArgs a) {
-  const int nx = a.nx, ny = a.ny, nb = ny / 4, fld = blockIdx.y;
+// Pass 2, one block per plane and field: B_i = sum of the nb band partials /
+// ny per column; C = sum_j (-1)^j A_j / ny, each thread summing its rows j in
+// order, then a fixed tree.
+__global__ void __launch_bounds__(LES_BLOCK) k_nyq_bc(NyqArgs a, int nb) {
+  const int nx = a.nx, ny = a.ny, fld = blockIdx.y;
   const long k = blockIdx.x;
-  const double* __restrict__ P = a.P + (long)fld * nb * nx * a.nzg;
+  const double* __restrict__ P = a.P + (long)fld * (ny / 4) * nx * a.nzg;
   const double* __restrict__ A = a.A + (long)fld * ny * a.nzg + k * ny;
   double* __restrict__ B = a.B + (long)fld * nx * a.nzg + k * nx;
   for (int i = threadIdx.x; i < nx; i += blockDim.x) {
@@ -483,19 +498,25 @@ __global__ void __launch_bounds__(LES_BLOCK) k_nyq_bc(NyqArgs a) {
   if (threadIdx.x == 0) a.C[(long)fld * a.nzg + k] = red[0] / (double)ny;
 }
 
-// Pass 3: f -= (-1)^i A_j + (-1)^j B_i - (-1)^(i+j) C, one block per field row.
+// Pass 3: f -= (-1)^i A_j + (-1)^j B_i - (-1)^(i+j) C, RPW rows per wavefront.
+template <int RPW>
 __global__ void __launch_bounds__(LES_BLOCK) k_nyq_apply(NyqArgs a) {
   const int nx = a.nx, ny = a.ny, fld = blockIdx.y;
-  const long r = blockIdx.x;
-  const int j = (int)(r % ny);
-  const long k = r / ny;
-  const double sj = (j & 1) ? -1.0 : 1.0;
-  const double av = a.A[(long)fld * ny * a.nzg + r], c = a.C[(long)fld * a.nzg + k];
-  const double* __restrict__ B = a.B + (long)fld * nx * a.nzg + k * nx;
-  double* __restrict__ f = a.f[fld] + r * nx;
-  for (int i = threadIdx.x; i < nx; i += blockDim.x) {
-    const double si = (i & 1) ? -1.0 : 1.0;
-    f[i] -= si * av + sj * B[i] - si * sj * c;
+  const int lane = threadIdx.x & (LES_WAVE - 1), wave = threadIdx.x / LES_WAVE;
+  const long row0 = ((long)blockIdx.x * (LES_BLOCK / LES_WAVE) + wave) * RPW;
+#pragma unroll
+  for (int r = 0; r < RPW; ++r) {
+    const long row = row0 + r;
+    const int j = (int)(row % ny);
+    const long k = row / ny;
+    const double sj = (j & 1) ? -1.0 : 1.0;
+    const double av = a.A[(long)fld * ny * a.nzg + row], c = a.C[(long)fld * a.nzg + k];
+    const double* __restrict__ B = a.B + (long)fld * nx * a.nzg + k * nx;
+    double* __restrict__ f = a.f[fld] + row * nx;
+    for (int i = lane; i < nx; i += LES_WAVE) {
+      const double si = (i & 1) ? -1.0 : 1.0;
+      f[i] -= si * av + sj * B[i] - si * sj * c;
+    }
   }
 }
 
@@ -510,10 +531,14 @@ static void nyquist_remove_fp64(Les& L) {
   a.P = L.nyq_part;
   a.nx = L.nx; a.ny = L.ny; a.nzg = L.nzg;
   const long nrows = (long)L.ny * L.nzg;
+  const int rpw = L.ny % 16 == 0 ? 4 : 1;
+  const unsigned nblocks = (unsigned)(nrows / (4 * rpw));
   kt_begin(L, KT_POINTWISE);
-  hipLaunchKernelGGL(k_nyq_reduce, dim3((unsigned)(nrows / 4), 3), dim3(LES_BLOCK), 0, L.stream, a);
-  hipLaunchKernelGGL(k_nyq_bc, dim3((unsigned)L.nzg, 3), dim3(LES_BLOCK), 0, L.stream, a);
-  hipLaunchKernelGGL(k_nyq_apply, dim3((unsigned)nrows, 3), dim3(LES_BLOCK), 0, L.stream, a);
+  if (rpw == 4) hipLaunchKernelGGL(k_nyq_reduce<4>, dim3(nblocks, 3), dim3(LES_BLOCK), 0, L.stream, a);
+  else hipLaunchKernelGGL(k_nyq_reduce<1>, dim3(nblocks, 3), dim3(LES_BLOCK), 0, L.stream, a);
+  hipLaunchKernelGGL(k_nyq_bc, dim3((unsigned)L.nzg, 3), dim3(LES_BLOCK), 0, L.stream, a, L.ny / (4 * rpw));
+  if (rpw == 4) hipLaunchKernelGGL(k_nyq_apply<4>, dim3(nblocks, 3), dim3(LES_BLOCK), 0, L.stream, a);
+  else hipLaunchKernelGGL(k_nyq_apply<1>, dim3(nblocks, 3), dim3(LES_BLOCK), 0, L.stream, a);
   kt_end(L, KT_POINTWISE, 3.0 * 24.0 * L.nf);
 }
 

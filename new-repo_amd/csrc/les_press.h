@@ -22,10 +22,12 @@ struct TriArgs {
 // pair computes the same factors and one component each (twice the
 // parallelism of one thread per mode, whose few waves leave most SIMDs idle;
 // adjacent lanes read and write the two halves of each complex value).  The
-// sweeps are sequential in z but their loads are not: each pass over a chunk
-// of LES_TRI_U levels issues the chunk's loads first.  Per component the
-// arithmetic and its order are those of the plain sweep.
-#define LES_TRI_U 8
+// sweeps are sequential in z but their loads are not: the loads of chunk n+1
+// of LES_TRI_U levels are in flight while chunk n is computed.  Per
+// component the arithmetic and its order are those of the plain sweep.
+#ifndef LES_TRI_U
+#define LES_TRI_U 16
+#endif
 __global__ void __launch_bounds__(64) k_tridiag(TriArgs a) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long q = t >> 1;
@@ -54,27 +56,31 @@ __global__ void __launch_bounds__(64) k_tridiag(TriArgs a) {
   double x = r1 / bet;
   ph[2 * q] = x;
   double hz = Hzc[2 * (ns + q)];
-  // rows 2..nz+1 (interior levels 1..nz): r = i kx Hx + i ky Hy + dHz/dz
-  for (int k0 = 2; k0 <= nz + 1; k0 += LES_TRI_U) {
-    double hxs[LES_TRI_U], hys[LES_TRI_U], hzs[LES_TRI_U];
+  // rows 2..nz+1 (interior levels 1..nz): r = i kx Hx + i ky Hy + dHz/dz.
+  // Software-pipelined: the loads of the next chunk of LES_TRI_U levels are
+  // issued before the current chunk's recurrence runs.
+  double hA[3][LES_TRI_U], hB[3][LES_TRI_U];
+  auto fwd_load = [&](double (&h)[3][LES_TRI_U], int k0) {
 #pragma unroll
     for (int u = 0; u < LES_TRI_U; ++u) {
       const int k = k0 + u;
       if (k <= nz + 1) {
         const long p = (long)(k - 1) * ns + q;
-        hxs[u] = Hxo[2 * p];
-        hys[u] = Hyo[2 * p];
-        hzs[u] = (k - 1 < nz) ? Hzc[2 * (p + ns)] : 0.0;
+        h[0][u] = Hxo[2 * p];
+        h[1][u] = Hyo[2 * p];
+        h[2][u] = (k - 1 < nz) ? Hzc[2 * (p + ns)] : 0.0;
       }
     }
+  };
+  auto fwd_step = [&](const double (&h)[3][LES_TRI_U], int k0) {
 #pragma unroll
     for (int u = 0; u < LES_TRI_U; ++u) {
       const int k = k0 + u;
       if (k <= nz + 1) {
         // real part: -(kx Hx.y + ky Hy.y) + dHz.x/dz; imaginary: (kx Hx.x + ky Hy.x) + dHz.y/dz
-        const double sk = kx * hxs[u] + ky * hys[u];
-        const double r = (c == 0 ? -sk : sk) + (hzs[u] - hz) / dz;
-        hz = hzs[u];
+        const double sk = kx * h[0][u] + ky * h[1][u];
+        const double r = (c == 0 ? -sk : sk) + (h[2][u] - hz) / dz;
+        hz = h[2][u];
         const double cprev = (k == 2) ? c1 : idz2;
         const double g = cprev / bet;
         gam[(long)k * ns + q] = g;  // both lanes of the pair store the same value
@@ -83,6 +89,14 @@ __global__ void __launch_bounds__(64) k_tridiag(TriArgs a) {
         ph[2 * ((long)(k - 1) * ns + q)] = x;
       }
     }
+  };
+  fwd_load(hA, 2);
+  for (int k0 = 2; k0 <= nz + 1; k0 += 2 * LES_TRI_U) {
+    if (k0 + LES_TRI_U <= nz + 1) fwd_load(hB, k0 + LES_TRI_U);
+    fwd_step(hA, k0);
+    if (k0 + LES_TRI_U > nz + 1) break;
+    if (k0 + 2 * LES_TRI_U <= nz + 1) fwd_load(hA, k0 + 2 * LES_TRI_U);
+    fwd_step(hB, k0 + LES_TRI_U);
   }
   // row nz+2: top layer
   {
@@ -94,25 +108,36 @@ __global__ void __launch_bounds__(64) k_tridiag(TriArgs a) {
     bet = bk - am * g;
     x = (r - am * x) / bet;
   }
-  // back substitution: x holds x(nz+2); each lane reads back its own stores
-  for (int k1 = nz + 1; k1 >= 1; k1 -= LES_TRI_U) {
-    double gs[LES_TRI_U], xs[LES_TRI_U];
+  // back substitution: x holds x(nz+2); each lane reads back its own
+  // stores, again with the next chunk's loads issued first
+  double gA[2][LES_TRI_U], gB[2][LES_TRI_U];
+  auto bwd_load = [&](double (&b)[2][LES_TRI_U], int k1) {
 #pragma unroll
     for (int u = 0; u < LES_TRI_U; ++u) {
       const int k = k1 - u;
       if (k >= 1) {
-        gs[u] = gam[(long)(k + 1) * ns + q];
-        xs[u] = ph[2 * ((long)(k - 1) * ns + q)];
+        b[0][u] = gam[(long)(k + 1) * ns + q];
+        b[1][u] = ph[2 * ((long)(k - 1) * ns + q)];
       }
     }
+  };
+  auto bwd_step = [&](const double (&b)[2][LES_TRI_U], int k1) {
 #pragma unroll
     for (int u = 0; u < LES_TRI_U; ++u) {
       const int k = k1 - u;
       if (k >= 1) {
-        x = xs[u] - gs[u] * x;
+        x = b[1][u] - b[0][u] * x;
         ph[2 * ((long)(k - 1) * ns + q)] = x;
       }
     }
+  };
+  bwd_load(gA, nz + 1);
+  for (int k1 = nz + 1; k1 >= 1; k1 -= 2 * LES_TRI_U) {
+    if (k1 - LES_TRI_U >= 1) bwd_load(gB, k1 - LES_TRI_U);
+    bwd_step(gA, k1);
+    if (k1 - LES_TRI_U < 1) break;
+    if (k1 - 2 * LES_TRI_U >= 1) bwd_load(gA, k1 - 2 * LES_TRI_U);
+    bwd_step(gB, k1 - LES_TRI_U);
   }
   ph[2 * ((long)(nz + 1) * ns + q)] = 0.0;
 }
