@@ -30,27 +30,40 @@ typedef float f4v __attribute__((ext_vector_type(4)));
 // b4[2] xmean[12] xstd[12] ymean[2] ystd[2] xlim
 #define MLP_NPUB (64 * 12 + 64 + 64 * 64 + 64 + 64 * 64 + 64 + 2 * 64 + 2 + 12 + 12 + 2 + 2 + 1)
 
-__device__ __forceinline__ short f2bf(float f) {
-  unsigned u = __float_as_uint(f);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (short)(u >> 16);
-}
+// float -> bf16, round to nearest even (v_cvt_pk_bf16_f32 on gfx950)
+__device__ __forceinline__ short f2bf(float f) { return __builtin_bit_cast(short, (__bf16)f); }
 __device__ __forceinline__ float bf2f(short b) { return __uint_as_float(((unsigned)(unsigned short)b) << 16); }
 
 __device__ __forceinline__ f4v mfma16(s4v a, s4v b, f4v c) {
   return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
-// GELU, tanh form, and its derivative
 // GELU, tanh form: 0.5 x (1 + tanh(y)), y = c (x + 0.044715 x^3), evaluated
-// as x s with s = 0.5 (1 + tanh(y)) = 1 / (1 + exp(-2y)): one hardware exp
-// and one reciprocal instead of tanhf.  Derivative: s + 2 x s (1 - s) y'/x'.
+// as x s with s = 0.5 (1 + tanh(y)) = 1 / (1 + exp(-2y)) = 1 / (1 + 2^(x (k1 +
+// k2 x^2))), k1 = -2 c log2(e), k2 = 0.044715 k1: one multiply, one fused
+// multiply-add and one multiply feed the hardware exp2 and reciprocal.
+// Derivative: s + 2 x s (1 - s) y'/x'.
 __device__ __forceinline__ float gelu_s(float x) {
-  const float c = 0.7978845608028654f;
-  const float y = c * (x + 0.044715f * x * x * x);
-  return __builtin_amdgcn_rcpf(1.0f + __expf(-2.0f * y));
+  constexpr double k1d = -2.0 * 0.7978845608028654 * 1.4426950408889634;
+  constexpr float k1 = (float)k1d, k2 = (float)(0.044715 * k1d);
+  const float p = __builtin_fmaf(x * x, k2, k1);
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(x * p));
 }
 __device__ __forceinline__ float gelu(float x) { return x * gelu_s(x); }
+// GELU of four accumulators as four bf16 operand values: two
+// v_cvt_pk_bf16_f32 (round to nearest even), each packing a pair into one
+// register (the compiler otherwise converts one value per instruction and
+// packs with v_perm).
+__device__ __forceinline__ unsigned cvt_pk_bf16(float a, float b) {
+  unsigned r;
+  __asm__("v_cvt_pk_bf16_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+__device__ __forceinline__ s4v gelu_bf4(f4v v) {
+  typedef unsigned u2v __attribute__((ext_vector_type(2)));
+  const u2v w = {cvt_pk_bf16(gelu(v[0]), gelu(v[1])), cvt_pk_bf16(gelu(v[2]), gelu(v[3]))};
+  return __builtin_bit_cast(s4v, w);
+}
 __device__ __forceinline__ float gelu_d(float x) {
   const float c = 0.7978845608028654f;
   const float s = gelu_s(x);
@@ -256,17 +269,17 @@ __global__ void __launch_bounds__(256) k_mlp_infer(InferArgs a) {
     for (int rb = 0; rb < 4; ++rb) {
       f4v acc = bb1[4 * rb];
       acc = mfma16(w1[rb * 64], x, acc);
-      for (int j = 0; j < 4; ++j) h1[rb][j] = f2bf(gelu(acc[j]));
+      h1[rb] = gelu_bf4(acc);
     }
     for (int rb = 0; rb < 4; ++rb) {
       f4v acc = bb2[4 * rb];
       for (int cb = 0; cb < 4; ++cb) acc = mfma16(w2[(rb * 4 + cb) * 64], h1[cb], acc);
-      for (int j = 0; j < 4; ++j) h2[rb][j] = f2bf(gelu(acc[j]));
+      h2[rb] = gelu_bf4(acc);
     }
     for (int rb = 0; rb < 4; ++rb) {
       f4v acc = bb3[4 * rb];
       for (int cb = 0; cb < 4; ++cb) acc = mfma16(w3[(rb * 4 + cb) * 64], h2[cb], acc);
-      for (int j = 0; j < 4; ++j) h3[rb][j] = f2bf(gelu(acc[j]));
+      h3[rb] = gelu_bf4(acc);
     }
     f4v y = bb4;
     for (int cb = 0; cb < 4; ++cb) y = mfma16(w4[cb * 64], h3[cb], y);
