@@ -100,12 +100,12 @@ struct RowC2RArgs {
   int ls;  // LDS line stride per wavefront (row_line<T>(nx))
 };
 
+// Load the half spectra of rows (2jp, 2jp+1) of plane k as one complex line
+// and run the inverse transform: z then holds row 2jp in .x, row 2jp+1 in .y.
 template <typename T, typename TO, typename FFT>
-__device__ __forceinline__ void rows_c2r_body(const RowC2RArgs<T, TO>& a, Cplx<T>* z, int lane, long job, FFT&& fft) {
+__device__ __forceinline__ void rows_c2r_line(const RowC2RArgs<T, TO>& a, Cplx<T>* z, int lane, int k, int jp, FFT&& fft) {
   const int nx = a.nx, lhx = a.lhx, h = nx / 2;
-  const int npairs = a.ny / 2;
   const int f = blockIdx.y;
-  const int k = a.k0 + (int)(job / npairs), jp = (int)(job % npairs);
   const Cplx<T>* __restrict__ src = a.src[f] + ((long)k * a.ny + 2 * jp) * lhx;
   for (int kk = lane; kk < lhx; kk += LES_WAVE) {
     Cplx<T> A = src[kk], B = src[kk + lhx];
@@ -118,7 +118,15 @@ __device__ __forceinline__ void rows_c2r_body(const RowC2RArgs<T, TO>& a, Cplx<T
   }
   wave_sync();
   fft(z, true);
-  TO* __restrict__ dst = a.dst[f] + ((long)k * a.ny + 2 * jp) * nx;
+}
+
+template <typename T, typename TO, typename FFT>
+__device__ __forceinline__ void rows_c2r_body(const RowC2RArgs<T, TO>& a, Cplx<T>* z, int lane, long job, FFT&& fft) {
+  const int nx = a.nx;
+  const int npairs = a.ny / 2;
+  const int k = a.k0 + (int)(job / npairs), jp = (int)(job % npairs);
+  rows_c2r_line(a, z, lane, k, jp, fft);
+  TO* __restrict__ dst = a.dst[blockIdx.y] + ((long)k * a.ny + 2 * jp) * nx;
   for (int n = lane; n < nx; n += LES_WAVE) {
     const Cplx<T> v = z[n];
     dst[n] = (TO)v.x;

@@ -281,7 +281,6 @@ static void convection(Les& L, double* cx, double* cy, double* cz) {
 template <typename T>
 struct DivEpiArgs {
   double* rhs;
-  const T* divh;
   const T* tz;
   int stag, save_wk;
   double *bottom_wk, *top_wk;
@@ -294,28 +293,94 @@ struct DivEpiArgs {
   int coriolis;
 };
 
+// rhs(k) += (horizontal divergence) + dz(tz) + forcing, applied as the store
+// of the inverse x transform of the divergence spectrum, so the divergence
+// never goes through HBM; levels 1..nz+1 (nz+1 only for top_wk).
 template <typename T>
-__global__ void k_div_epi(DivEpiArgs<T> a) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= a.np * a.nz) return;
-  const int k = 1 + (int)(t / a.np);
-  const long q = t % a.np;
+__device__ __forceinline__ void div_epi_point(const DivEpiArgs<T>& a, int k, long q, double dh) {
   const long c = (long)k * a.np + q;
-  const double dzt = a.stag ? a.cz * ((double)a.tz[c + a.np] - (double)a.tz[c])
-                            : a.cz * ((double)a.tz[c] - (double)a.tz[c - a.np]);
-  double r = a.rhs[c] + (double)a.divh[c] + dzt;
-  if (a.forcing == 1) {
-    r = r + a.pf;
-    if (a.coriolis) r = r + a.coriol * a.vel[c] - a.coriol * a.geo;
-  } else if (a.forcing == 2) {
-    r = r + a.pf;
-    if (a.coriolis) r = r - a.coriol * a.vel[c] + a.coriol * a.geo;
+  if (k <= a.nz) {
+    const double dzt = a.stag ? a.cz * ((double)a.tz[c + a.np] - (double)a.tz[c])
+                              : a.cz * ((double)a.tz[c] - (double)a.tz[c - a.np]);
+    double r = a.rhs[c] + dh + dzt;
+    if (a.forcing == 1) {
+      r = r + a.pf;
+      if (a.coriolis) r = r + a.coriol * a.vel[c] - a.coriol * a.geo;
+    } else if (a.forcing == 2) {
+      r = r + a.pf;
+      if (a.coriolis) r = r - a.coriol * a.vel[c] + a.coriol * a.geo;
+    }
+    a.rhs[c] = r;
+    if (a.save_wk && k == 1) a.bottom_wk[q] = dh + a.cz * ((double)a.tz[c] - (double)a.tz[q]);
+  } else if (a.save_wk) {
+    a.top_wk[q] = dh;
   }
-  a.rhs[c] = r;
-  if (a.save_wk && k == 1) {
-    a.bottom_wk[q] = (double)a.divh[c] + a.cz * ((double)a.tz[c] - (double)a.tz[q]);
-    a.top_wk[q] = (double)a.divh[(long)(a.nz + 1) * a.np + q];
+}
+
+template <typename T>
+struct DivEpi3 {
+  DivEpiArgs<T> c[3];  // per component (blockIdx.y)
+};
+
+template <typename T>
+__global__ void __launch_bounds__(LES_BLOCK) k_rows_c2r_div(RowC2RArgs<T, T> a, DivEpi3<T> ep, FftPlan pl,
+                                                            const Cplx<T>* __restrict__ tw) {
+  const DivEpiArgs<T>& e = ep.c[blockIdx.y];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int lane = threadIdx.x & (LES_WAVE - 1), wave = threadIdx.x / LES_WAVE;
+  Cplx<T>* z = reinterpret_cast<Cplx<T>*>(smem_raw) + wave * a.ls;
+  const long job = (long)blockIdx.x * (blockDim.x / LES_WAVE) + wave;
+  const int npairs = a.ny / 2;
+  if (job >= (long)a.nplanes * npairs) return;
+  const int k = a.k0 + (int)(job / npairs), jp = (int)(job % npairs);
+  if (pl.n == 256) {
+    TwReg<T, 256> tr;
+    tw_load(tr, tw + pl.twf, lane);
+    rows_c2r_line(a, z, lane, k, jp, [&](Cplx<T>* p, bool inv) { wave_fft_t<T, 256, row_layout<T>(256)>(p, tr, inv, lane); });
+  } else {
+    rows_c2r_line(a, z, lane, k, jp, [&](Cplx<T>* p, bool inv) { wave_fft(p, pl, tw, inv, lane); });
   }
+  const long q0 = (long)(2 * jp) * a.nx;
+  for (int n = lane; n < a.nx; n += LES_WAVE) {
+    const Cplx<T> v = z[n];
+    div_epi_point(e, k, q0 + n, (double)v.x);
+    div_epi_point(e, k, q0 + a.nx + n, (double)v.y);
+  }
+}
+
+template <typename T>
+static DivEpiArgs<T> div_epi_args(Les& L, int rhs_id, int tz, int stag, int save_wk, int forcing) {
+  DivEpiArgs<T> a;
+  a.rhs = D(L, rhs_id); a.tz = F<T>(L, tz);
+  a.stag = stag; a.save_wk = save_wk;
+  a.bottom_wk = L.layer[LES_BOTTOM_WK]; a.top_wk = L.layer[LES_TOP_WK];
+  a.cz = 1.0 / L.dz; a.np = L.np; a.nz = L.nz;
+  a.forcing = forcing;
+  a.pf = forcing == 1 ? L.P.pfx : L.P.pfy;
+  a.coriol = L.P.coriol;
+  a.geo = forcing == 1 ? L.P.vg : L.P.ug;
+  a.vel = forcing == 1 ? D(L, LES_V) : D(L, LES_U);
+  a.coriolis = L.P.coriolis;
+  return a;
+}
+
+// Inverse x transform of nf divergence spectra (planes 1..nz+1) with the
+// rhs update of each component as the store.
+template <typename T>
+static void rows_c2r_div(Les& L, const Cplx<T>* const* src, const DivEpi3<T>& ep, int nf) {
+  RowC2RArgs<T, T> ca;
+  for (int f = 0; f < nf; ++f) { ca.src[f] = src[f]; ca.dst[f] = nullptr; }
+  ca.nx = L.nx; ca.ny = L.ny; ca.lhx = L.lhx; ca.k0 = 1; ca.nplanes = L.nz + 1;
+  ca.ls = row_line<T>(L.nx);
+  const long jobs = (long)ca.nplanes * (L.ny / 2);
+  const size_t sh = (size_t)kWavesPerBlock * ca.ls * sizeof(Cplx<T>);
+  double extra = 0.0;
+  for (int f = 0; f < nf; ++f) extra += (ep.c[f].coriolis && ep.c[f].forcing) ? 8.0 : 0.0;
+  kt_begin(L, KT_ROWS_C2R);
+  hipLaunchKernelGGL((k_rows_c2r_div<T>), dim3((unsigned)((jobs + kWavesPerBlock - 1) / kWavesPerBlock), nf),
+                     dim3(LES_BLOCK), sh, L.stream, ca, ep, L.px, twpool<T>(L));
+  kt_end(L, KT_ROWS_C2R, (double)nf * (L.nz + 1) * L.ny * ((double)L.lhx * sizeof(Cplx<T>)) +
+                             (double)L.np * L.nz * (nf * (16.0 + sizeof(T)) + extra));
 }
 
 template <typename T>
@@ -328,24 +393,32 @@ static void div_stress(Les& L, int rhs_id, int tx, int ty, int tz, int stag, int
   const T c = (T)(1.0 / (double)(L.nx * L.ny));
   rows_r2c<T, T>(L, src, dst, 2, 1, L.nz + 1, c);
   cols<T>(L, COP_DIV, s0, s1, s2, nullptr, nullptr, 1, 1, 1, 1, L.nz + 1);
-  T* divh = (T*)L.wk_real[62];
+  DivEpi3<T> ep;
+  ep.c[0] = div_epi_args<T>(L, rhs_id, tz, stag, save_wk, forcing);
   const Cplx<T>* cs[1] = {s2};
-  T* outs[1] = {divh};
-  rows_c2r<T, T>(L, cs, outs, 1, 1, L.nz + 1);
-  DivEpiArgs<T> a;
-  a.rhs = D(L, rhs_id); a.divh = divh; a.tz = F<T>(L, tz);
-  a.stag = stag; a.save_wk = save_wk;
-  a.bottom_wk = L.layer[LES_BOTTOM_WK]; a.top_wk = L.layer[LES_TOP_WK];
-  a.cz = 1.0 / L.dz; a.np = L.np; a.nz = L.nz;
-  a.forcing = forcing;
-  a.pf = forcing == 1 ? L.P.pfx : L.P.pfy;
-  a.coriol = L.P.coriol;
-  a.geo = forcing == 1 ? L.P.vg : L.P.ug;
-  a.vel = forcing == 1 ? D(L, LES_V) : D(L, LES_U);
-  a.coriolis = L.P.coriolis;
-  kt_begin(L, KT_POINTWISE);
-  hipLaunchKernelGGL((k_div_epi<T>), dim3(nblk(L.np * L.nz)), dim3(LES_BLOCK), 0, L.stream, a);
-  kt_end(L, KT_POINTWISE, (double)L.np * L.nz * (16.0 + 2.0 * sizeof(T)));
+  rows_c2r_div<T>(L, cs, ep, 1);
+}
+
+// The three components together: the five distinct stress fields are
+// transformed once (txy enters the x and y divergences), then one inverse
+// launch updates rhsx, rhsy, rhsz.  The arithmetic per component is that of
+// div_stress.
+template <typename T>
+static void div_stress_xyz(Les& L) {
+  Cplx<T>* sp[8];
+  for (int i = 0; i < 8; ++i) sp[i] = (Cplx<T>*)L.wk_spec[i];
+  static_assert(LES_NSPEC >= 8, "div_stress_xyz needs 8 spectral workspaces");
+  const T* src[5] = {F<T>(L, LES_TXX), F<T>(L, LES_TXY), F<T>(L, LES_TYY), F<T>(L, LES_TXZ), F<T>(L, LES_TYZ)};
+  const T c = (T)(1.0 / (double)(L.nx * L.ny));
+  rows_r2c<T, T>(L, src, sp, 5, 1, L.nz + 1, c);
+  cols<T>(L, COP_DIV, sp[0], sp[1], sp[5], nullptr, nullptr, 1, 1, 1, 1, L.nz + 1);
+  cols<T>(L, COP_DIV, sp[1], sp[2], sp[6], nullptr, nullptr, 1, 1, 1, 1, L.nz + 1);
+  cols<T>(L, COP_DIV, sp[3], sp[4], sp[7], nullptr, nullptr, 1, 1, 1, 1, L.nz + 1);
+  DivEpi3<T> ep;
+  const int rhs[3] = {LES_RHSX, LES_RHSY, LES_RHSZ}, tz[3] = {LES_TXZ, LES_TYZ, LES_TZZ};
+  for (int f = 0; f < 3; ++f) ep.c[f] = div_epi_args<T>(L, rhs[f], tz[f], f < 2 ? 1 : 0, f == 2 ? 1 : 0, f < 2 ? f + 1 : 0);
+  const Cplx<T>* cs[3] = {sp[5], sp[6], sp[7]};
+  rows_c2r_div<T>(L, cs, ep, 3);
 }
 
 template <typename T>
@@ -358,7 +431,5 @@ static void compute_rhs_adv(Les& L) {
     L.f[LES_RHSX_F + c] = t;
   }
   convection<T>(L, D(L, LES_RHSX), D(L, LES_RHSY), D(L, LES_RHSZ));
-  div_stress<T>(L, LES_RHSX, LES_TXX, LES_TXY, LES_TXZ, 1, 0, 1);
-  div_stress<T>(L, LES_RHSY, LES_TXY, LES_TYY, LES_TYZ, 1, 0, 2);
-  div_stress<T>(L, LES_RHSZ, LES_TXZ, LES_TYZ, LES_TZZ, 0, 1, 0);
+  div_stress_xyz<T>(L);
 }

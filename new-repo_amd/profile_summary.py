@@ -70,7 +70,7 @@ def _per_dispatch(d: str, counter: str):
 
 
 # kernel-name patterns of the benchmark's kernel families
-FAMILY_KERNELS = {"rows_r2c": r"\bk_rows_r2c<", "rows_c2r": r"\bk_rows_c2r<",
+FAMILY_KERNELS = {"rows_r2c": r"\bk_rows_r2c<", "rows_c2r": r"\bk_rows_c2r(_div)?<",
                   "cols": r"\bk_cols(_256|_c16)?<", "rows_resample": r"\bk_rows_resample<",
                   "cols_resample": r"\bk_(cols_resample|colres_c16)<", "tridiag": r"\bk_tridiag",
                   "lagrangian": r"\bk_lagr<"}
