@@ -555,6 +555,9 @@ static void ab2(Les& L, double tadv1, double tadv2) {
 }
 
 // ------------------------------------------------------------------ step
+#ifndef LES_OVERLAP
+#define LES_OVERLAP 1
+#endif
 template <typename T>
 static int les_step_t(Les& L) {
   L.jt_total += 1;
@@ -564,9 +567,28 @@ static int les_step_t(Les& L) {
   wall_law<T>(L);
   compute_vel_grad<T>(L);
   if (L.timing) hipEventRecord(L.ph_ev[1], L.stream);
-  sgs_stress<T>(L, L.jt, jtt);
-  if (L.timing) hipEventRecord(L.ph_ev[2], L.stream);
-  compute_rhs_adv<T>(L);
+  if (LES_OVERLAP && !L.timing && L.P.dealias == 0) {
+    // The SGS stage and the padded convection read the same velocity and
+    // gradients and write disjoint buffers: the SGS stage runs on the second
+    // stream while the convection runs on the first; the stress divergence
+    // waits for both.  (The nx x ny convections share the spectral
+    // workspaces with the dynamic models and stay sequential, as does the
+    // per-kernel timing mode.)
+    hipEventRecord(L.ev_fork, L.stream);
+    hipStreamWaitEvent(L.stream2, L.ev_fork, 0);
+    hipStream_t s1 = L.stream;
+    L.stream = L.stream2;
+    sgs_stress<T>(L, L.jt, jtt);
+    hipEventRecord(L.ev_join, L.stream2);
+    L.stream = s1;
+    rhs_convection<T>(L);
+    hipStreamWaitEvent(L.stream, L.ev_join, 0);
+    div_stress_xyz<T>(L);
+  } else {
+    sgs_stress<T>(L, L.jt, jtt);
+    if (L.timing) hipEventRecord(L.ph_ev[2], L.stream);
+    compute_rhs_adv<T>(L);
+  }
   if (L.timing) hipEventRecord(L.ph_ev[3], L.stream);
   double tadv1 = (jtt == 1) ? 1.0 : 1.5;
   double tadv2 = (jtt == 1) ? 0.0 : 1.0 - tadv1;
@@ -633,6 +655,9 @@ static void free_all(Les* L) {
   L->allocs.clear();
   for (int i = 0; i < KT_COUNT; ++i) { hipEventDestroy(L->ev_a[i]); hipEventDestroy(L->ev_b[i]); }
   for (int i = 0; i < 6; ++i) hipEventDestroy(L->ph_ev[i]);
+  hipEventDestroy(L->ev_fork);
+  hipEventDestroy(L->ev_join);
+  hipStreamDestroy(L->stream2);
   hipStreamDestroy(L->stream);
 }
 
@@ -684,6 +709,14 @@ int les_init(const les_params* p) {
   L->F_coeff_init = P.lagran_init != 0;
   L->mixed = P.precision != 0;
   if (hipStreamCreate(&L->stream) != hipSuccess) { delete L; les_set_error("stream"); return 5; }
+  if (hipStreamCreate(&L->stream2) != hipSuccess) {
+    hipStreamDestroy(L->stream);
+    delete L;
+    les_set_error("stream");
+    return 5;
+  }
+  hipEventCreateWithFlags(&L->ev_fork, hipEventDisableTiming);
+  hipEventCreateWithFlags(&L->ev_join, hipEventDisableTiming);
   {
     int dev = 0;
     L->ncu = 0;

@@ -421,15 +421,20 @@ static void div_stress_xyz(Les& L) {
   rows_c2r_div<T>(L, cs, ep, 3);
 }
 
+// First part of compute_RHS_adv: rhs_f <- rhs (src/LES_engine.f90:517-522)
+// as a buffer swap, the new rhs being fully overwritten by the convective term.
 template <typename T>
-static void compute_rhs_adv(Les& L) {
-  // rhs_f <- rhs (src/LES_engine.f90:517-522) as a buffer swap: the new rhs
-  // is fully overwritten by the convective term below.
+static void rhs_convection(Les& L) {
   for (int c = 0; c < 3; ++c) {
     void* t = L.f[LES_RHSX + c];
     L.f[LES_RHSX + c] = L.f[LES_RHSX_F + c];
     L.f[LES_RHSX_F + c] = t;
   }
   convection<T>(L, D(L, LES_RHSX), D(L, LES_RHSY), D(L, LES_RHSZ));
+}
+
+template <typename T>
+static void compute_rhs_adv(Les& L) {
+  rhs_convection<T>(L);
   div_stress_xyz<T>(L);
 }

@@ -24,6 +24,8 @@ struct Les {
   int jt;                // steps since les_init (the reference's local count)
   int F_coeff_init;
   hipStream_t stream;
+  hipStream_t stream2;   // second stream of the step: the SGS stage beside the convection
+  hipEvent_t ev_fork, ev_join;
   int mixed;             // flow kernels in FP32
 
   // host-built FFT plans and twiddles
