@@ -17,7 +17,10 @@
 
 // ------------------------------------------------------------------ row r2c
 // Load modes of the real input.
-enum { LOAD_SCALE = 0, LOAD_SCALE_DIV = 1 };
+// LOAD_AB2_SCALE_DIV: the AB2 predictor u + dt (t1 r + t2 r_f) of each
+// value (the arithmetic of k_ab2, src/LES_engine.f90:253-283), then
+// LOAD_SCALE_DIV: the pressure right-hand side straight from u, rhs, rhs_f.
+enum { LOAD_SCALE = 0, LOAD_SCALE_DIV = 1, LOAD_AB2_SCALE_DIV = 2 };
 
 template <typename TI, typename T>
 struct RowR2CArgs {
@@ -28,6 +31,9 @@ struct RowR2CArgs {
   int mode;
   T scale, div;
   int ls;           // LDS line stride per wavefront (row_line<T>(nx))
+  const TI* ab_r[3];  // LOAD_AB2_SCALE_DIV: rhs and rhs_f of field f
+  const TI* ab_f[3];
+  T ab_dt, ab_t1, ab_t2;
 };
 
 // LDS layout of the row transforms: the register-twiddle 256-point
@@ -53,7 +59,14 @@ __device__ __forceinline__ void rows_r2c_body(const RowR2CArgs<TI, T>& a, Cplx<T
   const TI* __restrict__ src = a.src[f] + ((long)k * a.ny + 2 * jp) * nx;
   for (int n = lane; n < nx; n += LES_WAVE) {
     T v0 = (T)src[n], v1 = (T)src[n + nx];
-    if (a.mode == LOAD_SCALE_DIV) {
+    if (a.mode == LOAD_AB2_SCALE_DIV) {
+      const long o = ((long)k * a.ny + 2 * jp) * nx + n;
+      const TI* __restrict__ r = a.ab_r[f];
+      const TI* __restrict__ rf = a.ab_f[f];
+      v0 = v0 + a.ab_dt * (a.ab_t1 * (T)r[o] + a.ab_t2 * (T)rf[o]);
+      v1 = v1 + a.ab_dt * (a.ab_t1 * (T)r[o + nx] + a.ab_t2 * (T)rf[o + nx]);
+    }
+    if (a.mode != LOAD_SCALE) {
       v0 = a.scale * (v0 / a.div);
       v1 = a.scale * (v1 / a.div);
     } else {

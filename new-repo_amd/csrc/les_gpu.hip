@@ -124,18 +124,23 @@ static const int kWavesPerBlock = LES_BLOCK / LES_WAVE;
 
 template <typename TI, typename T>
 static void rows_r2c(Les& L, const TI* const* src, Cplx<T>* const* dst, int nf, int k0, int npl, T scale, int mode = LOAD_SCALE,
-                     T div = (T)1, int ny_override = 0) {
+                     T div = (T)1, int ny_override = 0, const TI* const* ab = nullptr, const T* abc = nullptr) {
   RowR2CArgs<TI, T> a;
   for (int f = 0; f < nf; ++f) { a.src[f] = src[f]; a.dst[f] = dst[f]; }
   a.nx = L.nx; a.ny = ny_override ? ny_override : L.ny; a.lhx = L.lhx; a.k0 = k0; a.nplanes = npl;
   a.mode = mode; a.scale = scale; a.div = div;
   a.ls = row_line<T>(L.nx);
+  if (mode == LOAD_AB2_SCALE_DIV) {
+    for (int f = 0; f < nf && f < 3; ++f) { a.ab_r[f] = ab[f]; a.ab_f[f] = ab[3 + f]; }
+    a.ab_dt = abc[0]; a.ab_t1 = abc[1]; a.ab_t2 = abc[2];
+  }
   long jobs = (long)npl * (a.ny / 2);
   dim3 grid((unsigned)((jobs + kWavesPerBlock - 1) / kWavesPerBlock), nf);
   size_t sh = (size_t)kWavesPerBlock * a.ls * sizeof(Cplx<T>);
   kt_begin(L, KT_ROWS_R2C);
   hipLaunchKernelGGL((k_rows_r2c<TI, T>), grid, dim3(LES_BLOCK), sh, L.stream, a, L.px, twpool<T>(L));
-  kt_end(L, KT_ROWS_R2C, (double)nf * npl * a.ny * ((double)L.nx * sizeof(TI) + (double)L.lhx * sizeof(Cplx<T>)));
+  const double in_fields = mode == LOAD_AB2_SCALE_DIV ? 3.0 : 1.0;
+  kt_end(L, KT_ROWS_R2C, (double)nf * npl * a.ny * (in_fields * L.nx * sizeof(TI) + (double)L.lhx * sizeof(Cplx<T>)));
 }
 
 template <typename T, typename TO>
@@ -592,9 +597,10 @@ static int les_step_t(Les& L) {
   if (L.timing) hipEventRecord(L.ph_ev[3], L.stream);
   double tadv1 = (jtt == 1) ? 1.0 : 1.5;
   double tadv2 = (jtt == 1) ? 0.0 : 1.0 - tadv1;
-  ab2(L, tadv1, tadv2);
-  press_stag_array<T>(L, tadv1);
-  projection(L, tadv1);
+  // AB2 folded into the pressure right-hand side and the projection: the
+  // predictor is recomputed where it is read instead of being stored
+  press_stag_array<T>(L, tadv1, true, tadv2);
+  projection(L, tadv1, true, tadv2);
   if (L.timing) {
     hipEventRecord(L.ph_ev[4], L.stream);
     hipEventSynchronize(L.ph_ev[4]);

@@ -142,8 +142,11 @@ __global__ void __launch_bounds__(64) k_tridiag(TriArgs a) {
   ph[2 * ((long)(nz + 1) * ns + q)] = 0.0;
 }
 
+// With ab2 set the right-hand side is formed from u + dt (tadv1 rhs + tadv2
+// rhs_f) on load (u, v, w themselves are left as they are; the projection
+// forms the same predictor again).
 template <typename T>
-static void press_stag_array(Les& L, double tadv1) {
+static void press_stag_array(Les& L, double tadv1, bool ab2 = false, double tadv2 = 0.0) {
   const int nz = L.nz;
   const double c = 1.0 / (double)(L.nx * L.ny);
   Cplx<double>* Hx = (Cplx<double>*)L.pspec[0];
@@ -153,7 +156,14 @@ static void press_stag_array(Les& L, double tadv1) {
   // rH = const/tadv1 * (u/dt) on levels 1..nz (src/compute_pressure.f90:136-149)
   const double* src[3] = {D(L, LES_U), D(L, LES_V), D(L, LES_W)};
   Cplx<double>* dst[3] = {Hx, Hy, Hz};
-  rows_r2c<double, double>(L, src, dst, 3, 1, nz, c / tadv1, LOAD_SCALE_DIV, L.P.dt);
+  if (ab2) {
+    const double* ab[6] = {D(L, LES_RHSX), D(L, LES_RHSY), D(L, LES_RHSZ),
+                           D(L, LES_RHSX_F), D(L, LES_RHSY_F), D(L, LES_RHSZ_F)};
+    const double abc[3] = {L.P.dt, tadv1, tadv2};
+    rows_r2c<double, double>(L, src, dst, 3, 1, nz, c / tadv1, LOAD_AB2_SCALE_DIV, L.P.dt, 0, ab, abc);
+  } else {
+    rows_r2c<double, double>(L, src, dst, 3, 1, nz, c / tadv1, LOAD_SCALE_DIV, L.P.dt);
+  }
   for (int f = 0; f < 3; ++f) cols<double>(L, COP_COPY, dst[f], nullptr, dst[f], nullptr, nullptr, 1, 1, 0, 1, nz);
   // bottom and top layers (src/compute_pressure.f90:152-165, 196-203)
   Cplx<double>* lb = (Cplx<double>*)L.playr[0];
@@ -191,6 +201,9 @@ struct ProjArgs {
   int nz;
   int ubc_u_neu, ubc_v_neu;
   double ubc_u, ubc_v, ubc_w, bogus;
+  int ab2;                      // u, v, w hold the old velocity: form the AB2 predictor first
+  const double *fx, *fy, *fz;   // rhs_f
+  double dt, t1, t2;
 };
 
 __global__ void k_proj(ProjArgs a) {
@@ -202,16 +215,24 @@ __global__ void k_proj(ProjArgs a) {
   const double gz = a.cz * (a.p[c] - a.p[c - a.np]);
   a.dpdz[c] = gz;
   const double gx = a.dpdx[c], gy = a.dpdy[c];
-  const double un = a.u[c] - a.dtt * gx;
-  const double vn = a.v[c] - a.dtt * gy;
-  double wn = a.w[c] - a.dtt * gz;
-  a.rx[c] = a.rx[c] - gx;
-  a.ry[c] = a.ry[c] - gy;
-  a.rz[c] = a.rz[c] - gz;
+  const double rxc = a.rx[c], ryc = a.ry[c], rzc = a.rz[c];
+  double us = a.u[c], vs = a.v[c], ws = a.w[c];
+  if (a.ab2) {
+    us = us + a.dt * (a.t1 * rxc + a.t2 * a.fx[c]);
+    vs = vs + a.dt * (a.t1 * ryc + a.t2 * a.fy[c]);
+    ws = ws + a.dt * (a.t1 * rzc + a.t2 * a.fz[c]);
+  }
+  const double un = us - a.dtt * gx;
+  const double vn = vs - a.dtt * gy;
+  double wn = ws - a.dtt * gz;
+  a.rx[c] = rxc - gx;
+  a.ry[c] = ryc - gy;
+  a.rz[c] = rzc - gz;
   if (k == 1) {
     wn = 0.0;
     a.u[q] = un - a.lbc_u[q] * a.dz;
     a.v[q] = vn - a.lbc_v[q] * a.dz;
+    if (a.ab2) a.w[q] = 0.0;  // the AB2 pass sets w = 0 on the ghost levels
   }
   a.u[c] = un;
   a.v[c] = vn;
@@ -225,8 +246,11 @@ __global__ void k_proj(ProjArgs a) {
   }
 }
 
-static void projection(Les& L, double tadv1) {
+static void projection(Les& L, double tadv1, bool ab2 = false, double tadv2 = 0.0) {
   ProjArgs a;
+  a.ab2 = ab2 ? 1 : 0;
+  a.fx = D(L, LES_RHSX_F); a.fy = D(L, LES_RHSY_F); a.fz = D(L, LES_RHSZ_F);
+  a.dt = L.P.dt; a.t1 = tadv1; a.t2 = tadv2;
   a.u = D(L, LES_U); a.v = D(L, LES_V); a.w = D(L, LES_W); a.p = D(L, LES_P);
   a.dpdx = D(L, LES_DPDX); a.dpdy = D(L, LES_DPDY); a.dpdz = D(L, LES_DPDZ);
   a.rx = D(L, LES_RHSX); a.ry = D(L, LES_RHSY); a.rz = D(L, LES_RHSZ);
@@ -237,5 +261,5 @@ static void projection(Les& L, double tadv1) {
   a.ubc_u = L.P.ubc_u; a.ubc_v = L.P.ubc_v; a.ubc_w = L.P.ubc_w; a.bogus = BOGUS;
   kt_begin(L, KT_POINTWISE);
   hipLaunchKernelGGL(k_proj, dim3(nblk(L.np * L.nz)), dim3(LES_BLOCK), 0, L.stream, a);
-  kt_end(L, KT_POINTWISE, 8.0 * L.np * L.nz * 16.0);
+  kt_end(L, KT_POINTWISE, 8.0 * L.np * L.nz * (16.0 + (ab2 ? 3.0 : 0.0)));
 }
