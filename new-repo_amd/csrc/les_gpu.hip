@@ -637,6 +637,8 @@ static int setup_kernel_attrs() {
   c16_attrs<float>(big);
   hipFuncSetAttribute((const void*)k_colres_c16<float, true>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
   hipFuncSetAttribute((const void*)k_colres_c16<float, false>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+  hipFuncSetAttribute((const void*)k_colres_c16<double, true>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+  hipFuncSetAttribute((const void*)k_colres_c16<double, false>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
   hipFuncSetAttribute((const void*)k_cols_256<float, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
   hipFuncSetAttribute((const void*)k_cols<double, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
   hipFuncSetAttribute((const void*)k_cols<double, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, big);

@@ -78,14 +78,18 @@ __global__ void k_set_planes(SetPlaneArgs s) {
 #ifndef LES_COLRES_TILE_BYTES
 #define LES_COLRES_TILE_BYTES 64
 #endif
-// FP32 256 <-> 384: register-resident kernel (k_colres_c16); LES_COLRES16=0
-// selects the LDS-engine kernel (k_cols_resample) instead.
+// 256 <-> 384: register-resident kernel (k_colres_c16); LES_COLRES16=0
+// selects the LDS-engine kernel (k_cols_resample) instead, LES_COLRES16_FP64=0
+// for the FP64 path only.
 #ifndef LES_COLRES16
 #define LES_COLRES16 1
 #endif
+#ifndef LES_COLRES16_FP64
+#define LES_COLRES16_FP64 1
+#endif
 template <typename T>
 static void launch_cols_resample(Les& L, ColResArgs<T>& a, int nf, const FftPlan& p1, const FftPlan& p2) {
-  if constexpr (sizeof(T) == 4 && LES_COLRES16) {
+  if constexpr (LES_COLRES16 && (sizeof(T) == 4 || LES_COLRES16_FP64)) {
    if ((a.h1 == 256 && a.h2 == 384) || (a.h1 == 384 && a.h2 == 256)) {
     // register-resident 256 <-> 384 (k_colres_c16)
     a.ncp = (a.W / 2 + 15) / 16;

@@ -124,6 +124,9 @@ def report(args):
     lines = [ln for ln in pathlib.Path(args.trace_json).read_text().splitlines() if ln.startswith("{")]
     tj = json.loads(lines[-1])
     fam = tj["kernel_families"]
+    bb = json.loads(pathlib.Path(args.bench_json).read_text().splitlines()[-1])
+    # the default run times the families of its headline variant only
+    fam_def = bb.get("kernel_families", {}) if bb.get("config", {}).get("variant") == args.variant else {}
     rows = []
     for f, pat in FAMILY_KERNELS.items():
         if f in fam:
@@ -131,16 +134,20 @@ def report(args):
             tot = sum(t[2] for t in tab if re.search(pat, t[0]))
             if c:
                 shown = pat.replace("|", "\\|")  # a literal bar inside a markdown table cell
-                rows.append(f"| {f} | `{shown}` | {fam[f]['avg_ms']:.4f} | {tot / c / 1e6:.4f} | {c} |")
-    bb = json.loads(pathlib.Path(args.bench_json).read_text().splitlines()[-1])
+                dflt = f"{fam_def[f]['avg_ms']:.4f}" if f in fam_def else "-"
+                rows.append(f"| {f} | `{shown}` | {fam[f]['avg_ms']:.4f} | {dflt} | {tot / c / 1e6:.4f} | {c} |")
     steps = [f"| {k} | {v['ms_per_step']:.3f} | {v['mpts']:.0f} |" for k, v in bb["variants"].items() if "mpts" in v]
     r = bb["roofline"]
     out = [f"# {args.title}", "", "Commands (repository root, GPU box):", "", "```"] + args.command + ["```", "",
            args.workload, "", "## Top kernels (rocprofv3 --stats)", "", text, "",
-           "## HIP-event family averages (same traced process) vs rocprofv3", "",
-           "bench.py times every launch of its 5-step cycle with HIP events on the library's stream; "
-           "rocprofv3 averages all launches of the same kernels over the whole trace.", "",
-           "| family | kernels | bench avg ms | rocprof avg ms | rocprof launches |", "|---|---|---|---|---|"] + rows + [
+           "## HIP-event family averages vs rocprofv3", "",
+           "bench.py times every launch of its 5-step cycle with HIP events on the library's stream (one "
+           "stream, each launch bracketed and synchronised): in the traced process, where the profiler adds its "
+           "dispatch overhead to every bracket, and in the default run without the profiler.  rocprofv3 "
+           "averages all launches of the same kernels over the whole trace, including the timed steps, where "
+           "the SGS stage runs on a second stream beside the convection.", "",
+           "| family | kernels | bench avg ms (traced) | bench avg ms (default run) | rocprof avg ms | rocprof launches |",
+           "|---|---|---|---|---|---|"] + rows + [
            "", "## Default benchmark line (`python3 bench.py`, separate run)", "",
            "| variant | ms/step | Mpt-steps/s |", "|---|---|---|"] + steps + [
            "", f"Dominant single kernel: `{r['kernel_family']}` at {r['achieved']:.0f} GB/s algorithmic "
@@ -172,6 +179,7 @@ def main(argv=None):
     rp.add_argument("--workload", required=True)
     rp.add_argument("--command", action="append", default=[])
     rp.add_argument("--top", type=int, default=30)
+    rp.add_argument("--variant", default="learned", help="variant of the traced run")
     rp.add_argument("--out")
     a = ap.parse_args(argv)
     if a.cmd == "kernels":
