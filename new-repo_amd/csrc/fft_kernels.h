@@ -646,6 +646,36 @@ __global__ void __launch_bounds__(LES_BLOCK) k_rows_resample(RowResArgs<T, TI, T
       return;
     }
   }
+#ifndef LES_RES64_FIXED
+#define LES_RES64_FIXED 1
+#endif
+  if constexpr (sizeof(T) == 8 && LES_RES64_FIXED) {
+    // FP64 256 <-> 384: register twiddles with the XOR placement (line of n2)
+    if ((p1.n == 256 && p2.n == 384) || (p1.n == 384 && p2.n == 256)) {
+      if (p1.n == 256) {
+        TwReg<T, 256> ta;
+        TwReg<T, 384> tb;
+        tw_load(ta, tw + p1.twf, lane);
+        tw_load(tb, tw + p2.twf, lane);
+        rows_resample_body(a, z, lane, job, [&](Cplx<T>* q) {
+          wave_fft_t<T, 256, LES_LAY_XOR>(q, ta, false, lane);
+          wave_remap(q, 256, 384, lane);
+          wave_fft_t<T, 384, LES_LAY_XOR>(q, tb, true, lane);
+        });
+      } else {
+        TwReg<T, 384> ta;
+        TwReg<T, 256> tb;
+        tw_load(ta, tw + p1.twf, lane);
+        tw_load(tb, tw + p2.twf, lane);
+        rows_resample_body(a, z, lane, job, [&](Cplx<T>* q) {
+          wave_fft_t<T, 384, LES_LAY_XOR>(q, ta, false, lane);
+          wave_remap(q, 384, 256, lane);
+          wave_fft_t<T, 256, LES_LAY_XOR>(q, tb, true, lane);
+        });
+      }
+      return;
+    }
+  }
   rows_resample_body(a, z, lane, job, [&](Cplx<T>* q) {
     wave_fft(q, p1, tw, false, lane);
     wave_remap(q, a.n1, a.n2, lane);
