@@ -574,21 +574,22 @@ static int les_step_t(Les& L) {
   if (L.timing) hipEventRecord(L.ph_ev[1], L.stream);
   if (LES_OVERLAP && !L.timing && L.P.dealias == 0) {
     // The SGS stage and the padded convection read the same velocity and
-    // gradients and write disjoint buffers: the SGS stage runs on the second
-    // stream while the convection runs on the first; the stress divergence
-    // waits for both.  (The nx x ny convections share the spectral
-    // workspaces with the dynamic models and stay sequential, as does the
-    // per-kernel timing mode.)
+    // gradients and write disjoint buffers: the SGS stage and the spectra of
+    // the stress divergence run on the second stream while the convection
+    // runs on the first; the rhs update waits for both.  (The nx x ny
+    // convections share the spectral workspaces with the dynamic models and
+    // stay sequential, as does the per-kernel timing mode.)
     hipEventRecord(L.ev_fork, L.stream);
     hipStreamWaitEvent(L.stream2, L.ev_fork, 0);
     hipStream_t s1 = L.stream;
     L.stream = L.stream2;
     sgs_stress<T>(L, L.jt, jtt);
+    div_stress_spectra<T>(L);
     hipEventRecord(L.ev_join, L.stream2);
     L.stream = s1;
     rhs_convection<T>(L);
     hipStreamWaitEvent(L.stream, L.ev_join, 0);
-    div_stress_xyz<T>(L);
+    div_stress_apply<T>(L);
   } else {
     sgs_stress<T>(L, L.jt, jtt);
     if (L.timing) hipEventRecord(L.ph_ev[2], L.stream);

@@ -407,8 +407,10 @@ static void div_stress(Les& L, int rhs_id, int tx, int ty, int tz, int stag, int
 // transformed once (txy enters the x and y divergences), then one inverse
 // launch updates rhsx, rhsy, rhsz.  The arithmetic per component is that of
 // div_stress.
+// First half: the spectra of the three horizontal divergences (needs the
+// stresses only).
 template <typename T>
-static void div_stress_xyz(Les& L) {
+static void div_stress_spectra(Les& L) {
   Cplx<T>* sp[8];
   for (int i = 0; i < 8; ++i) sp[i] = (Cplx<T>*)L.wk_spec[i];
   static_assert(LES_NSPEC >= 8, "div_stress_xyz needs 8 spectral workspaces");
@@ -418,11 +420,24 @@ static void div_stress_xyz(Les& L) {
   cols<T>(L, COP_DIV, sp[0], sp[1], sp[5], nullptr, nullptr, 1, 1, 1, 1, L.nz + 1);
   cols<T>(L, COP_DIV, sp[1], sp[2], sp[6], nullptr, nullptr, 1, 1, 1, 1, L.nz + 1);
   cols<T>(L, COP_DIV, sp[3], sp[4], sp[7], nullptr, nullptr, 1, 1, 1, 1, L.nz + 1);
+}
+
+// Second half: inverse transforms with the rhs update as their store.
+template <typename T>
+static void div_stress_apply(Les& L) {
+  Cplx<T>* sp[8];
+  for (int i = 0; i < 8; ++i) sp[i] = (Cplx<T>*)L.wk_spec[i];
   DivEpi3<T> ep;
   const int rhs[3] = {LES_RHSX, LES_RHSY, LES_RHSZ}, tz[3] = {LES_TXZ, LES_TYZ, LES_TZZ};
   for (int f = 0; f < 3; ++f) ep.c[f] = div_epi_args<T>(L, rhs[f], tz[f], f < 2 ? 1 : 0, f == 2 ? 1 : 0, f < 2 ? f + 1 : 0);
   const Cplx<T>* cs[3] = {sp[5], sp[6], sp[7]};
   rows_c2r_div<T>(L, cs, ep, 3);
+}
+
+template <typename T>
+static void div_stress_xyz(Les& L) {
+  div_stress_spectra<T>(L);
+  div_stress_apply<T>(L);
 }
 
 // First part of compute_RHS_adv: rhs_f <- rhs (src/LES_engine.f90:517-522)
