@@ -286,10 +286,20 @@ static void grad_filt(Les& L, int fid, int dx_id, int dy_id) {
 static void nyquist_remove_fp64(Les& L);
 
 template <typename T>
+static void vel_grad_bc_ddz(Les& L);
+
+template <typename T>
 static void compute_vel_grad(Les& L) {
   grad_filt<T>(L, LES_U, LES_DUDX, LES_DUDY);
   grad_filt<T>(L, LES_V, LES_DVDX, LES_DVDY);
   grad_filt<T>(L, LES_W, LES_DWDX, LES_DWDY);
+  vel_grad_bc_ddz<T>(L);
+}
+
+// Second part of compute_vel_grad: the FP64 Nyquist removal of the mixed
+// mode, the boundary conditions and the vertical derivatives.
+template <typename T>
+static void vel_grad_bc_ddz(Les& L) {
   if (L.mixed) nyquist_remove_fp64(L);
   BcDdzArgs<T> a;
   a.u = D(L, LES_U); a.v = D(L, LES_V); a.w = D(L, LES_W);
@@ -569,8 +579,28 @@ static int les_step_t(Les& L) {
   L.jt += 1;
   const int jtt = L.jt_total;
   if (L.timing) hipEventRecord(L.ph_ev[0], L.stream);
-  wall_law<T>(L);
-  compute_vel_grad<T>(L);
+  if (LES_OVERLAP && !L.timing && L.mixed) {
+    // Mixed mode: the horizontal gradients leave u, v, w unchanged, so the
+    // wall law (first level only, its own workspaces) runs on the second
+    // stream beside them; the Nyquist removal and the vertical derivatives
+    // wait for it.  (In FP64 the gradient pass writes the filtered velocity
+    // the wall law reads, so the order stays.)
+    hipEventRecord(L.ev_fork, L.stream);
+    hipStreamWaitEvent(L.stream2, L.ev_fork, 0);
+    hipStream_t s1 = L.stream;
+    L.stream = L.stream2;
+    wall_law<T>(L);
+    hipEventRecord(L.ev_join, L.stream2);
+    L.stream = s1;
+    grad_filt<T>(L, LES_U, LES_DUDX, LES_DUDY);
+    grad_filt<T>(L, LES_V, LES_DVDX, LES_DVDY);
+    grad_filt<T>(L, LES_W, LES_DWDX, LES_DWDY);
+    hipStreamWaitEvent(L.stream, L.ev_join, 0);
+    vel_grad_bc_ddz<T>(L);
+  } else {
+    wall_law<T>(L);
+    compute_vel_grad<T>(L);
+  }
   if (L.timing) hipEventRecord(L.ph_ev[1], L.stream);
   if (LES_OVERLAP && !L.timing && L.P.dealias == 0) {
     // The SGS stage and the padded convection read the same velocity and
