@@ -32,7 +32,7 @@ extern "C" {
  * (src/parameters_IO.f90:240-298); replaces the namelist globals. */
 typedef struct les_params {
   int nx, ny, nz;
-  int sgs_model;   /* 0 cst_Smag, 1 dyn_Smag, 2 dyn_LagScaleDep (src/compute_sgs.f90:149-175) */
+  int sgs_model;   /* 0 cst_Smag, 1 dyn_Smag, 2 dyn_LagScaleDep, 3 dns (src/compute_sgs.f90:124-175) */
   int cs_count;    /* LASD update period (src/compute_LagScaleDep.f90:237) */
   int dyn_init;    /* first LASD step */
   int lagran_init; /* restart with F_* present (src/parameters_IO.f90:77) */
@@ -45,6 +45,7 @@ typedef struct les_params {
   int dealias;     /* convection dealiasing, the reference's compile-time DSLG
                       (src/Makefile:17-40): 0 PADD, 1 TRUNC, 2 FSMTHG, 3 NODLSG */
   double lx, ly, lz, dt, z0, pfx, pfy, cs0, coriol, ug, vg, ubc_u, ubc_v, ubc_w;
+  double re;       /* Reynolds number (physical_param Re), used by sgs_model dns */
   double learned_zmin; /* learned SGS: levels with z / delta < learned_zmin (the
                           near-wall band) keep the plain procedure on every update */
 } les_params;

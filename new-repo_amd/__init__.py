@@ -49,7 +49,7 @@ class _Params(ctypes.Structure):
                  "ubc_u_neu", "ubc_v_neu", "halo_mode", "coriolis", "precision", "judge_every", "dealias")] + \
                [(n, ctypes.c_double) for n in
                 ("lx", "ly", "lz", "dt", "z0", "pfx", "pfy", "cs0", "coriol", "ug", "vg",
-                 "ubc_u", "ubc_v", "ubc_w", "learned_zmin")]
+                 "ubc_u", "ubc_v", "ubc_w", "re", "learned_zmin")]
 
 
 _lib = None

@@ -376,7 +376,8 @@ static void learned_update(Les& L, int jtt);
 template <typename T>
 static void dynsmag_update(Les& L);
 
-// sgs_stress (src/compute_sgs.f90:105-247).  Model 0 cst_Smag, 1 dyn_Smag
+// sgs_stress (src/compute_sgs.f90:105-247).  Model 3 dns: the molecular
+// stress only (k_dns_stress).  Model 0 cst_Smag, 1 dyn_Smag
 // (updated when the local step count jt is a multiple of cs_count,
 // src/compute_DynSmag.f90:70-80), 2 dyn_LagScaleDep (on the global count jtt,
 // src/compute_LagScaleDep.f90:237); both dynamic models hold Cs^2 = 0.03
@@ -384,6 +385,12 @@ static void dynsmag_update(Les& L);
 template <typename T>
 static void sgs_stress(Les& L, int jt, int jtt) {
   SgsArgs<T> a = sgs_args<T>(L);
+  if (L.P.sgs_model == 3) {
+    kt_begin(L, KT_POINTWISE);
+    hipLaunchKernelGGL((k_dns_stress<T>), dim3(nblk(L.np * L.nz)), dim3(LES_BLOCK), 0, L.stream, a, (T)(2.0 / L.P.re));
+    kt_end(L, KT_POINTWISE, (double)L.np * L.nz * 15.0 * sizeof(T));
+    return;
+  }
   bool lasd = L.P.sgs_model == 2;
   bool dyns = L.P.sgs_model == 1;
   bool acc = lasd && jtt >= L.P.dyn_init;
@@ -717,7 +724,8 @@ int les_init(const les_params* p) {
     if (m != 1 || n % 4 || n < 8) { les_set_error("les_init: nx, ny must be 2^a 3^b, multiples of 4, >= 8"); return 10; }
   }
   if (P.nz < 3) { les_set_error("les_init: nz must be >= 3"); return 10; }
-  if (P.sgs_model < 0 || P.sgs_model > 2) { les_set_error("Invalid SGS model"); return 1; }
+  if (P.sgs_model < 0 || P.sgs_model > 3) { les_set_error("Invalid SGS model"); return 1; }
+  if (P.sgs_model == 3 && !(P.re > 0.0)) { les_set_error("sgs_model dns needs Re > 0"); return 1; }
   if (P.precision == 2 && P.sgs_model != 2) {
     les_set_error("the learned SGS surrogate replaces dyn_LagScaleDep only");
     return 1;

@@ -222,6 +222,22 @@ __global__ void k_sgs(SgsArgs<T> a) {
   if (a.accumulate) lag_accumulate(a, k, q);
 }
 
+// sgs_model dns (src/compute_sgs.f90:124-140): tau_ij = 2/Re S_ij on levels
+// 1..nz, each component on the nodes of its gradients; S_ij, nu_t and Cs^2
+// are not touched.
+template <typename T>
+__global__ void k_dns_stress(SgsArgs<T> a, T c) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.np * a.nz) return;
+  const long i = a.np + t;  // levels 1..nz
+  a.txx[i] = c * a.dudx[i];
+  a.txy[i] = c * (T)0.5 * (a.dudy[i] + a.dvdx[i]);
+  a.tyy[i] = c * a.dvdy[i];
+  a.tzz[i] = c * a.dwdz[i];
+  a.txz[i] = c * (T)0.5 * (a.dudz[i] + a.dwdx[i]);
+  a.tyz[i] = c * (T)0.5 * (a.dvdz[i] + a.dwdy[i]);
+}
+
 template <typename T>
 struct CsFillArgs {
   T* cs;

@@ -14,7 +14,7 @@ module les_gpu_abi
      integer(c_int) :: nx, ny, nz, sgs_model, cs_count, dyn_init, lagran_init, wall_model
      integer(c_int) :: ubc_u_neu, ubc_v_neu, halo_mode, coriolis, precision, judge_every, dealias
      real(c_double) :: lx, ly, lz, dt, z0, pfx, pfy, cs0, coriol, ug, vg, ubc_u, ubc_v, ubc_w
-     real(c_double) :: learned_zmin
+     real(c_double) :: re, learned_zmin
   end type les_params_c
 
   ! field ids (enum les_field)

@@ -93,6 +93,8 @@ contains
        cparams%sgs_model = 2
     else if (trim(sgs_model) == 'dyn_Smag') then
        cparams%sgs_model = 1
+    else if (trim(sgs_model) == 'dns') then
+       cparams%sgs_model = 3
     else if (trim(sgs_model) == 'cst_Smag') then
        cparams%sgs_model = 0
     else
@@ -125,6 +127,7 @@ contains
        error stop 1
     end select
     cparams%learned_zmin = learned_zmin
+    cparams%re = Re
     cparams%coriolis = 0; cparams%coriol = 0._rprec; cparams%ug = 0._rprec; cparams%vg = 0._rprec
     if (coriolis_flag) then
        cparams%coriolis = 1

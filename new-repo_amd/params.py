@@ -21,6 +21,7 @@ PI = 3.1415926535897932384626433
 SGS_CST_SMAG = 0
 SGS_DYN_SMAG = 1
 SGS_LASD = 2
+SGS_DNS = 3
 # convection dealiasing, the reference's compile-time DSLG choice (src/Makefile:17-40)
 DEALIAS = {"PADD": 0, "TRUNC": 1, "FSMTHG": 2, "NODLSG": 3}
 PREC_FP64 = 0
@@ -61,6 +62,7 @@ class LesParams:
     precision: int = PREC_FP64
     judge_every: int = 50       # learned SGS: plain LASD judge cadence
     learned_zmin: float = 4.0   # learned SGS: plain LASD on levels with z / delta below this
+    re: float = 1.0e5           # physical_param Re (the dns stress uses 2/Re)
 
     # ---- vectors shared with the CPU oracle (orc_init argument order)
     def int_vector(self) -> List[int]:
@@ -69,7 +71,7 @@ class LesParams:
 
     def real_vector(self) -> List[float]:
         return [self.lx, self.ly, self.lz, self.dt, self.z0, self.pfx, self.pfy, self.cs0, self.coriol,
-                self.ug, self.vg, self.ubc_u, self.ubc_v, self.ubc_w]
+                self.ug, self.vg, self.ubc_u, self.ubc_v, self.ubc_w, self.re]
 
     @property
     def points(self) -> int:
@@ -88,8 +90,8 @@ class LesParams:
                 raise ValueError(f"horizontal size {n}: the 3/2-padded transform length is limited to 1024")
         if self.nz < 3:
             raise ValueError("nz must be >= 3")
-        if self.sgs_model not in (SGS_CST_SMAG, SGS_DYN_SMAG, SGS_LASD):
-            raise ValueError("sgs_model must be cst_Smag, dyn_Smag or dyn_LagScaleDep")
+        if self.sgs_model not in (SGS_CST_SMAG, SGS_DYN_SMAG, SGS_LASD, SGS_DNS):
+            raise ValueError("sgs_model must be cst_Smag, dyn_Smag, dyn_LagScaleDep or dns")
         if self.precision == PREC_LEARNED and self.sgs_model != SGS_LASD:
             raise ValueError("the learned SGS surrogate replaces dyn_LagScaleDep only")
         if self.cs_count < 1:
@@ -150,6 +152,7 @@ def params_from_namelist(text: str, precision: int = PREC_FP64) -> LesParams:
     p.dt = dt * u_scale / zi if sim.get("dim_dt_flag", False) else dt
     p.z0 = float(phys.get("lbc_dz0", 1e-4)) / zi
     p.pfx, p.pfy = float(phys.get("pfx", 0.0)), float(phys.get("pfy", 0.0))
+    p.re = float(phys.get("re", p.re))
     if phys.get("coriolis_flag", False):
         p.coriolis = 1
         p.ug = float(phys.get("ug_dim", 0.0)) / u_scale
@@ -160,6 +163,8 @@ def params_from_namelist(text: str, precision: int = PREC_FP64) -> LesParams:
         p.sgs_model = SGS_LASD
     elif model == "dyn_Smag":
         p.sgs_model = SGS_DYN_SMAG
+    elif model == "dns":
+        p.sgs_model = SGS_DNS
     elif model == "cst_Smag":
         p.sgs_model = SGS_CST_SMAG
     else:

@@ -54,7 +54,7 @@ static const double KVONK = 0.4;
 
 struct Params {
   int nx, ny, nz;
-  int sgs_model;    // 0 constant Smagorinsky, 1 plane-averaged dynamic Smagorinsky, 2 Lagrangian scale-dependent
+  int sgs_model;    // 0 constant Smagorinsky, 1 plane-averaged dynamic Smagorinsky, 2 Lagrangian scale-dependent, 3 dns
   int cs_count, dyn_init, lagran_init;
   int wall_model;   // 0 wall_filter (2-delta filtered first level), 1 wall_inst
   int ubc_u_neu, ubc_v_neu;  // top BC type for u, v (1 Neumann, 0 Dirichlet)
@@ -63,6 +63,7 @@ struct Params {
   int dealias;      // convection: 0 PADD (3/2 rule), 1 TRUNC, 2 FSMTHG, 3 NODLSG
   double lx, ly, lz, dt, z0, pfx, pfy, cs0, coriol, ug, vg;
   double ubc_u, ubc_v, ubc_w;
+  double re;        // Reynolds number (sgs_model 3, dns)
 };
 
 struct Oracle {
@@ -649,6 +650,23 @@ static void dynsmag(Oracle& o, int jt, int jtt) {
 // sgs_stress (src/compute_sgs.f90:105-247)
 static void sgs_stress(Oracle& o, int jt, int jtt) {
   int nz = o.nz;
+  if (o.P.sgs_model == 3) {
+    // molecular stress 2/Re S_ij on levels 1..nz, every component on the
+    // gradients' own nodes (src/compute_sgs.f90:124-140); S_ij, nu_t, Cs^2
+    // and the ghost levels are left as they are
+    const double c = 2.0 / o.P.re;
+    for (int k = 1; k <= nz; ++k)
+      for (size_t q = 0; q < o.np; ++q) {
+        size_t a = o.I(0, 0, k) + q;
+        o.txx[a] = c * o.dudx[a];
+        o.txy[a] = c * 0.5 * (o.dudy[a] + o.dvdx[a]);
+        o.tyy[a] = c * o.dvdy[a];
+        o.tzz[a] = c * o.dwdz[a];
+        o.txz[a] = c * 0.5 * (o.dudz[a] + o.dwdx[a]);
+        o.tyz[a] = c * 0.5 * (o.dvdz[a] + o.dwdy[a]);
+      }
+    return;
+  }
   compute_sij(o);
   std::vector<double> l(nz + 2, o.delta);
   if (o.P.sgs_model == 0) {
@@ -1278,6 +1296,7 @@ int orc_init(const int* ip, const double* rp) {
   if (P.dealias < 0 || P.dealias > 3) return 1;
   P.lx = rp[0]; P.ly = rp[1]; P.lz = rp[2]; P.dt = rp[3]; P.z0 = rp[4]; P.pfx = rp[5]; P.pfy = rp[6];
   P.cs0 = rp[7]; P.coriol = rp[8]; P.ug = rp[9]; P.vg = rp[10]; P.ubc_u = rp[11]; P.ubc_v = rp[12]; P.ubc_w = rp[13];
+  P.re = rp[14];
   o.nx = P.nx; o.ny = P.ny; o.nz = P.nz; o.nzg = P.nz + 2; o.lhx = P.nx / 2 + 1;
   o.np = (size_t)o.nx * o.ny;
   o.nf = o.np * o.nzg;

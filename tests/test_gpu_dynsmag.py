@@ -1,5 +1,7 @@
-"""The plane-averaged dynamic Smagorinsky model (sgs_model = dyn_Smag,
-src/compute_DynSmag.f90:61-245) on the GPU against the CPU oracle.
+"""The sibling SGS options on the GPU against the CPU oracle: the
+plane-averaged dynamic Smagorinsky model (sgs_model = dyn_Smag,
+src/compute_DynSmag.f90:61-245) and the dns branch of sgs_stress
+(src/compute_sgs.f90:124-140).
 
 FP64: the pointwise Germano terms reuse the Lagrangian model's kernels
 (exact against the oracle) and the plane sums are formed in the reference's
@@ -115,3 +117,30 @@ def test_learned_precision_requires_lagrangian_model():
     prm.precision = P.PREC_LEARNED
     with pytest.raises(pkg.LesError):
         pkg.LesGpu(prm)
+
+
+def test_dns_branch_matches_oracle():
+    # sgs_model dns (src/compute_sgs.f90:124-140): molecular stress only
+    pkg, O, P = load_pkg(), load_oracle(), load_params()
+    ic = load_ic()
+    prm = P.LesParams(nx=32, ny=32, nz=16, sgs_model=P.SGS_DNS, re=2.0e3)
+    prm.validate()
+    u, v, w = ic.log_law_ic(32, 32, 16, prm.lz, prm.z0, 0.1, seed=13)
+    print("dns branch parity 32x32x16", flush=True)
+    o = O.Oracle(prm)
+    o.set_ic(u, v, w)
+    with pkg.LesGpu(prm) as g:
+        g.set_ic(u, v, w)
+        for n in (1, 5):
+            o.step(n)
+            g.step(n)
+            for f in FIELDS:
+                a, b = _levels(f, g.download(f), prm.nz), _levels(f, o.get(f), prm.nz)
+                e = rel_l2(a, b)
+                assert e <= 1e-11, (f, e)
+        # the operator alone, bit for bit
+        o.sgs_stress(7, 7)
+        g.sgs_stress(7, 7)
+        for f in ("txx", "txy", "txz", "tyy", "tyz", "tzz"):
+            assert np.array_equal(g.download(f)[1:prm.nz + 1], o.get(f)[1:prm.nz + 1]), f
+        assert np.all(g.download("cs_opt2") == 0) and not g.check_nan()

@@ -171,7 +171,7 @@ def test_init_rejects_bad_options():
     with pytest.raises(pkg.LesError) as e:
         pkg.LesGpu(bad)
     assert e.value.code == 10
-    bad = P.LesParams(nx=16, ny=16, nz=8, sgs_model=3)
+    bad = P.LesParams(nx=16, ny=16, nz=8, sgs_model=4)
     with pytest.raises(pkg.LesError) as e:
         pkg.LesGpu(bad)
     assert e.value.code == 1
