@@ -366,9 +366,11 @@ static SgsArgs<T> sgs_args(Les& L) {
   a.lvec = L.lvec;
   a.np = L.np; a.nz = L.nz;
   a.accumulate = 0; a.write_sij = 0;
+  a.acc_kmax = L.nz + 1;
   return a;
 }
 
+static int lag_levels(const Les& L, int jtt);
 template <typename T>
 static void lasd_update(Les& L, int jtt, int nlev);
 template <typename T>
@@ -396,6 +398,7 @@ static void sgs_stress(Les& L, int jt, int jtt) {
   bool acc = lasd && jtt >= L.P.dyn_init;
   bool update = lasd && jtt >= L.P.dyn_init && (jtt % L.P.cs_count) == 0;
   bool dyn_update = dyns && jtt >= L.P.dyn_init && (jt % L.P.cs_count) == 0;
+  if (acc) a.acc_kmax = lag_levels(L, jtt);
   if (L.P.sgs_model == 0) {
     CsFillArgs<T> c;
     c.cs = F<T>(L, LES_CS_OPT2); c.val = (T)(L.P.cs0 * L.P.cs0); c.np = L.np; c.nz = L.nz;

@@ -200,6 +200,24 @@ static double reduce_rel(Les& L, const void* a, const void* b, int ea, int eb, i
   return s2 > 0 ? std::sqrt(s1 / s2) : std::sqrt(s1);
 }
 
+// Levels 0..K of the u_lag sums the next coefficient update reads.  The
+// plain procedure reads all of them; the learned mode's non-judge updates
+// run it on the near-wall band 1..nband only, whose Lagrangian departure
+// points use u_lag on levels 0..max(nband, 2) (none without a band), so
+// the sums above are skipped unless the next update (the first update step
+// at or after jtt) is a judge update.  An update is a judge one when the
+// weights are not loaded, before the first plain update, or on a multiple
+// of judge_every; the first two can only turn off while a window runs, so
+// deciding at each step is safe.
+static int lag_levels(const Les& L, int jtt) {
+  const int all = L.nz + 1;
+  if (L.P.precision != 2 || !L.mlp || !L.mlp->ready || !L.F_coeff_init) return all;
+  const int cs = L.P.cs_count;
+  const int next = ((jtt + cs - 1) / cs) * cs;
+  if (L.P.judge_every > 0 && next % L.P.judge_every == 0) return all;
+  return L.nband > 0 ? (L.nband > 2 ? L.nband : 2) : -1;
+}
+
 // Coefficient update of the learned mode on a dynamic-procedure step.
 // The first update, judge steps and every step until weights are loaded run
 // the plain procedure on the whole column; the surrogate is evaluated beside

@@ -113,6 +113,7 @@ struct SgsArgs {
   long np;
   int nz;
   int accumulate, write_sij;
+  int acc_kmax;  // accumulate u_lag on levels 0..acc_kmax only
 };
 
 // compute_sij (src/compute_sgs.f90:260-335): first level on uvp nodes with
@@ -158,6 +159,7 @@ template <typename T>
 __device__ __forceinline__ void lag_accumulate(const SgsArgs<T>& a, int k, long q) {
   // u_lag += u over k = 0..nz+1 (src/compute_LagScaleDep.f90:220-233)
   const long c = (long)k * a.np + q;
+  if (k > a.acc_kmax) return;
   a.ulag[c] = (T)((double)a.ulag[c] + a.u[c]);
   a.vlag[c] = (T)((double)a.vlag[c] + a.v[c]);
   a.wlag[c] = (T)((double)a.wlag[c] + a.w[c]);
