@@ -11,9 +11,9 @@ traffic FETCH WRITE    HBM bytes per launch of each kernel family from two
                        of the same dispatches (bench.py --steps 0 output)
 
 Usage:
-  python new-repo_amd/profile_summary.py kernels gpurun_out/trace
-  python new-repo_amd/profile_summary.py traffic gpurun_out/pmc_fetch gpurun_out/pmc_write \\
-      --bench-json gpurun_out/pmc_fetch.json --variant learned --copy-mib 256 \\
+  python new-repo_amd/profile_summary.py kernels out/trace
+  python new-repo_amd/profile_summary.py traffic out/pmc_fetch out/pmc_write \\
+      --bench-json out/pmc_fetch.json --variant learned --copy-mib 256 \\
       --command "..." --out profiles/pmc_traffic.json
 """
 from __future__ import annotations
@@ -138,7 +138,7 @@ def report(args):
                 rows.append(f"| {f} | `{shown}` | {fam[f]['avg_ms']:.4f} | {dflt} | {tot / c / 1e6:.4f} | {c} |")
     steps = [f"| {k} | {v['ms_per_step']:.3f} | {v['mpts']:.0f} |" for k, v in bb["variants"].items() if "mpts" in v]
     r = bb["roofline"]
-    out = [f"# {args.title}", "", "Commands (repository root, GPU box):", "", "```"] + args.command + ["```", "",
+    out = [f"# {args.title}", "", "Commands (repository root, on the MI355X):", "", "```"] + args.command + ["```", "",
            args.workload, "", "## Top kernels (rocprofv3 --stats)", "", text, "",
            "## HIP-event family averages vs rocprofv3", "",
            "bench.py times every launch of its 5-step cycle with HIP events on the library's stream (one "
