@@ -91,12 +91,14 @@ def load_traffic(variant, family, grid):
     return None
 
 
-def run_variant(pkg, P, ic, precision, steps, warmup, nx, ny, nz, kernel_stats=True, seed=1234):
+def run_variant(pkg, P, ic, precision, steps, warmup, nx, ny, nz, kernel_stats=True, seed=1234, serial=False):
     prm = P.LesParams(nx=nx, ny=ny, nz=nz, dt=DT, precision=precision, judge_every=50)
     prm.validate()
     u, v, w = ic.log_law_ic(nx, ny, nz, prm.lz, prm.z0, 0.10, seed=seed)
     res = {}
     with pkg.LesGpu(prm) as g:
+        if serial:
+            g.set_overlap(False)
         g.set_ic(u, v, w)
         if precision == P.PREC_LEARNED:
             g.mlp_set_weights(load_weights())
@@ -218,6 +220,8 @@ def main(argv=None):
     ap.add_argument("--profile-steps", type=int, default=500,
                     help="steps of the matched-error profile comparison (0 to skip)")
     ap.add_argument("--no-kernel-stats", action="store_true", help="skip the per-launch HIP-event pass")
+    ap.add_argument("--serial", action="store_true",
+                    help="issue each step on one stream (default: the SGS stage on a second stream)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first launch a copy kernel of MIB MiB (counter calibration)")
     args = ap.parse_args(argv)
@@ -242,7 +246,7 @@ def main(argv=None):
     for name in wanted:
         try:
             results[name] = run_variant(pkg, P, ic, order[name], args.steps, args.warmup, nx, ny, nz,
-                                        kernel_stats=not args.no_kernel_stats)
+                                        kernel_stats=not args.no_kernel_stats, serial=args.serial)
         except Exception as e:  # a variant that fails is reported, not hidden
             results[name] = {"error": str(e)}
         print(f"# {name}: {results[name].get('mpts', results[name].get('error'))}", file=sys.stderr, flush=True)
@@ -294,7 +298,8 @@ def main(argv=None):
         "dtype": {"fp64": "f64", "mixed": "f32+f64", "learned": "f32+f64+bf16"}[head],
         "data": "synthetic: seeded log-law initial field (u* = 1, z0 = 1e-4, +-10% perturbation in z <= 0.5)",
         "config": {"workload": f"LES time step, {head}, LASD cs_count=5, dt={DT}", "grid": [nx, ny, nz], "variant": head,
-                   "parallelism": "single GPU, whole domain resident"},
+                   "parallelism": "single GPU, whole domain resident",
+                   "schedule": "one stream" if args.serial else "two streams (SGS stage beside the convection)"},
         "roofline": roof,
         "step_roofline": {"bytes_per_point_step": STEP_BYTES[step_key],
                           "achieved_GBs": step_bytes / (hr["ms_per_step"] * 1e-3) / 1e9,

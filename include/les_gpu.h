@@ -108,6 +108,13 @@ int les_timers(double* t, int reset);
  * t[7 + 3f] algorithmic bytes (inputs read once + outputs written once).
  * t must hold 32 doubles.  Timing synchronises after every timed launch. */
 int les_set_timing(int on);
+/* Schedule of les_step (no counterpart in the reference, whose step is one
+ * sequence): 1 (default) runs the SGS stage and the spectra of the stress
+ * divergence on a second stream beside the padded convection, and in mixed
+ * precision the wall law beside the horizontal gradients; 0 issues the whole
+ * step on one stream.  Results are the same either way.  The per-kernel
+ * timing mode always uses one stream. */
+int les_set_overlap(int on);
 
 /* --- operator-level entry points (the compute_* module procedures) -------- */
 /* obukhov + wallstress (src/compute_wall_law.f90:107-254). */

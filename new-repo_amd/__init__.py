@@ -78,7 +78,7 @@ def library() -> ctypes.CDLL:
             "les_upload_layer": [ctypes.c_int, dp], "les_download_layer": [ctypes.c_int, dp],
             "les_field_ptr": [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), ip],
             "les_step": [ctypes.c_int], "les_sync": [], "les_jt_total": [], "les_set_jt_total": [ctypes.c_int],
-            "les_check_nan": [ip], "les_timers": [dp, ctypes.c_int], "les_set_timing": [ctypes.c_int],
+            "les_check_nan": [ip], "les_timers": [dp, ctypes.c_int], "les_set_timing": [ctypes.c_int], "les_set_overlap": [ctypes.c_int],
             "les_wall_law": [], "les_compute_vel_grad": [],
             "les_grad_velocity_filt": [ctypes.c_int] * 3, "les_ddz_uv": [ctypes.c_int] * 2,
             "les_ddz_w": [ctypes.c_int] * 2, "les_sgs_stress": [ctypes.c_int] * 2,
@@ -194,6 +194,10 @@ class LesGpu:
 
     def set_timing(self, on: bool) -> None:
         check(self.lib.les_set_timing(int(on)))
+
+    def set_overlap(self, on: bool) -> None:
+        """les_step on two streams (default) or one."""
+        check(self.lib.les_set_overlap(int(on)))
 
     def timers(self, reset: bool = False) -> np.ndarray:
         t = np.zeros(5 + 3 * 9)

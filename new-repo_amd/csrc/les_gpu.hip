@@ -589,7 +589,7 @@ static int les_step_t(Les& L) {
   L.jt += 1;
   const int jtt = L.jt_total;
   if (L.timing) hipEventRecord(L.ph_ev[0], L.stream);
-  if (LES_OVERLAP && !L.timing && L.mixed) {
+  if (L.overlap && !L.timing && L.mixed) {
     // Mixed mode: the horizontal gradients leave u, v, w unchanged, so the
     // wall law (first level only, its own workspaces) runs on the second
     // stream beside them; the Nyquist removal and the vertical derivatives
@@ -612,7 +612,7 @@ static int les_step_t(Les& L) {
     compute_vel_grad<T>(L);
   }
   if (L.timing) hipEventRecord(L.ph_ev[1], L.stream);
-  if (LES_OVERLAP && !L.timing && L.P.dealias == 0) {
+  if (L.overlap && !L.timing && L.P.dealias == 0) {
     // The SGS stage and the padded convection read the same velocity and
     // gradients and write disjoint buffers: the SGS stage and the spectra of
     // the stress divergence run on the second stream while the convection
@@ -756,6 +756,7 @@ int les_init(const les_params* p) {
   }
   L->jt_total = 0;
   L->jt = 0;
+  L->overlap = LES_OVERLAP;
   L->F_coeff_init = P.lagran_init != 0;
   L->mixed = P.precision != 0;
   if (hipStreamCreate(&L->stream) != hipSuccess) { delete L; les_set_error("stream"); return 5; }
@@ -1093,6 +1094,12 @@ int les_timers(double* t, int reset) {
 int les_set_timing(int on) {
   if (!g_les) return 1;
   g_les->timing = on;
+  return 0;
+}
+
+int les_set_overlap(int on) {
+  if (!g_les) return 1;
+  g_les->overlap = on ? 1 : 0;
   return 0;
 }
 

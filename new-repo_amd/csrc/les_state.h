@@ -26,6 +26,7 @@ struct Les {
   hipStream_t stream;
   hipStream_t stream2;   // second stream of the step: the SGS stage beside the convection
   hipEvent_t ev_fork, ev_join;
+  int overlap;           // les_step on two streams (les_set_overlap)
   int mixed;             // flow kernels in FP32
 
   // host-built FFT plans and twiddles

@@ -205,3 +205,22 @@ def test_full_size_invariants():
         cs, beta = g.download("cs_opt2")[1:nz + 2], g.download("beta")[1:nz + 2]
         assert np.array_equal(cs.astype(np.float32).astype(np.float64), cs) and cs.min() >= np.float32(1e-24)
         assert beta.min() >= 0.125 and np.all(beta[-1] == 1.0)
+
+
+@pytest.mark.parametrize("precision", [0, 2])
+def test_two_stream_schedule_gives_the_same_state(precision):
+    # les_step on two streams (default) and on one stream perform the same
+    # operations on the same data; the states must be equal bit for bit
+    pkg, P = load_pkg(), load_params()
+    prm, u, v, w = _setup(32, 32, 16, seed=21)
+    prm.precision = precision
+    print(f"schedule equality, precision {precision}", flush=True)
+    states = []
+    for overlap in (True, False):
+        with pkg.LesGpu(prm) as g:
+            g.set_overlap(overlap)
+            g.set_ic(u, v, w)
+            g.step(11)
+            states.append({n: g.download(n) for n in ("u", "v", "w", "p", "rhsx", "txz", "cs_opt2", "F_LM")})
+    for n in states[0]:
+        assert np.array_equal(states[0][n], states[1][n]), n
