@@ -161,3 +161,58 @@ def test_band_levels_follow_the_plain_procedure():
     cs_l, u_l, _ = _run_cs(P.PREC_LEARNED, 10, _some_weights(), learned_zmin=4.0)
     assert np.array_equal(cs_m[1:nband + 1], cs_l[1:nband + 1])
     assert not np.array_equal(cs_m[nband + 1:18], cs_l[nband + 1:18])
+
+
+def test_surrogate_update_is_the_network_on_the_step_features():
+    # A surrogate coefficient update (features and inference in one kernel)
+    # against the CPU forward pass of the network (bf16 roundings where the
+    # device rounds) on the features les_sgs_samples reports for the same
+    # state: in-range points get the network's Cs^2 and beta, out-of-range
+    # points keep their previous coefficients.
+    torch = pytest.importorskip("torch")
+    pkg, P, ic = load_pkg(), load_params(), load_ic()
+    prm = P.LesParams(nx=32, ny=32, nz=16, precision=P.PREC_LEARNED, judge_every=1000)
+    u, v, w = ic.log_law_ic(prm.nx, prm.ny, prm.nz, prm.lz, prm.z0, 0.1, seed=21)
+    np_, nz = prm.nx * prm.ny, prm.nz
+    print("surrogate update vs CPU forward pass", flush=True)
+    with pkg.LesGpu(prm) as g:
+        g.set_ic(u, v, w)
+        g.step(prm.cs_count)  # first update: the plain procedure on every level
+        npts = np_ * (nz + 1)
+        X = torch.zeros((npts, 12), dtype=torch.float32, device="cuda")
+        Y = torch.zeros((npts, 2), dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+        n0 = g.sgs_samples(X.data_ptr(), Y.data_ptr(), npts, 1)
+        F0 = X[:n0].cpu().numpy()
+        xm, xs = F0.mean(0).astype(np.float32), (F0.std(0) + 1e-3).astype(np.float32)
+        xlim = np.float32(2.5)
+        ym, ys = np.array([0.02, 1.0], np.float32), np.array([0.01, 0.2], np.float32)
+        params = M.init_params(7).astype(np.float32)
+        g.mlp_set_weights(np.concatenate([params, xm, xs, ym, ys, [xlim]]).astype(np.float32))
+        g.step(prm.cs_count - 1)
+        cs0, be0 = g.download("cs_opt2"), g.download("beta")
+        g.step(1)  # a surrogate update: plain procedure on the band, the network above
+        cs1, be1 = g.download("cs_opt2"), g.download("beta")
+        torch.cuda.synchronize()
+        n1 = g.sgs_samples(X.data_ptr(), Y.data_ptr(), npts, 1)
+        F1 = X[:n1].cpu().numpy()
+    p0 = npts - n1  # first sampled point: level nband + 1
+    assert p0 % np_ == 0 and 0 < p0 // np_ < nz
+    Xn = (F1 - xm) / xs  # float32, as on the device
+    out = ~np.all(np.abs(Xn) <= xlim, axis=1)
+    frac = out.mean()
+    print(f"  band of {p0 // np_} levels, {n1} surrogate points, {frac:.3f} out of range", flush=True)
+    assert 0.001 < frac < 0.6
+    _, _, yn = M.loss_and_grad(params.astype(np.float64), Xn.astype(np.float64), np.zeros((n1, 2)), "bf16")
+    cs_ref = np.maximum(yn[:, 0] * ys[0] + ym[0], 1e-24)
+    be_ref = np.maximum(yn[:, 1] * ys[1] + ym[1], 0.125)
+    lev = 1 + (p0 + np.arange(n1)) // np_
+    be_ref[lev == nz + 1] = 1.0
+    sl = slice(np_ + p0, np_ + p0 + n1)
+    c1, b1 = cs1.ravel()[sl], be1.ravel()[sl]
+    c0, b0 = cs0.ravel()[sl], be0.ravel()[sl]
+    assert np.array_equal(c1[out], c0[out]) and np.array_equal(b1[out], b0[out])
+    ec, eb = _rel(c1[~out], cs_ref[~out]), _rel(b1[~out], be_ref[~out])
+    print(f"  in range: Cs^2 rel L2 {ec:.2e}, beta rel L2 {eb:.2e}", flush=True)
+    assert ec < 3e-3 and eb < 3e-3
+    assert not np.array_equal(c1[~out], c0[~out])
