@@ -8,8 +8,10 @@
  * an int status that mirrors the reference's decomp_2d_abort codes:
  *   0 ok, 1 invalid option/model, 3 out of memory, 4 double initialisation,
  *   5 HIP runtime error, 9 NaN found / singular solve, 10 bad shape or id.
- * Calls come from one host thread; all kernels run on one HIP stream; host
- * buffers are synchronised only by download/status calls.  Not reentrant,
+ * Calls come from one host thread; every entry point runs its kernels on the
+ * library's HIP stream (les_step also uses a second one inside a step, joined
+ * before it returns, see les_set_overlap); host buffers are synchronised only
+ * by download/status calls.  Not reentrant,
  * like the reference's module state.
  *
  * Sizes: nx, ny multiples of 4 of the form 2^a 3^b with 3 nx / 2, 3 ny / 2 <= 1024

@@ -2,7 +2,9 @@
 // driver and the C-ABI declared in include/les_gpu.h.
 //
 // One LES time step of the reference (src/LES_engine.f90:189-353) runs as a
-// sequence of kernels on one HIP stream; the whole domain is resident in HBM.
+// sequence of kernels on the library's HIP stream (les_step puts the SGS stage
+// on a second stream beside the convection); the whole domain is resident in
+// HBM.
 // The 2DECOMP pencil transposes and MPI halo exchanges of the reference are
 // replaced by strided column passes and by index arithmetic inside kernels.
 #include <hip/hip_runtime.h>
