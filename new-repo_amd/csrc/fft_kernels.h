@@ -621,26 +621,32 @@ __global__ void __launch_bounds__(LES_BLOCK) k_rows_resample(RowResArgs<T, TI, T
   const long job = (long)blockIdx.x * (blockDim.x / LES_WAVE) + wave;
   if (job >= (long)a.nplanes * (a.ny / 2)) return;
   if constexpr (sizeof(T) == 4) {
+    // each transform's twiddles are loaded where it starts, so the two sets
+    // are never live together (fewer VGPRs, more waves per SIMD)
     if (res_fixed<T>(p1.n, p2.n) && p1.n == 256) {
-      TwReg<T, 256> ta;
-      TwReg<T, 384> tb;
-      tw_load(ta, tw + p1.twf, lane);
-      tw_load(tb, tw + p2.twf, lane);
       rows_resample_body(a, z, lane, job, [&](Cplx<T>* q) {
-        wave_fft_t<T, 256, LES_LAY_PAD>(q, ta, false, lane);
+        {
+          TwReg<T, 256> ta;
+          tw_load(ta, tw + p1.twf, lane);
+          wave_fft_t<T, 256, LES_LAY_PAD>(q, ta, false, lane);
+        }
         wave_remap(q, 256, 384, lane);
+        TwReg<T, 384> tb;
+        tw_load(tb, tw + p2.twf, lane);
         wave_fft_t<T, 384, LES_LAY_PAD>(q, tb, true, lane);
       });
       return;
     }
     if (res_fixed<T>(p1.n, p2.n) && p1.n == 384) {
-      TwReg<T, 384> ta;
-      TwReg<T, 256> tb;
-      tw_load(ta, tw + p1.twf, lane);
-      tw_load(tb, tw + p2.twf, lane);
       rows_resample_body(a, z, lane, job, [&](Cplx<T>* q) {
-        wave_fft_t<T, 384, LES_LAY_PAD>(q, ta, false, lane);
+        {
+          TwReg<T, 384> ta;
+          tw_load(ta, tw + p1.twf, lane);
+          wave_fft_t<T, 384, LES_LAY_PAD>(q, ta, false, lane);
+        }
         wave_remap(q, 384, 256, lane);
+        TwReg<T, 256> tb;
+        tw_load(tb, tw + p2.twf, lane);
         wave_fft_t<T, 256, LES_LAY_PAD>(q, tb, true, lane);
       });
       return;
@@ -653,23 +659,27 @@ __global__ void __launch_bounds__(LES_BLOCK) k_rows_resample(RowResArgs<T, TI, T
     // FP64 256 <-> 384: register twiddles with the XOR placement (line of n2)
     if ((p1.n == 256 && p2.n == 384) || (p1.n == 384 && p2.n == 256)) {
       if (p1.n == 256) {
-        TwReg<T, 256> ta;
-        TwReg<T, 384> tb;
-        tw_load(ta, tw + p1.twf, lane);
-        tw_load(tb, tw + p2.twf, lane);
         rows_resample_body(a, z, lane, job, [&](Cplx<T>* q) {
-          wave_fft_t<T, 256, LES_LAY_XOR>(q, ta, false, lane);
+          {
+            TwReg<T, 256> ta;
+            tw_load(ta, tw + p1.twf, lane);
+            wave_fft_t<T, 256, LES_LAY_XOR>(q, ta, false, lane);
+          }
           wave_remap(q, 256, 384, lane);
+          TwReg<T, 384> tb;
+          tw_load(tb, tw + p2.twf, lane);
           wave_fft_t<T, 384, LES_LAY_XOR>(q, tb, true, lane);
         });
       } else {
-        TwReg<T, 384> ta;
-        TwReg<T, 256> tb;
-        tw_load(ta, tw + p1.twf, lane);
-        tw_load(tb, tw + p2.twf, lane);
         rows_resample_body(a, z, lane, job, [&](Cplx<T>* q) {
-          wave_fft_t<T, 384, LES_LAY_XOR>(q, ta, false, lane);
+          {
+            TwReg<T, 384> ta;
+            tw_load(ta, tw + p1.twf, lane);
+            wave_fft_t<T, 384, LES_LAY_XOR>(q, ta, false, lane);
+          }
           wave_remap(q, 384, 256, lane);
+          TwReg<T, 256> tb;
+          tw_load(tb, tw + p2.twf, lane);
           wave_fft_t<T, 256, LES_LAY_XOR>(q, tb, true, lane);
         });
       }
