@@ -794,8 +794,11 @@ __device__ __forceinline__ void c24_pair_b(Cplx<T>* X, const Cplx<T>* __restrict
 }
 
 // PAD: 256 -> 384 rows (forward 256, pad, inverse 384); else 384 -> 256.
+// FP32: held to 96 VGPRs (5 waves per SIMD instead of 4; two VGPRs spill
+// in the padding direction).
 template <typename T, bool PAD>
-__global__ void __launch_bounds__(384) k_colres_c16(ColResArgs<T> a, const Cplx<T>* __restrict__ tw256,
+__global__ void __launch_bounds__(384) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 5 : 1)))
+k_colres_c16(ColResArgs<T> a, const Cplx<T>* __restrict__ tw256,
                                                     const Cplx<T>* __restrict__ tw384) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Cplx<T>* ex = reinterpret_cast<Cplx<T>*>(smem_raw);
