@@ -87,8 +87,10 @@ __device__ __forceinline__ void rows_r2c_body(const RowR2CArgs<TI, T>& a, Cplx<T
   }
 }
 
+// FP32: held to 64 VGPRs (8 waves per SIMD instead of 7; two VGPRs spill).
 template <typename TI, typename T>
-__global__ void __launch_bounds__(LES_BLOCK) k_rows_r2c(RowR2CArgs<TI, T> a, FftPlan pl, const Cplx<T>* __restrict__ tw) {
+__global__ void __launch_bounds__(LES_BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 8 : 1)))
+k_rows_r2c(RowR2CArgs<TI, T> a, FftPlan pl, const Cplx<T>* __restrict__ tw) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int lane = threadIdx.x & (LES_WAVE - 1), wave = threadIdx.x / LES_WAVE;
   Cplx<T>* z = reinterpret_cast<Cplx<T>*>(smem_raw) + wave * a.ls;
@@ -147,8 +149,10 @@ __device__ __forceinline__ void rows_c2r_body(const RowC2RArgs<T, TO>& a, Cplx<T
   }
 }
 
+// FP32: held to 64 VGPRs, as the r2c kernel.
 template <typename T, typename TO>
-__global__ void __launch_bounds__(LES_BLOCK) k_rows_c2r(RowC2RArgs<T, TO> a, FftPlan pl, const Cplx<T>* __restrict__ tw) {
+__global__ void __launch_bounds__(LES_BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 8 : 1)))
+k_rows_c2r(RowC2RArgs<T, TO> a, FftPlan pl, const Cplx<T>* __restrict__ tw) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int lane = threadIdx.x & (LES_WAVE - 1), wave = threadIdx.x / LES_WAVE;
   Cplx<T>* z = reinterpret_cast<Cplx<T>*>(smem_raw) + wave * a.ls;

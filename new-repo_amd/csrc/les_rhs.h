@@ -326,9 +326,10 @@ struct DivEpi3 {
   DivEpiArgs<T> c[3];  // per component (blockIdx.y)
 };
 
+// FP32: held to 80 VGPRs (6 waves per SIMD instead of 5, no spills).
 template <typename T>
-__global__ void __launch_bounds__(LES_BLOCK) k_rows_c2r_div(RowC2RArgs<T, T> a, DivEpi3<T> ep, FftPlan pl,
-                                                            const Cplx<T>* __restrict__ tw) {
+__global__ void __launch_bounds__(LES_BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 1)))
+k_rows_c2r_div(RowC2RArgs<T, T> a, DivEpi3<T> ep, FftPlan pl, const Cplx<T>* __restrict__ tw) {
   const DivEpiArgs<T>& e = ep.c[blockIdx.y];
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int lane = threadIdx.x & (LES_WAVE - 1), wave = threadIdx.x / LES_WAVE;
