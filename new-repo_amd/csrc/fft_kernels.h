@@ -615,10 +615,11 @@ __device__ __forceinline__ void rows_resample_body(const RowResArgs<T, TI, TO>& 
   }
 }
 
-// The FP32 256 <-> 384 pair runs on register twiddles (37 per lane).
+// The FP32 256 <-> 384 pair runs on register twiddles (37 per lane); FP32
+// instances are held to 80 VGPRs (6 waves per SIMD, no spills).
 template <typename T, typename TI, typename TO>
-__global__ void __launch_bounds__(LES_BLOCK) k_rows_resample(RowResArgs<T, TI, TO> a, FftPlan p1, FftPlan p2,
-                                                             const Cplx<T>* __restrict__ tw) {
+__global__ void __launch_bounds__(LES_BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 1)))
+k_rows_resample(RowResArgs<T, TI, TO> a, FftPlan p1, FftPlan p2, const Cplx<T>* __restrict__ tw) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int lane = threadIdx.x & (LES_WAVE - 1), wave = threadIdx.x / LES_WAVE;
   Cplx<T>* z = reinterpret_cast<Cplx<T>*>(smem_raw) + wave * a.ls;
