@@ -463,10 +463,13 @@ struct NyqArgs {
   int nx, ny, nzg;
 };
 
+// LDS: the four waves' column partials, 4 nx doubles (dynamic, sized at the
+// launch so the block holds only what nx needs).
 template <int RPW>
 __global__ void __launch_bounds__(LES_BLOCK) k_nyq_reduce(NyqArgs a) {
-  __shared__ double part[LES_BLOCK / LES_WAVE][LES_MAX_N];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
   const int nx = a.nx, ny = a.ny, fld = blockIdx.y;
+  double* part0 = reinterpret_cast<double*>(smem_raw);
   const int lane = threadIdx.x & (LES_WAVE - 1), wave = threadIdx.x / LES_WAVE;
   const long row0 = ((long)blockIdx.x * (LES_BLOCK / LES_WAVE) + wave) * RPW;  // global row k * ny + j
   const double* __restrict__ fr = a.f[fld] + row0 * nx;
@@ -486,7 +489,7 @@ __global__ void __launch_bounds__(LES_BLOCK) k_nyq_reduce(NyqArgs a) {
       s[r] += (i & 1) ? -v[r] : v[r];
       if (r > 0) acc += sj[r] * v[r];
     }
-    part[wave][i] = acc;
+    part0[wave * nx + i] = acc;
   }
 #pragma unroll
   for (int r = 0; r < RPW; ++r) {
@@ -496,7 +499,7 @@ __global__ void __launch_bounds__(LES_BLOCK) k_nyq_reduce(NyqArgs a) {
   __syncthreads();
   double* __restrict__ P = a.P + (long)fld * (ny / 4) * nx * a.nzg;
   for (int i = threadIdx.x; i < nx; i += blockDim.x)
-    P[(long)blockIdx.x * nx + i] = ((part[0][i] + part[1][i]) + part[2][i]) + part[3][i];
+    P[(long)blockIdx.x * nx + i] = ((part0[i] + part0[nx + i]) + part0[2 * nx + i]) + part0[3 * nx + i];
 }
 
 // Pass 2, one block per plane and field: B_i = sum of the nb band partials /
@@ -561,8 +564,9 @@ static void nyquist_remove_fp64(Les& L) {
   const int rpw = L.ny % 16 == 0 ? 4 : 1;
   const unsigned nblocks = (unsigned)(nrows / (4 * rpw));
   kt_begin(L, KT_POINTWISE);
-  if (rpw == 4) hipLaunchKernelGGL(k_nyq_reduce<4>, dim3(nblocks, 3), dim3(LES_BLOCK), 0, L.stream, a);
-  else hipLaunchKernelGGL(k_nyq_reduce<1>, dim3(nblocks, 3), dim3(LES_BLOCK), 0, L.stream, a);
+  const size_t shp = sizeof(double) * (LES_BLOCK / LES_WAVE) * (size_t)L.nx;
+  if (rpw == 4) hipLaunchKernelGGL(k_nyq_reduce<4>, dim3(nblocks, 3), dim3(LES_BLOCK), shp, L.stream, a);
+  else hipLaunchKernelGGL(k_nyq_reduce<1>, dim3(nblocks, 3), dim3(LES_BLOCK), shp, L.stream, a);
   hipLaunchKernelGGL(k_nyq_bc, dim3((unsigned)L.nzg, 3), dim3(LES_BLOCK), 0, L.stream, a, L.ny / (4 * rpw));
   if (rpw == 4) hipLaunchKernelGGL(k_nyq_apply<4>, dim3(nblocks, 3), dim3(LES_BLOCK), 0, L.stream, a);
   else hipLaunchKernelGGL(k_nyq_apply<1>, dim3(nblocks, 3), dim3(LES_BLOCK), 0, L.stream, a);
