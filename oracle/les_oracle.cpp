@@ -46,6 +46,18 @@
 
 #include "fftcore.h"
 
+// Planted mistakes, for the oracle's own pinning tests only.  Every library
+// that the tests, the smoke check and the benchmark load is built with
+// ORC_PLANT = 0, where each PLANT(n) below is the constant false and its
+// branch is compiled out.  tests/test_oracle_pins.py builds this file again
+// with -DORC_PLANT=n, one library per n, and checks that at least one of the
+// pinning tests named for n fails on that build: each n is a plausible
+// transcription mistake (a wrong exponent, constant, sign, index or order).
+#ifndef ORC_PLANT
+#define ORC_PLANT 0
+#endif
+#define PLANT(n) (ORC_PLANT == (n))
+
 namespace orc {
 
 static const double BOGUS = -1.23456789e20;
@@ -232,7 +244,7 @@ static void filter_plane(const Oracle& o, const double* in, double* out, const s
 static void wall_law(Oracle& o) {
   size_t np = o.np;
   std::vector<double> u1(np), v1(np);
-  if (o.P.wall_model == 0) {
+  if (o.P.wall_model == 0 || PLANT(35)) {
     filter_plane(o, &o.u[o.I(0, 0, 1)], u1.data(), o.G2);
     filter_plane(o, &o.v[o.I(0, 0, 1)], v1.data(), o.G2);
   } else {
@@ -403,7 +415,7 @@ static void dynamic_mum(Oracle& o, const std::vector<double>& Smod, std::vector<
       size_t a = o.I(0, 0, k) + q;
       double uf, vf, wf;
       if (k == 1) {
-        uf = o.u[a]; vf = o.v[a]; wf = 0.25 * o.w[o.I(0, 0, 2) + q];
+        uf = o.u[a]; vf = o.v[a]; wf = (PLANT(13) ? 0.5 : 0.25) * o.w[o.I(0, 0, 2) + q];
       } else {
         size_t b = o.I(0, 0, k - 1) + q;
         uf = 0.5 * (o.u[a] + o.u[b]);
@@ -421,10 +433,10 @@ static void dynamic_mum(Oracle& o, const std::vector<double>& Smod, std::vector<
     }
   std::vector<std::vector<Cx>> spec(21);
   for (int f = 0; f < 21; ++f) fwd_spec(o, in[f], spec[f]);
-  double cst = 2.0 * (o.delta * o.delta);
+  double cst = (PLANT(12) ? 1.0 : 2.0) * (o.delta * o.delta);
   for (int sc = 0; sc < nscale; ++sc) {
-    const std::vector<double>& G = sc == 0 ? o.G2 : o.G4;
-    double tf2 = sc == 0 ? 4.0 : 16.0;
+    const std::vector<double>& G = (sc == 0 || PLANT(11)) ? o.G2 : o.G4;
+    double tf2 = (sc == 0 || PLANT(10)) ? 4.0 : 16.0;
     std::vector<std::vector<double>> fl(21, std::vector<double>(nf, 0.0));
     for (int f = 0; f < 21; ++f) filter_from_spec(o, spec[f], G, fl[f]);
     std::vector<double>& AB = sc == 0 ? LM : QN;
@@ -468,8 +480,8 @@ static inline double ffval(const Oracle& o, const std::vector<double>& F, int i,
     if (j == 0) jj = 2;
     else if (j == ny + 1) jj = ny - 1;
     int ii = i;
-    if (i == 0) ii = nx;
-    else if (i == nx + 1) ii = 1;
+    if (i == 0) ii = PLANT(18) ? 2 : nx;
+    else if (i == nx + 1) ii = PLANT(18) ? nx - 1 : 1;
     return F[o.I(ii - 1, jj - 1, kk)];
   } else {
     if (k == 0) kk = 1;
@@ -486,7 +498,8 @@ static void lagrangian_interp(Oracle& o) {
   int nx = o.nx, ny = o.ny, nz = o.nz;
   double cs = (double)o.P.cs_count;
   std::vector<double> ul(o.nf), vl(o.nf), wl(o.nf);
-  for (size_t q = 0; q < o.nf; ++q) { ul[q] = o.u_lag[q] / cs; vl[q] = o.v_lag[q] / cs; wl[q] = o.w_lag[q] / cs; }
+  const double csd = PLANT(15) ? 1.0 : cs;
+  for (size_t q = 0; q < o.nf; ++q) { ul[q] = o.u_lag[q] / csd; vl[q] = o.v_lag[q] / csd; wl[q] = o.w_lag[q] / csd; }
   std::vector<double> xp(o.nf), yp(o.nf), zp(o.nf);
   for (int k = 1; k <= nz + 1; ++k)
     for (size_t q = 0; q < o.np; ++q) {
@@ -501,18 +514,19 @@ static void lagrangian_interp(Oracle& o) {
         vw = 0.5 * (vl[a] + vl[b]);
         ww = wl[a];
       }
-      xp[a] = -(uw * o.P.dt * cs / o.dx);
-      yp[a] = -(vw * o.P.dt * cs / o.dy);
-      zp[a] = -(ww * o.P.dt * cs / o.dz);
+      const double sg = PLANT(14) ? -1.0 : 1.0;
+      xp[a] = -(sg * uw * o.P.dt * cs / o.dx);
+      yp[a] = -(sg * vw * o.P.dt * cs / o.dy);
+      zp[a] = -(sg * ww * o.P.dt * cs / o.dz);
     }
   for (size_t q = 0; q < o.np; ++q) {
     size_t a1 = o.I(0, 0, 1) + q, a2 = o.I(0, 0, 2) + q;
-    zp[a2] = zp[a2] + std::fmin(zp[a2], 0.0);
+    if (!PLANT(17)) zp[a2] = zp[a2] + std::fmin(zp[a2], 0.0);
     zp[a1] = zp[a1] + std::fmax(zp[a1], 0.0);
     zp[a2] = std::fmin(1.0, zp[a2]);
-    zp[a1] = std::fmin(1.0, zp[a2]);
+    zp[a1] = std::fmin(1.0, PLANT(16) ? zp[a1] : zp[a2]);
     zp[a2] = std::fmax(-1.0, zp[a2]);
-    zp[a1] = std::fmax(-1.0, zp[a2]);
+    zp[a1] = std::fmax(-1.0, PLANT(16) ? zp[a1] : zp[a2]);
   }
   std::vector<double>* Fs[4] = {&o.F_LM, &o.F_MM, &o.F_QN, &o.F_NN};
   for (int f = 0; f < 4; ++f) {
@@ -538,13 +552,16 @@ static void lagrangian_interp(Oracle& o) {
 }
 
 // tmp**(-1/8) from correctly rounded operations
-static inline double pow_m18(double t) { return 1.0 / std::sqrt(std::sqrt(std::sqrt(t))); }
+static inline double pow_m18(double t) {
+  if (PLANT(1)) return 1.0 / std::sqrt(std::sqrt(t));
+  return 1.0 / std::sqrt(std::sqrt(std::sqrt(t)));
+}
 
 // update_interpolag_coeff (src/compute_LagScaleDep.f90:607-640)
 static void update_F(Oracle& o, std::vector<double>& FAB, std::vector<double>& FBB, const std::vector<double>& AB,
                      const std::vector<double>& BB) {
-  double opftdelta = 1.5 * o.delta;
-  double lagran_dt = o.P.dt * (double)o.P.cs_count;
+  double opftdelta = (PLANT(2) ? 1.0 : 1.5) * o.delta;
+  double lagran_dt = o.P.dt * (PLANT(3) ? 1.0 : (double)o.P.cs_count);
   for (int k = 1; k <= o.nz + 1; ++k)
     for (size_t q = 0; q < o.np; ++q) {
       size_t a = o.I(0, 0, k) + q;
@@ -553,9 +570,9 @@ static void update_F(Oracle& o, std::vector<double>& FAB, std::vector<double>& F
       tmp = fmax32(tmp, 1e-24f);
       double dumfac = lagran_dt / tmp;
       double epsi = dumfac / (1.0 + dumfac);
-      double fab = epsi * AB[a] + (1.0 - epsi) * FAB[a];
-      double fbb = epsi * BB[a] + (1.0 - epsi) * FBB[a];
-      FAB[a] = fmax32(fab, 1e-24f);
+      double fab = PLANT(4) ? (1.0 - epsi) * AB[a] + epsi * FAB[a] : epsi * AB[a] + (1.0 - epsi) * FAB[a];
+      double fbb = PLANT(4) ? (1.0 - epsi) * BB[a] + epsi * FBB[a] : epsi * BB[a] + (1.0 - epsi) * FBB[a];
+      FAB[a] = PLANT(5) ? fab : fmax32(fab, 1e-24f);
       FBB[a] = fbb;
     }
 }
@@ -568,11 +585,11 @@ static void dynamic_coeff(Oracle& o) {
       size_t a = o.I(0, 0, k) + q;
       double c2 = fmax32(o.F_LM[a] / o.F_MM[a], 1e-24f);
       double c4 = fmax32(o.F_QN[a] / o.F_NN[a], 1e-24f);
-      double b = c4 / c2;
-      b = fmax32(b, 0.125f);
-      if (k == o.nz + 1) b = 1.0;
+      double b = PLANT(6) ? c2 / c4 : c4 / c2;
+      b = fmax32(b, PLANT(7) ? 0.25f : 0.125f);
+      if (k == o.nz + 1 && !PLANT(8)) b = 1.0;
       o.beta[a] = b;
-      o.cs_opt2[a] = fmax32(c2 / b, 1e-24f);
+      o.cs_opt2[a] = fmax32(PLANT(9) ? c2 * b : c2 / b, 1e-24f);
     }
 }
 
@@ -592,19 +609,23 @@ static void lasd(Oracle& o, int jtt) {
         Smod[a] = smag(o.s11[a], o.s12[a], o.s13[a], o.s22[a], o.s23[a], o.s33[a]);
       }
     dynamic_mum(o, Smod, LM, MM, QN, NN);
+    bool interp_after = false;
     if (o.F_coeff_init) {
-      lagrangian_interp(o);
+      if (PLANT(20)) interp_after = true;
+      else lagrangian_interp(o);
     } else {
+      const double c0 = PLANT(19) ? 1.0 : 0.03;
       for (int k = 1; k <= o.nz + 1; ++k)
         for (size_t q = 0; q < o.np; ++q) {
           size_t a = o.I(0, 0, k) + q;
-          o.F_MM[a] = MM[a]; o.F_LM[a] = 0.03 * MM[a];
-          o.F_NN[a] = NN[a]; o.F_QN[a] = 0.03 * NN[a];
+          o.F_MM[a] = MM[a]; o.F_LM[a] = c0 * MM[a];
+          o.F_NN[a] = NN[a]; o.F_QN[a] = c0 * NN[a];
         }
       o.F_coeff_init = true;
     }
     update_F(o, o.F_LM, o.F_MM, LM, MM);
     update_F(o, o.F_QN, o.F_NN, QN, NN);
+    if (interp_after) lagrangian_interp(o);
     dynamic_coeff(o);
     o.lagran_flag = true;
     std::fill(o.u_lag.begin(), o.u_lag.end(), 0.0);
@@ -674,11 +695,11 @@ static void sgs_stress(Oracle& o, int jt, int jtt) {
     double z = 0.5 * o.dz;
     l[1] = std::pow(std::pow(o.P.cs0, n) * std::pow(KVONK * z, -n) + std::pow(o.delta, -n), -1.0 / n);
     for (int k = 2; k <= nz + 1; ++k) {
-      z = ((double)k - 1.0) * o.dz;
+      z = ((double)k - (PLANT(24) ? 0.0 : 1.0)) * o.dz;
       l[k] = std::pow(std::pow(o.P.cs0, n) * std::pow(KVONK * z, -n) + std::pow(o.delta, -n), -1.0 / n);
     }
     for (int k = 1; k <= nz + 1; ++k)
-      for (size_t q = 0; q < o.np; ++q) o.cs_opt2[o.I(0, 0, k) + q] = o.P.cs0 * o.P.cs0;
+      for (size_t q = 0; q < o.np; ++q) o.cs_opt2[o.I(0, 0, k) + q] = PLANT(25) ? o.P.cs0 : o.P.cs0 * o.P.cs0;
   } else if (o.P.sgs_model == 1) {
     dynsmag(o, jt, jtt);
   } else {
@@ -696,15 +717,17 @@ static void sgs_stress(Oracle& o, int jt, int jtt) {
     for (size_t q = 0; q < o.np; ++q) {
       size_t a = o.I(0, 0, k) + q;
       if (k <= nz) {
-        double nu = (k == 1) ? o.nu_t[a] : 0.5 * (o.nu_t[a] + o.nu_t[o.I(0, 0, k + 1) + q]);
+        double nu = (k == 1 && !PLANT(22)) ? o.nu_t[a] : 0.5 * (o.nu_t[a] + o.nu_t[o.I(0, 0, k + 1) + q]);
+        if (PLANT(21)) nu = o.nu_t[a];
         o.txx[a] = 2.0 * nu * o.dudx[a];
         o.txy[a] = 2.0 * nu * 0.5 * (o.dudy[a] + o.dvdx[a]);
         o.tyy[a] = 2.0 * nu * o.dvdy[a];
         o.tzz[a] = 2.0 * nu * o.dwdz[a];
       }
       if (k >= 2) {
-        o.txz[a] = 2.0 * o.nu_t[a] * o.s13[a];
-        o.tyz[a] = 2.0 * o.nu_t[a] * o.s23[a];
+        double nw = PLANT(23) ? 0.5 * (o.nu_t[a] + o.nu_t[o.I(0, 0, k - 1) + q]) : o.nu_t[a];
+        o.txz[a] = 2.0 * nw * o.s13[a];
+        o.tyz[a] = 2.0 * nw * o.s23[a];
       }
     }
   for (size_t q = 0; q < o.np; ++q) o.tzz[q] = o.tzz[o.I(0, 0, 1) + q];
@@ -816,13 +839,15 @@ static void convection(Oracle& o, std::vector<double>& cx, std::vector<double>& 
         for (size_t q = 0; q < nb; ++q) {
           double val;
           if (t == 0) {
-            if (k == 1) val = c2 * (-(vb[q] * o3[q]) + 0.5 * wbp[q] * o2p[q]);
+            if (k == 1) val = c2 * (-(vb[q] * o3[q]) + (PLANT(26) ? 1.0 : 0.5) * wbp[q] * o2p[q]);
+            else if (PLANT(28)) val = c2 * (-(vb[q] * o3[q]) + 0.5 * (wb[q] * o2[q] + big[2][nb * (k - 1) + q] * big[4][nb * (k - 1) + q]));
             else val = c2 * (-(vb[q] * o3[q]) + 0.5 * (wbp[q] * o2p[q] + wb[q] * o2[q]));
           } else if (t == 1) {
             if (k == 1) val = c2 * (ub[q] * o3[q] + 0.5 * wbp[q] * (-o1p[q]));
             else val = c2 * (ub[q] * o3[q] + 0.5 * (wb[q] * (-o1[q]) + wbp[q] * (-o1p[q])));
           } else {
             if (k == 1) val = 0.0;
+            else if (PLANT(27)) val = c2 * 0.5 * ((ub[q] + ub[q]) * (-o2[q]) + (vb[q] + vb[q]) * o1[q]);
             else val = c2 * 0.5 * ((ub[q] + ubm[q]) * (-o2[q]) + (vb[q] + vbm[q]) * o1[q]);
           }
           prod[q] = val;
@@ -940,14 +965,15 @@ static void div_stress(Oracle& o, std::vector<double>& rhs, const std::vector<do
   for (int k = 1; k <= nz; ++k)
     for (size_t q = 0; q < o.np; ++q) {
       size_t a = o.I(0, 0, k) + q;
-      double dz_ = stag ? cz * (tz[o.I(0, 0, k + 1) + q] - tz[a]) : cz * (tz[a] - tz[o.I(0, 0, k - 1) + q]);
+      bool st = PLANT(38) ? !stag : stag;
+      double dz_ = st ? cz * (tz[o.I(0, 0, k + 1) + q] - tz[a]) : cz * (tz[a] - tz[o.I(0, 0, k - 1) + q]);
       rhs[a] = rhs[a] + divh[a] + dz_;
     }
   if (save_wk) {
     for (size_t q = 0; q < o.np; ++q) {
       size_t a1 = o.I(0, 0, 1) + q;
       double dz1 = cz * (tz[a1] - tz[q]);
-      o.bottom_wk[q] = divh[a1] + dz1;
+      o.bottom_wk[q] = PLANT(37) ? divh[a1] : divh[a1] + dz1;
       o.top_wk[q] = divh[o.I(0, 0, nz + 1) + q];
     }
   }
@@ -1040,8 +1066,8 @@ static void pressure(Oracle& o, double tadv1) {
       for (int i = 0; i < lhx; ++i) {
         if (nyq(o, i, j)) continue;
         size_t q = i + (size_t)lhx * j;
-        Cx bl = cscale(bot[q], dz), tl = cscale(top[q], dz);
-        r[1] = (i == 0 && j == 0) ? Cx{0.0, 0.0} : bl;
+        Cx bl = cscale(bot[q], PLANT(29) ? 1.0 : dz), tl = cscale(PLANT(30) ? bot[q] : top[q], dz);
+        r[1] = (i == 0 && j == 0 && !PLANT(31)) ? Cx{0.0, 0.0} : bl;
         for (int k = 1; k <= nz; ++k) {
           Cx hx = Hx[ns * k + q], hy = Hy[ns * k + q], hz = Hz[ns * k + q], hzp = Hz[ns * (k + 1) + q];
           double sr = o.kkx[q] * hx.x + o.kky[q] * hy.x;
@@ -1085,8 +1111,8 @@ static void step(Oracle& o) {
   compute_vel_grad(o);
   sgs_stress(o, o.jt, jtt);
   compute_rhs(o);
-  double tadv1 = (jtt == 1) ? 1.0 : 1.5;
-  double tadv2 = (jtt == 1) ? 0.0 : 1.0 - tadv1;
+  double tadv1 = (jtt == 1 && !PLANT(32)) ? 1.0 : 1.5;
+  double tadv2 = (jtt == 1) ? 0.0 : (PLANT(34) ? 0.5 : 1.0 - tadv1);
   double dt = o.P.dt;
 #pragma omp parallel for schedule(static)
   for (int k = 1; k <= nz; ++k)
@@ -1106,9 +1132,11 @@ static void step(Oracle& o) {
       o.u[a] = o.u[a] - dtt * o.dpdx[a];
       o.v[a] = o.v[a] - dtt * o.dpdy[a];
       o.w[a] = o.w[a] - dtt * o.dpdz[a];
-      o.rhsx[a] = o.rhsx[a] - o.dpdx[a];
-      o.rhsy[a] = o.rhsy[a] - o.dpdy[a];
-      o.rhsz[a] = o.rhsz[a] - o.dpdz[a];
+      if (!PLANT(33)) {
+        o.rhsx[a] = o.rhsx[a] - o.dpdx[a];
+        o.rhsy[a] = o.rhsy[a] - o.dpdy[a];
+        o.rhsz[a] = o.rhsz[a] - o.dpdz[a];
+      }
     }
   for (size_t q = 0; q < o.np; ++q) { o.w[o.I(0, 0, 1) + q] = 0.0; o.w[o.I(0, 0, nz + 1) + q] = 0.0; }
   set_bc_all(o);
@@ -1305,7 +1333,7 @@ int orc_init(const int* ip, const double* rp) {
   o.px.build(o.nx); o.py.build(o.ny); o.px2.build(3 * o.nx / 2); o.py2.build(3 * o.ny / 2);
   if (!o.px.n || !o.py.n || !o.px2.n || !o.py2.n || (o.nx % 4) || (o.ny % 4)) return 1;
   build_tables(o);
-  o.logterm.assign(o.np, std::log((0.5 * o.dz) / P.z0));
+  o.logterm.assign(o.np, std::log(((PLANT(36) ? 1.0 : 0.5) * o.dz) / P.z0));
   std::vector<double>* all[] = {&o.u, &o.v, &o.w, &o.dudx, &o.dudy, &o.dudz, &o.dvdx, &o.dvdy, &o.dvdz, &o.dwdx,
                                 &o.dwdy, &o.dwdz, &o.p, &o.dpdx, &o.dpdy, &o.dpdz, &o.rhsx, &o.rhsy, &o.rhsz,
                                 &o.rhsx_f, &o.rhsy_f, &o.rhsz_f, &o.txx, &o.txy, &o.txz, &o.tyy, &o.tyz, &o.tzz,
@@ -1464,6 +1492,7 @@ int orc_op(int which, double arg) {
     case 6: convection(o, o.rhsx, o.rhsy, o.rhsz); break;
     case 7: lagrangian_interp(o); break;
     case 8: set_bc_all(o); break;
+    case 9: dynamic_coeff(o); break;
     default: return 2;
   }
   return 0;
@@ -1473,6 +1502,35 @@ int orc_set_jt_total(int jt) { if (!g) return 1; g->jt_total = jt; return 0; }
 int orc_jt(void) { return g ? g->jt : -1; }
 // sgs_stress(jt, jtt) with the two step counts given separately
 int orc_sgs_stress(int jt, int jtt) { if (!g) return 1; sgs_stress(*g, jt, jtt); return 0; }
+
+// The LASD update's parts on their own, for the tests that pin each of them:
+// update_interpolag_coeff of (F_LM, F_MM) (which = 0) or (F_QN, F_NN)
+// (which = 1) with the given AB, BB fields (nx, ny, nz+2); and
+// compute_dynamic_mum at both filter widths from the current u, v, w and S_ij
+// (LM, MM, QN, NN out, (nx, ny, nz+2), levels 1..nz+1 written).
+int orc_update_F(int which, const double* AB, const double* BB) {
+  if (!g) return 1;
+  std::vector<double> ab(AB, AB + g->nf), bb(BB, BB + g->nf);
+  if (which == 0) update_F(*g, g->F_LM, g->F_MM, ab, bb);
+  else update_F(*g, g->F_QN, g->F_NN, ab, bb);
+  return 0;
+}
+int orc_dynamic_mum(double* LM, double* MM, double* QN, double* NN) {
+  if (!g) return 1;
+  Oracle& o = *g;
+  std::vector<double> Smod(o.nf, 0.0), lm(o.nf, 0.0), mm(o.nf, 0.0), qn(o.nf, 0.0), nn(o.nf, 0.0);
+  for (int k = 1; k <= o.nz + 1; ++k)
+    for (size_t q = 0; q < o.np; ++q) {
+      size_t a = o.I(0, 0, k) + q;
+      Smod[a] = smag(o.s11[a], o.s12[a], o.s13[a], o.s22[a], o.s23[a], o.s33[a]);
+    }
+  dynamic_mum(o, Smod, lm, mm, qn, nn);
+  std::memcpy(LM, lm.data(), sizeof(double) * o.nf);
+  std::memcpy(MM, mm.data(), sizeof(double) * o.nf);
+  std::memcpy(QN, qn.data(), sizeof(double) * o.nf);
+  std::memcpy(NN, nn.data(), sizeof(double) * o.nf);
+  return 0;
+}
 
 // running statistics and diagnostics
 int orc_stat_reset(void) { if (!g) return 1; stat_reset(*g); return 0; }

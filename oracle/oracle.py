@@ -20,15 +20,18 @@ FIELDS = ["u", "v", "w", "dudx", "dudy", "dudz", "dvdx", "dvdy", "dvdz", "dwdx",
           "beta", "u_lag", "v_lag", "w_lag", "s11", "s12", "s13", "s22", "s23", "s33"]
 FID = {n: i for i, n in enumerate(FIELDS)}
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not LIB.exists():
-            raise RuntimeError(f"oracle library missing: {LIB} (run make in {HERE})")
-        _lib = ctypes.CDLL(str(LIB))
+def lib(path=None):
+    """The oracle library (liblesoracle.so beside this file), or, for the
+    oracle's own pinning tests, a build of it at `path` (one with a planted
+    mistake).  Each library holds its own oracle state."""
+    path = str(path or LIB)
+    if path not in _libs:
+        if not pathlib.Path(path).exists():
+            raise RuntimeError(f"oracle library missing: {path} (run make in {HERE})")
+        _lib = ctypes.CDLL(path)
         d = ctypes.POINTER(ctypes.c_double)
         i = ctypes.POINTER(ctypes.c_int)
         for name, args in {
@@ -46,11 +49,13 @@ def lib():
             "orc_sgs_stress": [ctypes.c_int, ctypes.c_int],
             "orc_stat_reset": [], "orc_stat_accumulate": [], "orc_stat_get": [d, d, i],
             "orc_stat_profiles": [d], "orc_diagnostics": [d],
+            "orc_update_F": [ctypes.c_int, d, d], "orc_dynamic_mum": [d, d, d, d],
         }.items():
             f = getattr(_lib, name)
             f.argtypes = args
             f.restype = ctypes.c_int
-    return _lib
+        _libs[path] = _lib
+    return _libs[path]
 
 
 def _dp(a):
@@ -106,11 +111,12 @@ def trunc_plane(plane):
 class Oracle:
     """FP64 CPU restatement of the reference step (one global instance)."""
 
-    def __init__(self, prm):
+    def __init__(self, prm, lib_path=None):
         self.prm = prm
+        self.L = lib(lib_path)
         ip = np.array(prm.int_vector(), dtype=np.int32)
         rp = np.array(prm.real_vector(), dtype=np.float64)
-        rc = lib().orc_init(ip.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _dp(rp))
+        rc = self.L.orc_init(ip.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _dp(rp))
         if rc:
             raise ValueError(f"oracle init failed ({rc})")
         self.shape = (prm.nz + 2, prm.ny, prm.nx)
@@ -121,37 +127,37 @@ class Oracle:
 
     def set_ic(self, u, v, w):
         u, v, w = (np.ascontiguousarray(a, dtype=np.float64) for a in (u, v, w))
-        lib().orc_set_ic(_dp(u), _dp(v), _dp(w))
+        self.L.orc_set_ic(_dp(u), _dp(v), _dp(w))
 
     def step(self, n=1):
-        lib().orc_step(n)
+        self.L.orc_step(n)
 
     def get(self, name):
         out = np.zeros(self.shape)
-        rc = lib().orc_get(FID[name], _dp(out))
+        rc = self.L.orc_get(FID[name], _dp(out))
         if rc:
             raise KeyError(name)
         return out
 
     def put(self, name, arr):
         a = np.ascontiguousarray(arr, dtype=np.float64).reshape(self.shape)
-        lib().orc_put(FID[name], _dp(a))
+        self.L.orc_put(FID[name], _dp(a))
 
     def get2d(self, which):
         idx = {"lbc_u": 0, "lbc_v": 1, "ustar": 2, "bottom_wk": 3, "top_wk": 4}[which]
         out = np.zeros(self.shape[1:])
-        lib().orc_get_2d(idx, _dp(out))
+        self.L.orc_get_2d(idx, _dp(out))
         return out
 
     def set2d(self, which, arr):
         idx = {"lbc_u": 0, "lbc_v": 1, "ustar": 2, "bottom_wk": 3, "top_wk": 4}[which]
         a = np.ascontiguousarray(arr, dtype=np.float64)
-        lib().orc_set_2d(idx, _dp(a))
+        self.L.orc_set_2d(idx, _dp(a))
 
     def table(self, which):
         idx = {"kkx": 0, "kky": 1, "kk2": 2, "G2": 3, "G4": 4, "GA": 5, "GB": 6}[which]
         out = np.zeros((self.prm.ny, self.prm.nx // 2 + 1))
-        if lib().orc_table(idx, _dp(out)):
+        if self.L.orc_table(idx, _dp(out)):
             raise KeyError(which)
         return out
 
@@ -159,26 +165,40 @@ class Oracle:
         rr = np.zeros(self.prm.nz + 3, dtype=np.complex128)
         rr[1:self.prm.nz + 3] = r
         x = np.zeros_like(rr)
-        lib().orc_tridiag(i, j, _dp(rr.view(np.float64)), _dp(x.view(np.float64)))
+        self.L.orc_tridiag(i, j, _dp(rr.view(np.float64)), _dp(x.view(np.float64)))
         return x[1:self.prm.nz + 3]
 
     def op(self, which, arg=0.0):
         code = {"wall_law": 0, "vel_grad": 1, "sgs": 2, "rhs": 3, "pressure": 4, "sij": 5, "conv": 6,
-                "lagrangian": 7, "set_bc": 8}[which]
-        lib().orc_op(code, float(arg))
+                "lagrangian": 7, "set_bc": 8, "dynamic_coeff": 9}[which]
+        self.L.orc_op(code, float(arg))
+
+    def update_F(self, which, AB, BB):
+        """update_interpolag_coeff of (F_LM, F_MM) (which 0) or (F_QN, F_NN)
+        (which 1) with the fields AB, BB."""
+        a = np.ascontiguousarray(np.broadcast_to(AB, self.shape), dtype=np.float64)
+        b = np.ascontiguousarray(np.broadcast_to(BB, self.shape), dtype=np.float64)
+        self.L.orc_update_F(int(which), _dp(a), _dp(b))
+
+    def dynamic_mum(self):
+        """compute_dynamic_mum at 2 delta and 4 delta from the current u, v,
+        w and S_ij: (LM, MM, QN, NN) on levels 1..nz+1."""
+        out = [np.zeros(self.shape) for _ in range(4)]
+        self.L.orc_dynamic_mum(*(_dp(x) for x in out))
+        return tuple(out)
 
     def set_jt_total(self, jt):
-        lib().orc_set_jt_total(jt)
+        self.L.orc_set_jt_total(jt)
 
     def jt_total(self):
-        return lib().orc_jt_total()
+        return self.L.orc_jt_total()
 
     def jt(self):
         """Steps taken since initialisation (the reference's local count jt)."""
-        return lib().orc_jt()
+        return self.L.orc_jt()
 
     def sgs_stress(self, jt, jtt):
-        lib().orc_sgs_stress(int(jt), int(jtt))
+        self.L.orc_sgs_stress(int(jt), int(jtt))
 
     # ---- running statistics and diagnostics
     STATS = ["u", "v", "w", "p", "u2", "v2", "w2", "u3", "v3", "w3", "u4", "v4", "w4", "uv", "uw", "vw",
@@ -187,26 +207,26 @@ class Oracle:
                 "R23", "tyz", "St23", "tke", "cs", "nut", "C13", "C23"]
 
     def stat_reset(self):
-        lib().orc_stat_reset()
+        self.L.orc_stat_reset()
 
     def stat_accumulate(self):
-        lib().orc_stat_accumulate()
+        self.L.orc_stat_accumulate()
 
     def stat_get(self):
         nz, ny, nx = self.prm.nz, self.prm.ny, self.prm.nx
         s = np.zeros((len(self.STATS), nz, ny, nx))
         us = np.zeros((ny, nx))
         it = ctypes.c_int()
-        lib().orc_stat_get(_dp(s), _dp(us), ctypes.byref(it))
+        self.L.orc_stat_get(_dp(s), _dp(us), ctypes.byref(it))
         return dict(zip(self.STATS, s)), us, it.value
 
     def stat_profiles(self):
         out = np.zeros((len(self.PROFILES), self.prm.nz))
-        if lib().orc_stat_profiles(_dp(out)):
+        if self.L.orc_stat_profiles(_dp(out)):
             raise ValueError("no accumulated statistics")
         return dict(zip(self.PROFILES, out))
 
     def diagnostics(self):
         out = np.zeros(4)
-        lib().orc_diagnostics(_dp(out))
+        self.L.orc_diagnostics(_dp(out))
         return {"cfl": out[0], "mke": out[1], "u_star": out[2], "divu": out[3]}
