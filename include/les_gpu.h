@@ -110,6 +110,14 @@ int les_timers(double* t, int reset);
  * t[7 + 3f] algorithmic bytes (inputs read once + outputs written once).
  * t must hold 32 doubles.  Timing synchronises after every timed launch. */
 int les_set_timing(int on);
+/* Per-kernel view of the same timing mode (no counterpart in the reference,
+ * whose profiling.txt has the five phases only): *count receives the number
+ * of kernels timed since the last les_timers reset; for 0 <= i < *count,
+ * name receives the kernel's name (template precisions and variant, e.g.
+ * "k_cols_c16<float> [GRAD]"; memset and copy groups are named after the
+ * runtime call), v[0] its summed ms, v[1] launches, v[2] algorithmic bytes,
+ * v[3] its family.  i < 0 with name and v NULL only sets *count. */
+int les_kernel_stat(int i, int* count, char* name, int name_cap, double* v);
 /* Schedule of les_step (no counterpart in the reference, whose step is one
  * sequence): 1 (default) runs the SGS stage and the spectra of the stress
  * divergence on a second stream beside the padded convection, and in mixed

@@ -78,7 +78,7 @@ def library() -> ctypes.CDLL:
             "les_upload_layer": [ctypes.c_int, dp], "les_download_layer": [ctypes.c_int, dp],
             "les_field_ptr": [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), ip],
             "les_step": [ctypes.c_int], "les_sync": [], "les_jt_total": [], "les_set_jt_total": [ctypes.c_int],
-            "les_check_nan": [ip], "les_timers": [dp, ctypes.c_int], "les_set_timing": [ctypes.c_int], "les_set_overlap": [ctypes.c_int],
+            "les_check_nan": [ip], "les_timers": [dp, ctypes.c_int], "les_set_timing": [ctypes.c_int], "les_kernel_stat": [ctypes.c_int, ip, ctypes.c_char_p, ctypes.c_int, dp], "les_set_overlap": [ctypes.c_int],
             "les_wall_law": [], "les_compute_vel_grad": [],
             "les_grad_velocity_filt": [ctypes.c_int] * 3, "les_ddz_uv": [ctypes.c_int] * 2,
             "les_ddz_w": [ctypes.c_int] * 2, "les_sgs_stress": [ctypes.c_int] * 2,
@@ -203,6 +203,19 @@ class LesGpu:
         t = np.zeros(5 + 3 * 9)
         check(self.lib.les_timers(_dp(t), int(reset)))
         return t
+
+    def kernel_stats(self) -> Dict[str, Dict[str, float]]:
+        """Per-kernel HIP-event statistics of the timing mode since the last
+        timers(reset=True): name -> ms, calls, bytes, family."""
+        n = ctypes.c_int()
+        check(self.lib.les_kernel_stat(-1, ctypes.byref(n), None, 0, None))
+        out = {}
+        buf = ctypes.create_string_buffer(200)
+        v = np.zeros(4)
+        for i in range(n.value):
+            check(self.lib.les_kernel_stat(i, ctypes.byref(n), buf, 200, _dp(v)))
+            out[buf.value.decode()] = {"ms": float(v[0]), "calls": int(v[1]), "bytes": float(v[2]), "family": int(v[3])}
+        return out
 
     # ---- the compute_* procedures
     def wall_law(self) -> None:

@@ -108,7 +108,13 @@ static bool field_is_fp64(int id) {
 static void kt_begin(Les& L, int fam) {
   if (L.timing) hipEventRecord(L.ev_a[fam], L.stream);
 }
-static void kt_end(Les& L, int fam, double bytes = 0.0) {
+// Ends the timed launch begun by kt_begin: adds its time, its algorithmic
+// bytes (every logical input read once, every output written once, at the
+// storage precision) and one launch to the kernel family `fam` and to the
+// kernel named `base` (with the element sizes ta, tb of its template
+// precisions, 0 for none, and an optional variant tag).
+static void kt_end(Les& L, int fam, double bytes, const char* base, int ta = 0, int tb = 0,
+                   const char* tag = nullptr) {
   if (L.timing) {
     L.kt_bytes[fam] += bytes;
     hipEventRecord(L.ev_b[fam], L.stream);
@@ -117,6 +123,21 @@ static void kt_end(Les& L, int fam, double bytes = 0.0) {
     hipEventElapsedTime(&ms, L.ev_a[fam], L.ev_b[fam]);
     L.kt_ms[fam] += ms;
     L.kt_calls[fam] += 1;
+    int i = 0;
+    for (; i < L.nkst; ++i) {
+      const Les::KStat& k = L.kst[i];
+      if (k.ta == ta && k.tb == tb && std::strcmp(k.base, base) == 0 &&
+          ((!k.tag && !tag) || (k.tag && tag && std::strcmp(k.tag, tag) == 0)))
+        break;
+    }
+    if (i == L.nkst) {
+      if (L.nkst == LES_MAX_KSTAT) return;
+      L.kst[i] = Les::KStat{base, ta, tb, tag, fam, 0.0, 0.0, 0};
+      L.nkst += 1;
+    }
+    L.kst[i].ms += ms;
+    L.kst[i].bytes += bytes;
+    L.kst[i].calls += 1;
   }
 }
 
@@ -142,7 +163,8 @@ static void rows_r2c(Les& L, const TI* const* src, Cplx<T>* const* dst, int nf, 
   kt_begin(L, KT_ROWS_R2C);
   hipLaunchKernelGGL((k_rows_r2c<TI, T>), grid, dim3(LES_BLOCK), sh, L.stream, a, L.px, twpool<T>(L));
   const double in_fields = mode == LOAD_AB2_SCALE_DIV ? 3.0 : 1.0;
-  kt_end(L, KT_ROWS_R2C, (double)nf * npl * a.ny * (in_fields * L.nx * sizeof(TI) + (double)L.lhx * sizeof(Cplx<T>)));
+  kt_end(L, KT_ROWS_R2C, (double)nf * npl * a.ny * (in_fields * L.nx * sizeof(TI) + (double)L.lhx * sizeof(Cplx<T>)),
+         "k_rows_r2c", sizeof(TI), sizeof(T), mode == LOAD_AB2_SCALE_DIV ? "AB2 predictor" : nullptr);
 }
 
 template <typename T, typename TO>
@@ -156,7 +178,8 @@ static void rows_c2r(Les& L, const Cplx<T>* const* src, TO* const* dst, int nf, 
   size_t sh = (size_t)kWavesPerBlock * a.ls * sizeof(Cplx<T>);
   kt_begin(L, KT_ROWS_C2R);
   hipLaunchKernelGGL((k_rows_c2r<T, TO>), grid, dim3(LES_BLOCK), sh, L.stream, a, L.px, twpool<T>(L));
-  kt_end(L, KT_ROWS_C2R, (double)nf * npl * L.ny * ((double)L.lhx * sizeof(Cplx<T>) + (double)L.nx * sizeof(TO)));
+  kt_end(L, KT_ROWS_C2R, (double)nf * npl * L.ny * ((double)L.lhx * sizeof(Cplx<T>) + (double)L.nx * sizeof(TO)),
+         "k_rows_c2r", sizeof(T), sizeof(TO));
 }
 
 // Column tiles: LES_COL_TILE_BYTES contiguous bytes per row for
@@ -251,7 +274,9 @@ static void cols(Les& L, int op, const Cplx<T>* in0, const Cplx<T>* in1, Cplx<T>
   if (cb == 4) launch_cols<T, 4>(L, a, npl);
   else if (cb == 8) launch_cols<T, 8>(L, a, npl);
   else launch_cols<T, 16>(L, a, npl);
-  kt_end(L, KT_COLS, (double)npl * L.ns * sizeof(Cplx<T>) * (a.nin + nout));
+  static const char* const opn[] = {"GRAD", "DIV", "FILT2", "FILT1", "COPY", "PGRAD"};
+  kt_end(L, KT_COLS, (double)npl * L.ns * sizeof(Cplx<T>) * (a.nin + nout),
+         (L.ny == 256 && LES_COL16) ? "k_cols_c16" : L.ny == 256 ? "k_cols_256" : "k_cols", sizeof(T), 0, opn[op]);
 }
 
 // ------------------------------------------------------------------ field access
@@ -312,7 +337,7 @@ static void vel_grad_bc_ddz(Les& L) {
   a.ubc_u = L.P.ubc_u; a.ubc_v = L.P.ubc_v; a.ubc_w = L.P.ubc_w;
   kt_begin(L, KT_POINTWISE);
   hipLaunchKernelGGL((k_bc_ddz<T>), dim3(nblk(L.np * (L.nz + 1))), dim3(LES_BLOCK), 0, L.stream, a);
-  kt_end(L, KT_POINTWISE, (double)L.np * (L.nz + 1) * (3 * 8.0 + 3.0 * sizeof(T)));
+  kt_end(L, KT_POINTWISE, (double)L.np * (L.nz + 1) * (3 * 8.0 + 3.0 * sizeof(T)), "k_bc_ddz", sizeof(T));
 }
 
 // 2-delta filtered first level (get_layer_filter, src/utility_tools.f90:261-307)
@@ -348,7 +373,7 @@ static void wall_law(Les& L) {
   a.np = L.np;
   kt_begin(L, KT_POINTWISE);
   hipLaunchKernelGGL((k_wall<T>), dim3(nblk(L.np)), dim3(LES_BLOCK), 0, L.stream, a);
-  kt_end(L, KT_POINTWISE, (double)L.np * (2.0 * sizeof(T) + 8 * 4.0 + 4.0 * sizeof(T)));
+  kt_end(L, KT_POINTWISE, (double)L.np * (2.0 * sizeof(T) + 8 * 4.0 + 4.0 * sizeof(T)), "k_wall", sizeof(T));
 }
 
 template <typename T>
@@ -392,7 +417,7 @@ static void sgs_stress(Les& L, int jt, int jtt) {
   if (L.P.sgs_model == 3) {
     kt_begin(L, KT_POINTWISE);
     hipLaunchKernelGGL((k_dns_stress<T>), dim3(nblk(L.np * L.nz)), dim3(LES_BLOCK), 0, L.stream, a, (T)(2.0 / L.P.re));
-    kt_end(L, KT_POINTWISE, (double)L.np * L.nz * 15.0 * sizeof(T));
+    kt_end(L, KT_POINTWISE, (double)L.np * L.nz * 15.0 * sizeof(T), "k_dns_stress", sizeof(T));
     return;
   }
   bool lasd = L.P.sgs_model == 2;
@@ -415,7 +440,8 @@ static void sgs_stress(Les& L, int jt, int jtt) {
     a.write_sij = 1;
     kt_begin(L, KT_POINTWISE);
     hipLaunchKernelGGL((k_sij<T>), dim3(nblk(L.np * (L.nz + 1))), dim3(LES_BLOCK), 0, L.stream, a);
-    kt_end(L, KT_POINTWISE, (double)L.np * (L.nz + 1) * (16.0 * sizeof(T) + (a.accumulate ? 3 * 8.0 + 6.0 * sizeof(T) : 0.0)));
+    kt_end(L, KT_POINTWISE, (double)L.np * (L.nz + 1) * (16.0 * sizeof(T) + (a.accumulate ? 3 * 8.0 + 6.0 * sizeof(T) : 0.0)),
+           "k_sij", sizeof(T), 0, a.accumulate ? "with u_lag sums" : nullptr);
     if constexpr (sizeof(T) == 4) {
       if (L.P.precision == 2) learned_update<T>(L, jtt);
       else lasd_update<T>(L, jtt, L.nz + 1);
@@ -427,7 +453,7 @@ static void sgs_stress(Les& L, int jt, int jtt) {
     a.write_sij = 1;
     kt_begin(L, KT_POINTWISE);
     hipLaunchKernelGGL((k_sij<T>), dim3(nblk(L.np * (L.nz + 1))), dim3(LES_BLOCK), 0, L.stream, a);
-    kt_end(L, KT_POINTWISE, (double)L.np * (L.nz + 1) * 16.0 * sizeof(T));
+    kt_end(L, KT_POINTWISE, (double)L.np * (L.nz + 1) * 16.0 * sizeof(T), "k_sij", sizeof(T));
     dynsmag_update<T>(L);
   } else {
     a.accumulate = acc ? 1 : 0;
@@ -435,7 +461,8 @@ static void sgs_stress(Les& L, int jt, int jtt) {
   a.write_sij = 0;
   kt_begin(L, KT_POINTWISE);
   hipLaunchKernelGGL((k_sgs<T>), dim3(nblk(L.np * (L.nz + 1))), dim3(LES_BLOCK), 0, L.stream, a);
-  kt_end(L, KT_POINTWISE, (double)L.np * (L.nz + 1) * (17.0 * sizeof(T) + (a.accumulate ? 3 * 8.0 + 6.0 * sizeof(T) : 0.0)));
+  kt_end(L, KT_POINTWISE, (double)L.np * (L.nz + 1) * (17.0 * sizeof(T) + (a.accumulate ? 3 * 8.0 + 6.0 * sizeof(T) : 0.0)),
+         "k_sgs", sizeof(T), 0, a.accumulate ? "with u_lag sums" : nullptr);
 }
 
 #include "les_lasd.h"
@@ -563,14 +590,21 @@ static void nyquist_remove_fp64(Les& L) {
   const long nrows = (long)L.ny * L.nzg;
   const int rpw = L.ny % 16 == 0 ? 4 : 1;
   const unsigned nblocks = (unsigned)(nrows / (4 * rpw));
+  // algorithmic bytes: the reduction reads u, v, w; the apply pass reads
+  // and writes them; the row sums A and column partials are not counted
+  // (1/nx and 4/nx of a field)
   kt_begin(L, KT_POINTWISE);
   const size_t shp = sizeof(double) * (LES_BLOCK / LES_WAVE) * (size_t)L.nx;
   if (rpw == 4) hipLaunchKernelGGL(k_nyq_reduce<4>, dim3(nblocks, 3), dim3(LES_BLOCK), shp, L.stream, a);
   else hipLaunchKernelGGL(k_nyq_reduce<1>, dim3(nblocks, 3), dim3(LES_BLOCK), shp, L.stream, a);
+  kt_end(L, KT_POINTWISE, 24.0 * L.nf, "k_nyq_reduce");
+  kt_begin(L, KT_POINTWISE);
   hipLaunchKernelGGL(k_nyq_bc, dim3((unsigned)L.nzg, 3), dim3(LES_BLOCK), 0, L.stream, a, L.ny / (4 * rpw));
+  kt_end(L, KT_POINTWISE, 3.0 * 8.0 * (double)L.nzg * ((double)L.ny / (4 * rpw) * L.nx + L.nx), "k_nyq_bc");
+  kt_begin(L, KT_POINTWISE);
   if (rpw == 4) hipLaunchKernelGGL(k_nyq_apply<4>, dim3(nblocks, 3), dim3(LES_BLOCK), 0, L.stream, a);
   else hipLaunchKernelGGL(k_nyq_apply<1>, dim3(nblocks, 3), dim3(LES_BLOCK), 0, L.stream, a);
-  kt_end(L, KT_POINTWISE, 3.0 * 24.0 * L.nf);
+  kt_end(L, KT_POINTWISE, 48.0 * L.nf, "k_nyq_apply");
 }
 
 // advection step (src/LES_engine.f90:253-283)
@@ -582,12 +616,16 @@ static void ab2(Les& L, double tadv1, double tadv2) {
   a.dt = L.P.dt; a.t1 = tadv1; a.t2 = tadv2; a.np = L.np; a.nz = L.nz;
   kt_begin(L, KT_POINTWISE);
   hipLaunchKernelGGL(k_ab2, dim3(nblk(L.np * (L.nz + 2))), dim3(LES_BLOCK), 0, L.stream, a);
-  kt_end(L, KT_POINTWISE, 12.0 * 8 * L.np * L.nz);
+  kt_end(L, KT_POINTWISE, 12.0 * 8 * L.np * L.nz, "k_ab2");
 }
 
 // ------------------------------------------------------------------ step
 #ifndef LES_OVERLAP
 #define LES_OVERLAP 1
+#endif
+// les_step's projection as the store of the pressure-gradient transforms
+#ifndef LES_FUSED_PROJ
+#define LES_FUSED_PROJ 0
 #endif
 template <typename T>
 static int les_step_t(Les& L) {
@@ -646,8 +684,9 @@ static int les_step_t(Les& L) {
   double tadv2 = (jtt == 1) ? 0.0 : 1.0 - tadv1;
   // AB2 folded into the pressure right-hand side and the projection: the
   // predictor is recomputed where it is read instead of being stored
-  press_stag_array<T>(L, tadv1, true, tadv2);
-  projection(L, tadv1, true, tadv2);
+  press_stag_array<T>(L, tadv1, true, tadv2, LES_FUSED_PROJ != 0);
+  if (LES_FUSED_PROJ) press_project(L, tadv1, tadv2);
+  else projection(L, tadv1, true, tadv2);
   if (L.timing) {
     hipEventRecord(L.ph_ev[4], L.stream);
     hipEventSynchronize(L.ph_ev[4]);
@@ -747,6 +786,7 @@ int les_init(const les_params* p) {
   std::memset((void*)L, 0, sizeof(Les) - sizeof(std::vector<void*>));
   new (&L->allocs) std::vector<void*>();
   L->P = P;
+  L->dev = dev;
   L->nx = P.nx; L->ny = P.ny; L->nz = P.nz; L->nzg = P.nz + 2; L->lhx = P.nx / 2 + 1;
   L->nx2 = 3 * P.nx / 2; L->ny2 = 3 * P.ny / 2;
   L->np = (long)P.nx * P.ny; L->nf = L->np * L->nzg; L->ns = (long)L->lhx * P.ny; L->nsp = L->ns * L->nzg;
@@ -975,6 +1015,8 @@ static int copy_field(int id, double* host, int up) {
   Les* L = g_les;
   if (!L) { les_set_error("not initialised"); return 1; }
   if (id < 0 || id >= LES_NFIELDS) { les_set_error("bad field id %d", id); return 10; }
+  // the step leaves the pressure gradient to be formed on request
+  if (id == LES_DPDX || id == LES_DPDY || id == LES_DPDZ || (up && id == LES_P)) refresh_gradp(*L);
   size_t n = (size_t)L->nf;
   if (L->fbytes[id] == 8) {
     if (up) { LES_CHECK(hipMemcpyAsync(L->f[id], host, 8 * n, hipMemcpyHostToDevice, L->stream)); }
@@ -1001,6 +1043,10 @@ int les_download_field(int id, double* host) { return copy_field(id, host, 0); }
 int les_field_ptr(int id, void** dptr, int* elem_bytes) {
   if (!g_les) return 1;
   if (id < 0 || id >= LES_NFIELDS) return 10;
+  if (id == LES_DPDX || id == LES_DPDY || id == LES_DPDZ || id == LES_P) {
+    refresh_gradp(*g_les);
+    LES_CHECK(hipStreamSynchronize(g_les->stream));
+  }
   *dptr = g_les->f[id];
   *elem_bytes = g_les->fbytes[id];
   return 0;
@@ -1093,7 +1139,31 @@ int les_timers(double* t, int reset) {
   if (reset) {
     for (int i = 0; i < 5; ++i) L->phase_ms[i] = 0;
     for (int i = 0; i < KT_COUNT; ++i) { L->kt_ms[i] = 0; L->kt_calls[i] = 0; L->kt_bytes[i] = 0; }
+    L->nkst = 0;
   }
+  return 0;
+}
+
+int les_kernel_stat(int i, int* count, char* name, int name_cap, double* v) {
+  Les* L = g_les;
+  if (!L) return 1;
+  if (count) *count = L->nkst;
+  if (i < 0 && !name && !v) return 0;
+  if (i < 0 || i >= L->nkst) { les_set_error("les_kernel_stat: index %d of %d", i, L->nkst); return 10; }
+  const Les::KStat& k = L->kst[i];
+  if (name && name_cap > 0) {
+    auto tn = [](int b) { return b == 8 ? "double" : b == 4 ? "float" : b == 2 ? "bf16" : "?"; };
+    char buf[160];
+    if (k.ta && k.tb) snprintf(buf, sizeof(buf), "%s<%s, %s>", k.base, tn(k.ta), tn(k.tb));
+    else if (k.ta) snprintf(buf, sizeof(buf), "%s<%s>", k.base, tn(k.ta));
+    else snprintf(buf, sizeof(buf), "%s", k.base);
+    if (k.tag) {
+      const size_t n = std::strlen(buf);
+      snprintf(buf + n, sizeof(buf) - n, " [%s]", k.tag);
+    }
+    snprintf(name, (size_t)name_cap, "%s", buf);
+  }
+  if (v) { v[0] = k.ms; v[1] = (double)k.calls; v[2] = k.bytes; v[3] = (double)k.fam; }
   return 0;
 }
 

@@ -162,7 +162,7 @@ static void lasd_update(Les& L, int jtt, int nlev) {
   ia.np = L.np; ia.nz = L.nz; ia.nlev = nlev;
   kt_begin(L, KT_POINTWISE);
   hipLaunchKernelGGL((k_lasd_inputs<T>), dim3(nblk(npt)), dim3(LES_BLOCK), 0, L.stream, ia);
-  kt_end(L, KT_POINTWISE, (double)npt * (3 * 8.0 + 22.0 * sizeof(T)));
+  kt_end(L, KT_POINTWISE, (double)npt * (3 * 8.0 + 22.0 * sizeof(T)), "k_lasd_inputs", sizeof(T));
   // test filters at 2 delta and 4 delta from one forward transform per field
   const T* g2 = sizeof(T) == 8 ? (const T*)L.g2_64 : (const T*)L.g2_32;
   const T* g4 = sizeof(T) == 8 ? (const T*)L.g4_64 : (const T*)L.g4_32;
@@ -205,7 +205,7 @@ static void lasd_update(Les& L, int jtt, int nlev) {
     ga.np = L.np; ga.nz = L.nz; ga.nlev = nlev;
     kt_begin(L, KT_POINTWISE);
     hipLaunchKernelGGL((k_germano<T>), dim3(nblk(npt)), dim3(LES_BLOCK), 0, L.stream, ga);
-    kt_end(L, KT_POINTWISE, (double)npt * 23.0 * sizeof(T));
+    kt_end(L, KT_POINTWISE, (double)npt * 23.0 * sizeof(T), "k_germano", sizeof(T));
   }
   // Lagrangian interpolation, relaxation and coefficients; F_* double buffered
   for (int f = 0; f < 4; ++f) {
@@ -216,8 +216,10 @@ static void lasd_update(Les& L, int jtt, int nlev) {
   if (nlev < L.nz + 1) {
     const size_t off = (size_t)(nlev + 1) * L.np;
     const size_t cnt = (size_t)(L.nz + 1 - nlev) * L.np;
+    kt_begin(L, KT_POINTWISE);
     for (int f = 0; f < 4; ++f)
       hipMemcpyAsync(D(L, LES_F_LM + f) + off, L.Fold[f] + off, 8 * cnt, hipMemcpyDeviceToDevice, L.stream);
+    kt_end(L, KT_POINTWISE, 4.0 * 16 * cnt, "hipMemcpyAsync F_* above the band");
   }
   LagrArgs<T> la;
   la.ulag = F<T>(L, LES_U_LAG); la.vlag = F<T>(L, LES_V_LAG); la.wlag = F<T>(L, LES_W_LAG);
@@ -233,12 +235,15 @@ static void lasd_update(Les& L, int jtt, int nlev) {
   la.halo_mode = L.P.halo_mode;
   kt_begin(L, KT_LAGR);
   hipLaunchKernelGGL((k_lagr<T>), dim3(nblk(npt)), dim3(LES_BLOCK), 0, L.stream, la);
-  kt_end(L, KT_LAGR, (double)npt * (3.0 * sizeof(T) + 4 * 8.0 + 4.0 * sizeof(T) + 4 * 8.0 + 2.0 * sizeof(T)));
+  kt_end(L, KT_LAGR, (double)npt * (3.0 * sizeof(T) + 4 * 8.0 + 4.0 * sizeof(T) + 4 * 8.0 + 2.0 * sizeof(T)), "k_lagr",
+         sizeof(T));
   L.F_coeff_init = 1;
   const size_t tb = sizeof(T) * (size_t)L.nf;
+  kt_begin(L, KT_POINTWISE);
   hipMemsetAsync(L.f[LES_U_LAG], 0, tb, L.stream);
   hipMemsetAsync(L.f[LES_V_LAG], 0, tb, L.stream);
   hipMemsetAsync(L.f[LES_W_LAG], 0, tb, L.stream);
+  kt_end(L, KT_POINTWISE, 3.0 * tb, "hipMemsetAsync u_lag");
 }
 
 // ---------------------------------------------------------------- plane-averaged dynamic model
@@ -342,7 +347,7 @@ static void dynsmag_update(Les& L) {
   ia.np = L.np; ia.nz = L.nz; ia.nlev = nlev;
   kt_begin(L, KT_POINTWISE);
   hipLaunchKernelGGL((k_lasd_inputs<T>), dim3(nblk(npt)), dim3(LES_BLOCK), 0, L.stream, ia);
-  kt_end(L, KT_POINTWISE, (double)npt * (3 * 8.0 + 22.0 * sizeof(T)));
+  kt_end(L, KT_POINTWISE, (double)npt * (3 * 8.0 + 22.0 * sizeof(T)), "k_lasd_inputs", sizeof(T));
   const T* g2 = sizeof(T) == 8 ? (const T*)L.g2_64 : (const T*)L.g2_32;
   T* fl[21];
   for (int f = 0; f < 21; ++f) fl[f] = (T*)L.wk_real[15 + f];
@@ -371,11 +376,11 @@ static void dynsmag_update(Les& L) {
   ga.np = L.np; ga.nz = L.nz; ga.nlev = nlev;
   kt_begin(L, KT_POINTWISE);
   hipLaunchKernelGGL((k_germano<T>), dim3(nblk(npt)), dim3(LES_BLOCK), 0, L.stream, ga);
-  kt_end(L, KT_POINTWISE, (double)npt * 23.0 * sizeof(T));
+  kt_end(L, KT_POINTWISE, (double)npt * 23.0 * sizeof(T), "k_germano", sizeof(T));
   DynPlaneArgs<T> pa;
   pa.LM = ga.AB; pa.MM = ga.BB; pa.cs = F<T>(L, LES_CS_OPT2);
   pa.np = L.np; pa.npl = (double)(L.nx * L.ny);
   kt_begin(L, KT_POINTWISE);
   hipLaunchKernelGGL((k_dynsmag_plane<T>), dim3(nlev), dim3(256), 0, L.stream, pa);
-  kt_end(L, KT_POINTWISE, (double)npt * 3.0 * sizeof(T));
+  kt_end(L, KT_POINTWISE, (double)npt * 3.0 * sizeof(T), "k_dynsmag_plane", sizeof(T));
 }

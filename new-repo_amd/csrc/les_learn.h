@@ -135,7 +135,9 @@ static void learned_coeffs(Les& L, void* cs, void* beta, int eb, int k0) {
   const long nsel = (long)nlev * L.np;
   SgsArgs<T> a = sgs_args<T>(L);
   T* rr[3] = {(T*)L.wk_real[0], (T*)L.wk_real[1], (T*)L.wk_real[2]};
+  kt_begin(L, KT_MLP);
   hipLaunchKernelGGL((k_rij<T>), dim3(nblk(npt)), dim3(LES_BLOCK), 0, L.stream, a, rr[0], rr[1], rr[2]);
+  kt_end(L, KT_MLP, (double)npt * 9.0 * sizeof(T), "k_rij", sizeof(T));
   const T* in9[9] = {F<T>(L, LES_S11), F<T>(L, LES_S12), F<T>(L, LES_S13), F<T>(L, LES_S22),
                      F<T>(L, LES_S23), F<T>(L, LES_S33), rr[0], rr[1], rr[2]};
   T* f2[9];
@@ -170,7 +172,9 @@ static void learned_coeffs(Les& L, void* cs, void* beta, int eb, int k0) {
   fa.stride = 1; fa.nsel = nsel; fa.p0 = p0;
   kt_begin(L, KT_MLP);
   hipLaunchKernelGGL(k_features, dim3(nblk(nsel)), dim3(LES_BLOCK), 0, L.stream, fa);
+  kt_end(L, KT_MLP, (double)nsel * (27.0 * sizeof(T) + 32.0), "k_features");
   if (cs) {
+    kt_begin(L, KT_MLP);
     InferArgs ia;
     ia.X = M->X; ia.Wb = M->wb; ia.bias = M->w + MLP_NMAT;
     ia.ymean0 = M->hn[24]; ia.ymean1 = M->hn[25]; ia.ystd0 = M->hn[26]; ia.ystd1 = M->hn[27];
@@ -178,8 +182,8 @@ static void learned_coeffs(Les& L, void* cs, void* beta, int eb, int k0) {
     ia.cs = cs; ia.beta = beta; ia.elem_bytes = eb;
     ia.raw_out = nullptr; ia.write_fields = 1; ia.p0 = p0;
     hipLaunchKernelGGL(k_mlp_infer, dim3(2048), dim3(256), 0, L.stream, ia);
+    kt_end(L, KT_MLP, (double)nsel * (32.0 + 2.0 * eb), "k_mlp_infer");
   }
-  kt_end(L, KT_MLP, (double)nsel * (27.0 * sizeof(T) + 2.0 * eb));
 }
 
 // relative L2 distance of a from b over levels k0..nz+1 (values above clip
@@ -191,7 +195,9 @@ static double reduce_rel(Les& L, const void* a, const void* b, int ea, int eb, i
   DiffArgs d;
   d.a = a; d.b = b; d.ea = ea; d.eb = eb; d.off = (long)k0 * L.np; d.n = npt; d.clip = clip; d.part = M->red;
   const unsigned nb = nblk(npt);
+  kt_begin(L, KT_MLP);
   hipLaunchKernelGGL(k_diff_norm, dim3(nb), dim3(LES_BLOCK), 0, L.stream, d);
+  kt_end(L, KT_MLP, (double)npt * (ea + eb), "k_diff_norm");
   std::vector<double> h(2 * nb);
   hipMemcpyAsync(h.data(), M->red, 16 * (size_t)nb, hipMemcpyDeviceToHost, L.stream);
   hipStreamSynchronize(L.stream);
@@ -237,8 +243,10 @@ static void learned_update(Les& L, int jtt) {
       void* tcs = L.wk_real[57];
       void* tbe = L.wk_real[58];
       const size_t bytes = sizeof(T) * (size_t)L.nf;
+      kt_begin(L, KT_MLP);
       hipMemcpyAsync(tcs, L.f[LES_CS_OPT2], bytes, hipMemcpyDeviceToDevice, L.stream);
       hipMemcpyAsync(tbe, L.f[LES_BETA], bytes, hipMemcpyDeviceToDevice, L.stream);
+      kt_end(L, KT_MLP, 4.0 * bytes, "hipMemcpyAsync Cs^2, beta (judge)");
       learned_coeffs<T>(L, tcs, tbe, sizeof(T), k0);
       L.judge[0] = reduce_rel(L, tcs, L.f[LES_CS_OPT2], sizeof(T), sizeof(T), k0, 0.0);
       L.judge[1] = reduce_rel(L, tbe, L.f[LES_BETA], sizeof(T), sizeof(T), k0, 8.0);
@@ -251,9 +259,11 @@ static void learned_update(Les& L, int jtt) {
   learned_coeffs<T>(L, L.f[LES_CS_OPT2], L.f[LES_BETA], sizeof(T), k0);
   if (L.nband == 0) {
     const size_t tb = sizeof(T) * (size_t)L.nf;
+    kt_begin(L, KT_POINTWISE);
     hipMemsetAsync(L.f[LES_U_LAG], 0, tb, L.stream);
     hipMemsetAsync(L.f[LES_V_LAG], 0, tb, L.stream);
     hipMemsetAsync(L.f[LES_W_LAG], 0, tb, L.stream);
+    kt_end(L, KT_POINTWISE, 3.0 * tb, "hipMemsetAsync u_lag");
   }
 }
 

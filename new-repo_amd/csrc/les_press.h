@@ -146,7 +146,7 @@ __global__ void __launch_bounds__(64) k_tridiag(TriArgs a) {
 // rhs_f) on load (u, v, w themselves are left as they are; the projection
 // forms the same predictor again).
 template <typename T>
-static void press_stag_array(Les& L, double tadv1, bool ab2 = false, double tadv2 = 0.0) {
+static void press_stag_array(Les& L, double tadv1, bool ab2 = false, double tadv2 = 0.0, bool fused = false) {
   const int nz = L.nz;
   const double c = 1.0 / (double)(L.nx * L.ny);
   Cplx<double>* Hx = (Cplx<double>*)L.pspec[0];
@@ -183,12 +183,14 @@ static void press_stag_array(Les& L, double tadv1, bool ab2 = false, double tadv
   t.dz = L.dz; t.idz2 = 1.0 / (L.dz * L.dz);
   kt_begin(L, KT_TRIDIAG);
   hipLaunchKernelGGL(k_tridiag, dim3(nblk(2 * L.ns, 64)), dim3(64), 0, L.stream, t);
-  kt_end(L, KT_TRIDIAG, 16.0 * L.ns * (3.0 * nz + 2.0 + (nz + 2.0)));
+  kt_end(L, KT_TRIDIAG, 16.0 * L.ns * (3.0 * nz + 2.0 + (nz + 2.0)), "k_tridiag");
   // gradient: i kx p, i ky p and p back to physical space (src/compute_pressure.f90:320-357)
   cols<double>(L, COP_PGRAD, ph, nullptr, Hx, Hy, Hz, 3, 0, 1, 0, nz + 2);
+  if (fused) return;  // les_step: the projection is the inverse x transform's store (press_project)
   const Cplx<double>* cs[3] = {Hx, Hy, Hz};
   double* outs[3] = {D(L, LES_DPDX), D(L, LES_DPDY), D(L, LES_P)};
   rows_c2r<double, double>(L, cs, outs, 3, 0, nz + 2);
+  L.gradp_lazy = 0;
 }
 
 // dpdz, projection, rhs -= grad p, w BCs and set_bc
@@ -246,7 +248,170 @@ __global__ void k_proj(ProjArgs a) {
   }
 }
 
+// ---- the step's projection, fused with the pressure-gradient transforms.
+// The x and y components are projected as the store of the inverse x
+// transform of i kx phat and i ky phat (blockIdx.y 0 and 1); blockIdx.y 2
+// stores p.  The gradient then never goes through HBM: dpdx, dpdy and dpdz
+// are formed again from phat and p only when the host asks for them
+// (refresh_gradp).  The arithmetic per value is k_proj's.
+struct ProjEpiArgs {
+  double *vel, *r;          // u or v, rhsx or rhsy (null for the p component)
+  const double *rf, *lbc;   // rhs_f, lbc_u or lbc_v
+  double* p;                // p (component 2)
+  int neu;                  // top BC Neumann
+  double ubc;
+};
+struct ProjEpi3 {
+  ProjEpiArgs c[3];
+  double dt, t1, t2, dtt, dz;
+  long np;
+  int nz;
+};
+
+__device__ __forceinline__ void proj_epi_point(const ProjEpi3& e, const ProjEpiArgs& a, int k, long q, double g) {
+  const long c = (long)k * e.np + q;
+  if (a.p) {
+    a.p[c] = g;
+    return;
+  }
+  if (k < 1 || k > e.nz) return;
+  const double rc = a.r[c];
+  double us = a.vel[c];
+  us = us + e.dt * (e.t1 * rc + e.t2 * a.rf[c]);
+  const double un = us - e.dtt * g;
+  a.r[c] = rc - g;
+  a.vel[c] = un;
+  if (k == 1) a.vel[q] = un - a.lbc[q] * e.dz;
+  if (k == e.nz) a.vel[c + e.np] = a.neu ? un + a.ubc * e.dz : 2.0 * a.ubc - un;
+}
+
+__global__ void __launch_bounds__(LES_BLOCK) k_rows_c2r_proj(RowC2RArgs<double, double> a, ProjEpi3 ep, FftPlan pl,
+                                                             const Cplx<double>* __restrict__ tw) {
+  const ProjEpiArgs& e = ep.c[blockIdx.y];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int lane = threadIdx.x & (LES_WAVE - 1), wave = threadIdx.x / LES_WAVE;
+  Cplx<double>* z = reinterpret_cast<Cplx<double>*>(smem_raw) + wave * a.ls;
+  const long job = (long)blockIdx.x * (blockDim.x / LES_WAVE) + wave;
+  const int npairs = a.ny / 2;
+  if (job >= (long)a.nplanes * npairs) return;
+  const int k = a.k0 + (int)(job / npairs), jp = (int)(job % npairs);
+  if (pl.n == 256) {
+    TwReg<double, 256> tr;
+    tw_load(tr, tw + pl.twf, lane);
+    rows_c2r_line(a, z, lane, k, jp, [&](Cplx<double>* p, bool inv) {
+      wave_fft_t<double, 256, row_layout<double>(256)>(p, tr, inv, lane);
+    });
+  } else {
+    rows_c2r_line(a, z, lane, k, jp, [&](Cplx<double>* p, bool inv) { wave_fft(p, pl, tw, inv, lane); });
+  }
+  const long q0 = (long)(2 * jp) * a.nx;
+  for (int n = lane; n < a.nx; n += LES_WAVE) {
+    const Cplx<double> v = z[n];
+    proj_epi_point(ep, e, k, q0 + n, v.x);
+    proj_epi_point(ep, e, k, q0 + a.nx + n, v.y);
+  }
+}
+
+// w: dpdz from p, projection, rhsz -= dpdz, the w BCs and p(nz+1) = BOGUS
+// (k_proj's arithmetic for the vertical component)
+struct ProjWArgs {
+  double *w, *rz, *p;
+  const double* fz;
+  double dt, t1, t2, dtt, cz, ubc_w, bogus;
+  long np;
+  int nz;
+};
+
+__global__ void k_proj_w(ProjWArgs a) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.np * a.nz) return;
+  const int k = 1 + (int)(t / a.np);
+  const long q = t % a.np;
+  const long c = (long)k * a.np + q;
+  const double gz = a.cz * (a.p[c] - a.p[c - a.np]);
+  const double rzc = a.rz[c];
+  double ws = a.w[c];
+  ws = ws + a.dt * (a.t1 * rzc + a.t2 * a.fz[c]);
+  double wn = ws - a.dtt * gz;
+  a.rz[c] = rzc - gz;
+  if (k == 1) {
+    wn = 0.0;
+    a.w[q] = 0.0;
+  }
+  a.w[c] = wn;
+  if (k == a.nz) {
+    a.w[c + a.np] = a.ubc_w;
+    a.p[c + a.np] = a.bogus;
+  }
+}
+
+// les_step's projection (AB2 predictor formed on load): after
+// press_stag_array(..., ab2 = true), whose last pass leaves i kx phat,
+// i ky phat and phat y-transformed in the pressure workspaces.
+static void press_project(Les& L, double tadv1, double tadv2) {
+  const int nz = L.nz;
+  RowC2RArgs<double, double> ca;
+  for (int f = 0; f < 3; ++f) { ca.src[f] = (const Cplx<double>*)L.pspec[f]; ca.dst[f] = nullptr; }
+  ca.nx = L.nx; ca.ny = L.ny; ca.lhx = L.lhx; ca.k0 = 0; ca.nplanes = nz + 2;
+  ca.ls = row_line<double>(L.nx);
+  ProjEpi3 ep;
+  ep.dt = L.P.dt; ep.t1 = tadv1; ep.t2 = tadv2; ep.dtt = L.P.dt * tadv1; ep.dz = L.dz;
+  ep.np = L.np; ep.nz = nz;
+  const int vel[2] = {LES_U, LES_V}, rr[2] = {LES_RHSX, LES_RHSY}, rf[2] = {LES_RHSX_F, LES_RHSY_F};
+  const int lbc[2] = {LES_LBC_U, LES_LBC_V};
+  for (int f = 0; f < 2; ++f) {
+    ep.c[f].vel = D(L, vel[f]); ep.c[f].r = D(L, rr[f]); ep.c[f].rf = D(L, rf[f]);
+    ep.c[f].lbc = L.layer[lbc[f]]; ep.c[f].p = nullptr;
+    ep.c[f].neu = f == 0 ? L.P.ubc_u_neu : L.P.ubc_v_neu;
+    ep.c[f].ubc = f == 0 ? L.P.ubc_u : L.P.ubc_v;
+  }
+  ep.c[2] = ProjEpiArgs{nullptr, nullptr, nullptr, nullptr, D(L, LES_P), 0, 0.0};
+  const long jobs = (long)ca.nplanes * (L.ny / 2);
+  const size_t sh = (size_t)kWavesPerBlock * ca.ls * sizeof(Cplx<double>);
+  kt_begin(L, KT_ROWS_C2R);
+  hipLaunchKernelGGL(k_rows_c2r_proj, dim3((unsigned)((jobs + kWavesPerBlock - 1) / kWavesPerBlock), 3),
+                     dim3(LES_BLOCK), sh, L.stream, ca, ep, L.px, twpool<double>(L));
+  // spectra in; p out; u, v, rhs, rhs_f in and u, v, rhs out on levels 1..nz
+  kt_end(L, KT_ROWS_C2R, 3.0 * (nz + 2) * L.ny * ((double)L.lhx * 16) + 8.0 * L.np * (nz + 2) + 2.0 * 8 * L.np * nz * 5,
+         "k_rows_c2r_proj");
+  ProjWArgs w;
+  w.w = D(L, LES_W); w.rz = D(L, LES_RHSZ); w.p = D(L, LES_P); w.fz = D(L, LES_RHSZ_F);
+  w.dt = L.P.dt; w.t1 = tadv1; w.t2 = tadv2; w.dtt = L.P.dt * tadv1; w.cz = 1.0 / L.dz;
+  w.ubc_w = L.P.ubc_w; w.bogus = BOGUS; w.np = L.np; w.nz = nz;
+  kt_begin(L, KT_POINTWISE);
+  hipLaunchKernelGGL(k_proj_w, dim3(nblk(L.np * nz)), dim3(LES_BLOCK), 0, L.stream, w);
+  // p (levels 0..nz), w, rhsz, rhsz_f in; w, rhsz out
+  kt_end(L, KT_POINTWISE, 8.0 * L.np * nz * 6.0, "k_proj_w");
+  L.gradp_lazy = 1;
+}
+
+// dpdx, dpdy (levels 0..nz+1) and dpdz (levels 1..nz) of the last step,
+// from its phat (the pressure workspace keeps it until the next pressure
+// solve) and p: the values the unfused projection stores, bit for bit.
+static __global__ void k_dpdz(const double* p, double* dpdz, long np, int nz, double cz) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= np * nz) return;
+  const long c = np + t;
+  dpdz[c] = cz * (p[c] - p[c - np]);
+}
+
+static void refresh_gradp(Les& L) {
+  if (!L.gradp_lazy) return;
+  const int nz = L.nz;
+  Cplx<double>* Hx = (Cplx<double>*)L.pspec[0];
+  Cplx<double>* Hy = (Cplx<double>*)L.pspec[1];
+  Cplx<double>* Hz = (Cplx<double>*)L.pspec[2];
+  cols<double>(L, COP_PGRAD, (Cplx<double>*)L.pspec[3], nullptr, Hx, Hy, Hz, 3, 0, 1, 0, nz + 2);
+  const Cplx<double>* cs[2] = {Hx, Hy};
+  double* outs[2] = {D(L, LES_DPDX), D(L, LES_DPDY)};
+  rows_c2r<double, double>(L, cs, outs, 2, 0, nz + 2);
+  hipLaunchKernelGGL(k_dpdz, dim3(nblk(L.np * nz)), dim3(LES_BLOCK), 0, L.stream, (const double*)D(L, LES_P),
+                     D(L, LES_DPDZ), L.np, nz, 1.0 / L.dz);
+  L.gradp_lazy = 0;
+}
+
 static void projection(Les& L, double tadv1, bool ab2 = false, double tadv2 = 0.0) {
+  refresh_gradp(L);
   ProjArgs a;
   a.ab2 = ab2 ? 1 : 0;
   a.fx = D(L, LES_RHSX_F); a.fy = D(L, LES_RHSY_F); a.fz = D(L, LES_RHSZ_F);
@@ -261,5 +426,5 @@ static void projection(Les& L, double tadv1, bool ab2 = false, double tadv2 = 0.
   a.ubc_u = L.P.ubc_u; a.ubc_v = L.P.ubc_v; a.ubc_w = L.P.ubc_w; a.bogus = BOGUS;
   kt_begin(L, KT_POINTWISE);
   hipLaunchKernelGGL(k_proj, dim3(nblk(L.np * L.nz)), dim3(LES_BLOCK), 0, L.stream, a);
-  kt_end(L, KT_POINTWISE, 8.0 * L.np * L.nz * (16.0 + (ab2 ? 3.0 : 0.0)));
+  kt_end(L, KT_POINTWISE, 8.0 * L.np * L.nz * (16.0 + (ab2 ? 3.0 : 0.0)), "k_proj");
 }

@@ -87,6 +87,11 @@ __global__ void k_set_planes(SetPlaneArgs s) {
 #ifndef LES_COLRES16_FP64
 #define LES_COLRES16_FP64 1
 #endif
+// the kernel launch_cols_resample runs for the column lengths h1 -> h2
+static const char* colres_name(int h1, int h2) {
+  return (LES_COLRES16 && ((h1 == 256 && h2 == 384) || (h1 == 384 && h2 == 256))) ? "k_colres_c16" : "k_cols_resample";
+}
+
 template <typename T>
 static void launch_cols_resample(Les& L, ColResArgs<T>& a, int nf, const FftPlan& p1, const FftPlan& p2) {
   if constexpr (LES_COLRES16 && (sizeof(T) == 4 || LES_COLRES16_FP64)) {
@@ -167,7 +172,7 @@ static void convection_nxny(Les& L, double* cx, double* cy, double* cz) {
     a.c = c; a.np = np; a.nlev = nz + 1;
     kt_begin(L, KT_POINTWISE);
     hipLaunchKernelGGL((k_conv_sources<T>), dim3(nblk(np * (nz + 1)), 6), dim3(LES_BLOCK), 0, L.stream, a);
-    kt_end(L, KT_POINTWISE, (double)np * (nz + 1) * (3.0 * 8 + 8.0 * sizeof(T) + 6.0 * sizeof(T)));
+    kt_end(L, KT_POINTWISE, (double)np * (nz + 1) * (3.0 * 8 + 8.0 * sizeof(T) + 6.0 * sizeof(T)), "k_conv_sources", sizeof(T));
   }
   const T* ga = sizeof(T) == 8 ? (const T*)L.ga_64 : (const T*)L.ga_32;
   const T* gb = sizeof(T) == 8 ? (const T*)L.gb_64 : (const T*)L.gb_32;
@@ -180,7 +185,7 @@ static void convection_nxny(Les& L, double* cx, double* cy, double* cz) {
     a.nbp = np; a.nz = nz; a.c2 = c;
     kt_begin(L, KT_POINTWISE);
     hipLaunchKernelGGL((k_conv_prod<T, T>), dim3(nblk(np * nz)), dim3(LES_BLOCK), 0, L.stream, a);
-    kt_end(L, KT_POINTWISE, 9.0 * np * nz * sizeof(T));
+    kt_end(L, KT_POINTWISE, 9.0 * np * nz * sizeof(T), "k_conv_prod", sizeof(T), sizeof(T), "nx x ny");
     double* out[3] = {cx, cy, cz};
     spectral_multiply<T, double>(L, pr, out, 3, nz, gb);
   } else {
@@ -190,11 +195,13 @@ static void convection_nxny(Les& L, double* cx, double* cy, double* cz) {
     a.nbp = np; a.nz = nz; a.c2 = c;
     kt_begin(L, KT_POINTWISE);
     hipLaunchKernelGGL((k_conv_prod<T, double>), dim3(nblk(np * nz)), dim3(LES_BLOCK), 0, L.stream, a);
-    kt_end(L, KT_POINTWISE, np * nz * (6.0 * sizeof(T) + 3.0 * 8));
+    kt_end(L, KT_POINTWISE, np * nz * (6.0 * sizeof(T) + 3.0 * 8), "k_conv_prod", sizeof(T), 8, "nx x ny");
   }
   SetPlaneArgs sp;
   sp.a = cx; sp.b = cy; sp.c = cz; sp.np = np; sp.k0 = 0; sp.k1 = nz + 1; sp.val = BOGUS;
+  kt_begin(L, KT_POINTWISE);
   hipLaunchKernelGGL(k_set_planes, dim3(nblk(np)), dim3(LES_BLOCK), 0, L.stream, sp);
+  kt_end(L, KT_POINTWISE, 6.0 * 8 * np, "k_set_planes");
 }
 
 template <typename T>
@@ -223,7 +230,8 @@ static void convection(Les& L, double* cx, double* cy, double* cz) {
     kt_begin(L, KT_ROWS_RES);
     hipLaunchKernelGGL((k_rows_resample<T, T, T>), dim3((unsigned)((jobs + kWavesPerBlock - 1) / kWavesPerBlock), 6),
                        dim3(LES_BLOCK), sh, L.stream, a, L.px, L.px2, twpool<T>(L));
-    kt_end(L, KT_ROWS_RES, (double)a.nplanes * L.np * (3.0 * 8 + 6.0 * sizeof(T)) + 6.0 * a.nplanes * nx2 * ny * sizeof(T));
+    kt_end(L, KT_ROWS_RES, (double)a.nplanes * L.np * (3.0 * 8 + 6.0 * sizeof(T)) + 6.0 * a.nplanes * nx2 * ny * sizeof(T),
+           "k_rows_resample", sizeof(T), sizeof(T), "pad");
   }
   // pad in y
   {
@@ -233,7 +241,7 @@ static void convection(Les& L, double* cx, double* cy, double* cz) {
     a.src_plane = (long)nx2 * ny; a.dst_plane = L.nbp;
     kt_begin(L, KT_COLS_RES);
     launch_cols_resample<T>(L, a, 6, L.py, L.py2);
-    kt_end(L, KT_COLS_RES, 6.0 * a.nplanes * nx2 * (double)(ny + ny2) * sizeof(T));
+    kt_end(L, KT_COLS_RES, 6.0 * a.nplanes * nx2 * (double)(ny + ny2) * sizeof(T), colres_name(ny, ny2), sizeof(T), 0, "pad");
   }
   // products
   {
@@ -245,7 +253,7 @@ static void convection(Les& L, double* cx, double* cy, double* cz) {
     a.c2 = (T)(1.0 / (double)(nx2 * ny2));
     kt_begin(L, KT_POINTWISE);
     hipLaunchKernelGGL((k_conv_prod<T, T>), dim3(nblk(L.nbp * nz)), dim3(LES_BLOCK), 0, L.stream, a);
-    kt_end(L, KT_POINTWISE, 9.0 * L.nbp * nz * sizeof(T));
+    kt_end(L, KT_POINTWISE, 9.0 * L.nbp * nz * sizeof(T), "k_conv_prod", sizeof(T), sizeof(T), "padded");
   }
   // truncate in y
   {
@@ -255,7 +263,7 @@ static void convection(Les& L, double* cx, double* cy, double* cz) {
     a.src_plane = L.nbp; a.dst_plane = (long)nx2 * ny;
     kt_begin(L, KT_COLS_RES);
     launch_cols_resample<T>(L, a, 3, L.py2, L.py);
-    kt_end(L, KT_COLS_RES, 3.0 * a.nplanes * nx2 * (double)(ny + ny2) * sizeof(T));
+    kt_end(L, KT_COLS_RES, 3.0 * a.nplanes * nx2 * (double)(ny + ny2) * sizeof(T), colres_name(ny2, ny), sizeof(T), 0, "truncate");
   }
   // truncate in x, straight into rhs
   {
@@ -272,11 +280,14 @@ static void convection(Les& L, double* cx, double* cy, double* cz) {
     kt_begin(L, KT_ROWS_RES);
     hipLaunchKernelGGL((k_rows_resample<T, T, double>), dim3((unsigned)((jobs + kWavesPerBlock - 1) / kWavesPerBlock), 3),
                        dim3(LES_BLOCK), sh, L.stream, a, L.px2, L.px, twpool<T>(L));
-    kt_end(L, KT_ROWS_RES, 3.0 * nz * ny * ((double)nx2 * sizeof(T) + (double)nx * 8));
+    kt_end(L, KT_ROWS_RES, 3.0 * nz * ny * ((double)nx2 * sizeof(T) + (double)nx * 8), "k_rows_resample", sizeof(T),
+           sizeof(T), "truncate");
   }
   SetPlaneArgs sp;
   sp.a = cx; sp.b = cy; sp.c = cz; sp.np = L.np; sp.k0 = 0; sp.k1 = nz + 1; sp.val = BOGUS;
+  kt_begin(L, KT_POINTWISE);
   hipLaunchKernelGGL(k_set_planes, dim3(nblk(L.np)), dim3(LES_BLOCK), 0, L.stream, sp);
+  kt_end(L, KT_POINTWISE, 6.0 * 8 * L.np, "k_set_planes");
 }
 
 // rhs(k) += d/dx tx + d/dy ty + d/dz tz for k = 1..nz, plus the forcing of
@@ -385,7 +396,8 @@ static void rows_c2r_div(Les& L, const Cplx<T>* const* src, const DivEpi3<T>& ep
   hipLaunchKernelGGL((k_rows_c2r_div<T>), dim3((unsigned)((jobs + kWavesPerBlock - 1) / kWavesPerBlock), nf),
                      dim3(LES_BLOCK), sh, L.stream, ca, ep, L.px, twpool<T>(L));
   kt_end(L, KT_ROWS_C2R, (double)nf * (L.nz + 1) * L.ny * ((double)L.lhx * sizeof(Cplx<T>)) +
-                             (double)L.np * L.nz * (nf * (16.0 + sizeof(T)) + extra));
+                             (double)L.np * L.nz * (nf * (16.0 + sizeof(T)) + extra),
+         "k_rows_c2r_div", sizeof(T));
 }
 
 template <typename T>

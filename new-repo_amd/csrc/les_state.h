@@ -14,6 +14,8 @@ enum {
   KT_ROWS_R2C = 0, KT_ROWS_C2R, KT_COLS, KT_ROWS_RES, KT_COLS_RES, KT_POINTWISE, KT_TRIDIAG, KT_LAGR, KT_MLP, KT_COUNT
 };
 
+#define LES_MAX_KSTAT 96
+
 struct Les {
   les_params P;
   int nx, ny, nz, nzg, lhx, nx2, ny2;
@@ -71,6 +73,8 @@ struct Les {
   double judge[4];        // rel. L2 of Cs^2 and beta vs the plain procedure, -, step
   int nband;              // levels 1..nband (z / delta < learned_zmin) keep the plain procedure
   int ncu;                // compute units of the device (persistent grids)
+  int dev;                // the HIP device current at les_init
+  int gradp_lazy;         // dpdx, dpdy, dpdz not stored by the last step (formed on request)
 
   // running statistics (allocated by les_stat_init) and diagnostics
   double* st_sum;         // LES_NSTAT sums of nx*ny*nz
@@ -89,6 +93,16 @@ struct Les {
   long kt_calls[KT_COUNT];
   hipEvent_t ph_ev[6];
   double phase_ms[5];
+  // per kernel (one entry per kernel instance and launch site name)
+  struct KStat {
+    const char* base;     // kernel name (string literal)
+    int ta, tb;           // element sizes of its template precisions (0: none)
+    const char* tag;      // variant (string literal or null)
+    int fam;
+    double ms, bytes;
+    long calls;
+  } kst[LES_MAX_KSTAT];
+  int nkst;
   std::vector<void*> allocs;
 };
 

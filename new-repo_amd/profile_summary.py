@@ -69,6 +69,42 @@ def _per_dispatch(d: str, counter: str):
     return [(k[1], v) for k, v in vals.items()]
 
 
+def rocprof_key(name: str) -> str:
+    """rocprofv3's kernel name -> the key of the library's per-kernel
+    statistics: the kernel name with its floating-point template arguments
+    (at most two), without other template arguments or the parameter list:
+    'void k_cols_c16<float, 1, 2, 2>(ColArgs<float>, ...)' ->
+    'k_cols_c16<float>', 'k_proj(ProjArgs)' -> 'k_proj'."""
+    n = name.strip()
+    if n.startswith("void "):
+        n = n[5:]
+    depth, cut = 0, len(n)
+    for i, ch in enumerate(n):
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        elif ch == "(" and depth == 0:
+            cut = i
+            break
+    n = n[:cut]
+    if "<" not in n:
+        return n
+    base, args = n.split("<", 1)
+    parts, depth, cur = [], 0, ""
+    for ch in args[:-1]:
+        if ch == "," and depth == 0:
+            parts.append(cur.strip())
+            cur = ""
+            continue
+        depth += ch == "<"
+        depth -= ch == ">"
+        cur += ch
+    parts.append(cur.strip())
+    types = [a for a in parts if a in ("float", "double")][:2]
+    return base + (f"<{', '.join(types)}>" if types else "")
+
+
 # kernel-name patterns of the benchmark's kernel families
 FAMILY_KERNELS = {"rows_r2c": r"\bk_rows_r2c<", "rows_c2r": r"\bk_rows_c2r(_div)?<",
                   "cols": r"\bk_cols(_256|_c16)?<", "rows_resample": r"\bk_rows_resample<",
@@ -113,6 +149,33 @@ def traffic(args):
             "traffic_bytes_per_launch": tb / len(f),
             "algorithmic_bytes_per_launch": fam[name]["bytes"] / calls,
             "traffic_over_algorithmic": tb / fam[name]["bytes"]}
+    # per kernel: the library's per-kernel statistics of the same launches,
+    # aggregated by key (all variants of one kernel at one precision)
+    lib = {}
+    for name, v in bench["variants"][args.variant].get("kernels", {}).items():
+        if not name.startswith("k_"):
+            continue
+        key = name.split(" [")[0]
+        e = lib.setdefault(key, {"calls": 0, "bytes": 0.0, "ms": 0.0})
+        e["calls"] += v["calls"]
+        e["bytes"] += v["bytes"]
+        e["ms"] += v["ms"]
+    out["kernels"] = {}
+    for key, e in sorted(lib.items(), key=lambda kv: -kv[1]["ms"]):
+        f = [x for n, x in fd if rocprof_key(n) == key]
+        w = [x for n, x in wd if rocprof_key(n) == key]
+        if not f or not w:
+            continue
+        fb, wb = sum(f) * sf * 1024.0, sum(w) * sw * 1024.0
+        out["kernels"][key] = {
+            "dispatches": [len(f), len(w)], "calls": e["calls"],
+            "fetch_bytes_per_launch": fb / len(f), "write_bytes_per_launch": wb / len(w),
+            "traffic_bytes_per_launch": fb / len(f) + wb / len(w),
+            "algorithmic_bytes_per_launch": e["bytes"] / e["calls"],
+            "traffic_over_algorithmic": (fb / len(f) + wb / len(w)) / (e["bytes"] / e["calls"]),
+            "hip_event_avg_ms": e["ms"] / e["calls"],
+            "traffic_GBs": (fb / len(f) + wb / len(w)) / (e["ms"] / e["calls"] * 1e-3) / 1e9,
+            "algorithmic_GBs": e["bytes"] / (e["ms"] * 1e-3) / 1e9}
     return out
 
 

@@ -996,8 +996,10 @@ static void compute_rhs(Oracle& o) {
       o.rhsx[a] = o.rhsx[a] + o.P.pfx;
       o.rhsy[a] = o.rhsy[a] + o.P.pfy;
       if (o.P.coriolis) {
-        o.rhsx[a] = o.rhsx[a] + o.P.coriol * o.v[a] - o.P.coriol * o.P.vg;
-        o.rhsy[a] = o.rhsy[a] - o.P.coriol * o.u[a] + o.P.coriol * o.P.ug;
+        const double fs = PLANT(39) ? -1.0 : 1.0;
+        const double ug = PLANT(40) ? o.P.vg : o.P.ug, vg = PLANT(40) ? o.P.ug : o.P.vg;
+        o.rhsx[a] = o.rhsx[a] + fs * o.P.coriol * o.v[a] - o.P.coriol * vg;
+        o.rhsy[a] = o.rhsy[a] - fs * o.P.coriol * o.u[a] + o.P.coriol * ug;
       }
     }
 }

@@ -652,3 +652,29 @@ def pin_wall_law_log_law(lib):
 
 
 PINS.update({f.__name__[4:]: f for f in (pin_wall_inst_unfiltered, pin_wall_law_log_law)})
+
+
+# ---------------------------------------------------------------- Coriolis forcing
+# src/LES_engine.f90:545-562: rhs_x += pfx + f v - f vg, rhs_y += pfy - f u + f ug.
+# A uniform velocity (no vorticity, no stress) leaves only the forcing in the
+# right-hand side of levels 1..nz.
+
+def pin_coriolis_forcing(lib):
+    f, ug, vg, pfx, pfy = 2.0 ** -7, 10.0, 3.0, 1.0, 0.5
+    prm, o = _mk(lib, 16, 16, 6, coriolis=1, coriol=f, ug=ug, vg=vg, pfx=pfx, pfy=pfy)
+    sh = _shape(prm)
+    _zero_state(o, prm)
+    for n in ("txx", "txy", "txz", "tyy", "tyz", "tzz", "rhsx", "rhsy", "rhsz"):
+        o.put(n, np.zeros(sh))
+    U, V = 7.0, -2.0
+    o.put("u", np.full(sh, U))
+    o.put("v", np.full(sh, V))
+    o.op("rhs")
+    nz = prm.nz
+    _close(o.get("rhsx")[1:nz + 1], np.full((nz, prm.ny, prm.nx), pfx + f * V - f * vg), 1e-14, "rhs_x forcing")
+    _close(o.get("rhsy")[1:nz + 1], np.full((nz, prm.ny, prm.nx), pfy - f * U + f * ug), 1e-14, "rhs_y forcing")
+
+
+PINS["coriolis_forcing"] = pin_coriolis_forcing
+PLANTED[39] = ("Coriolis term with the opposite sign", ["coriolis_forcing"])
+PLANTED[40] = ("geostrophic wind components swapped", ["coriolis_forcing"])
