@@ -195,6 +195,17 @@ int les_stat_reset(void);
  * tke = (var_u + var_v + var_w on the uvp node) / 2).  Error 1 before any
  * accumulation. */
 int les_stat_profiles(double* prof);
+/* Allocation-free variant of les_stat_profiles.  It has no counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The same kernels and summation order as les_stat_profiles,
+ * hence the same profiles bit for bit, but the partial sums and the profiles
+ * live in a fixed device array of the library (3.19 MB on every GPU it runs
+ * on, held until the process ends) instead of les_stat_init's buffers; the
+ * running sums themselves still come from les_stat_init.  nz at most 1024,
+ * else status 10.  Calls on one GPU share that array: calls that overlap
+ * (two host threads, or two processes' streams) race on it. */
+int les_stat_profiles_noalloc(double* prof);
 /* compute_running_diagnistics (src/running_diagnostics.f90:80-305) of the
  * current state: out[0] CFL, out[1] MKE, out[2] plane-mean u*, out[3] rms of
  * du/dx + dv/dy + dw/dz over levels 1..nz. */

@@ -96,7 +96,7 @@ def library() -> ctypes.CDLL:
             "les_mlp_train_step": [vp, vp, ctypes.c_long, ctypes.c_double, dp],
             "les_mlp_forward": [vp, ctypes.c_long, vp],
             "les_stat_init": [], "les_stat_reset": [], "les_stat_accumulate": [],
-            "les_stat_download": [dp, dp, ip], "les_stat_profiles": [dp], "les_running_diagnostics": [dp],
+            "les_stat_download": [dp, dp, ip], "les_stat_profiles": [dp], "les_stat_profiles_noalloc": [dp], "les_running_diagnostics": [dp],
         }
         for name, args in sig.items():
             f = getattr(lib, name)
@@ -323,6 +323,13 @@ class LesGpu:
     def stat_profiles(self) -> Dict[str, np.ndarray]:
         out = np.empty((len(PROFILES), self.prm.nz))
         check(self.lib.les_stat_profiles(_dp(out)))
+        return dict(zip(PROFILES, out))
+
+    def stat_profiles_noalloc(self) -> Dict[str, np.ndarray]:
+        """les_stat_profiles_noalloc: the same profiles, partial sums in the
+        library's fixed device array (nothing allocated)."""
+        out = np.empty((len(PROFILES), self.prm.nz))
+        check(self.lib.les_stat_profiles_noalloc(_dp(out)))
         return dict(zip(PROFILES, out))
 
     def running_diagnostics(self) -> Dict[str, float]:

@@ -472,6 +472,35 @@ static void sgs_stress(Les& L, int jt, int jtt) {
 #include "les_learn.h"
 #include "les_stats.h"
 
+// ---------------------------------------------------------------- allocation-free profiles
+// Device array behind les_stat_profiles_noalloc: the per-slice partial sums
+// of k_stat_planes ([nz][LES_PROF_SLICES][PS_N]) followed by the profiles
+// (LES_NPROF x nz), for nz <= LES_NOALLOC_MAX_NZ: 1024 x (16 x 23 + 21) x 8 B
+// = 3.19 MB, held by every GPU the library runs on until the process ends.
+// Host code never passes the name `scratch` to a kernel or copy (in host code
+// it is a stand-in, not a device address): les_init looks up each device's
+// address once with hipGetSymbolAddress and keeps it in g_scratch[device].
+#define LES_NOALLOC_MAX_NZ 1024
+#define LES_NOALLOC_PART ((size_t)LES_NOALLOC_MAX_NZ * LES_PROF_SLICES * PS_N)
+#define LES_NOALLOC_DOUBLES (LES_NOALLOC_PART + (size_t)LES_NOALLOC_MAX_NZ * LES_NPROF)
+static __device__ __align__(256) double scratch[LES_NOALLOC_DOUBLES];
+#define LES_MAX_DEVICES 64
+static double* g_scratch[LES_MAX_DEVICES] = {};
+
+// Address of this device's `scratch` (the device must be current); 0 on success.
+static int scratch_lookup(int dev) {
+  if (dev < 0 || dev >= LES_MAX_DEVICES) { les_set_error("les_init: device id %d out of range", dev); return 10; }
+  if (g_scratch[dev]) return 0;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) {
+    les_set_error("les_init: hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+    return 5;
+  }
+  g_scratch[dev] = (double*)ptr;
+  return 0;
+}
+
 // ---------------------------------------------------------------- mixed-mode Nyquist filter
 // In the mixed-precision mode the velocity stays FP64, so the Nyquist-free
 // velocity of grad_velocity_filt (src/compute_derivatives.f90:105-134) is
@@ -765,6 +794,7 @@ int les_init(const les_params* p) {
   const les_params P = *p;
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) { les_set_error("les_init: no HIP device"); return 5; }
+  if (int rc = scratch_lookup(dev)) return rc;
   for (int n : {P.nx, P.ny}) {
     int m = n;
     while (m % 2 == 0) m /= 2;
@@ -1463,6 +1493,40 @@ int les_stat_profiles(double* prof) {
                      L->st_prof, L->np, L->nz);
   LES_CHECK(hipGetLastError());
   LES_CHECK(hipMemcpyAsync(prof, L->st_prof, sizeof(double) * LES_NPROF * (size_t)L->nz, hipMemcpyDeviceToHost,
+                           L->stream));
+  LES_CHECK(hipStreamSynchronize(L->stream));
+  return 0;
+}
+
+// les_stat_profiles with its partial sums and profiles in this device's
+// `scratch` (address from les_init) instead of les_stat_init's buffers: the
+// same kernels in the same order, so the same values bit for bit.  Nothing
+// is allocated: the launches read the running sums and write the scratch,
+// and one asynchronous copy brings the profiles to `prof`.  The scratch
+// needs no clearing: k_stat_planes writes every partial k_stat_profiles
+// reads, and k_stat_profiles every profile value copied out.
+int les_stat_profiles_noalloc(double* prof) {
+  Les* L = g_les;
+  if (!L) return 1;
+  if (L->nz > LES_NOALLOC_MAX_NZ) {
+    les_set_error("les_stat_profiles_noalloc: nz = %d exceeds the scratch limit of %d levels", L->nz,
+                  LES_NOALLOC_MAX_NZ);
+    return 10;
+  }
+  if (!prof) { les_set_error("les_stat_profiles_noalloc: null output"); return 10; }
+  if (!L->st_sum) { les_set_error("statistics not initialised (les_stat_init)"); return 1; }
+  if (L->st_iters < 1) { les_set_error("no accumulated statistics"); return 1; }
+  double* ptr = g_scratch[L->dev];
+  if (!ptr) { les_set_error("les_stat_profiles_noalloc: no scratch address for device %d", L->dev); return 5; }
+  StatPlaneArgs a;
+  a.s = L->st_sum; a.part = ptr; a.np = L->np; a.n = L->np * L->nz;
+  a.nx = L->nx; a.nz = L->nz; a.nt = (double)L->st_iters;
+  double* dprof = ptr + LES_NOALLOC_PART;
+  hipLaunchKernelGGL(k_stat_planes, dim3(LES_PROF_SLICES, L->nz), dim3(LES_BLOCK), 0, L->stream, a);
+  hipLaunchKernelGGL(k_stat_profiles, dim3(nblk(L->nz, 64)), dim3(64), 0, L->stream, (const double*)ptr, dprof,
+                     L->np, L->nz);
+  LES_CHECK(hipGetLastError());
+  LES_CHECK(hipMemcpyAsync(prof, dprof, sizeof(double) * LES_NPROF * (size_t)L->nz, hipMemcpyDeviceToHost,
                            L->stream));
   LES_CHECK(hipStreamSynchronize(L->stream));
   return 0;
