@@ -654,7 +654,7 @@ static void ab2(Les& L, double tadv1, double tadv2) {
 #endif
 // les_step's projection as the store of the pressure-gradient transforms
 #ifndef LES_FUSED_PROJ
-#define LES_FUSED_PROJ 0
+#define LES_FUSED_PROJ 1
 #endif
 template <typename T>
 static int les_step_t(Les& L) {

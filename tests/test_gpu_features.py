@@ -1,0 +1,60 @@
+"""The surrogate's input features on the GPU (k_rij, the 2-delta / 4-delta
+test filtering of S_ij and R_ij, k_features, reported by les_sgs_samples)
+against the FP64 numpy restatement oracle/features_oracle.py evaluated on
+the same state's velocity gradients (downloaded from the device), over every
+point above the near-wall band and below the lid.  The device computes in
+FP32; the bound is stated per feature in the test."""
+import importlib.util
+
+import numpy as np
+import pytest
+
+from conftest import ORACLE_DIR, load_ic, load_oracle, load_params, load_pkg
+
+pytestmark = pytest.mark.gpu
+
+spec = importlib.util.spec_from_file_location("features_oracle", ORACLE_DIR / "features_oracle.py")
+FO = importlib.util.module_from_spec(spec)
+spec.loader.exec_module(FO)
+
+
+@pytest.mark.parametrize("grid", [(32, 32, 16), (256, 256, 16)])
+def test_features_match_the_restatement(grid):
+    torch = pytest.importorskip("torch")
+    pkg, P, ic, O = load_pkg(), load_params(), load_ic(), load_oracle()
+    nx, ny, nz = grid
+    prm = P.LesParams(nx=nx, ny=ny, nz=nz, precision=P.PREC_LEARNED, dt=1e-4)
+    u, v, w = ic.log_law_ic(nx, ny, nz, prm.lz, prm.z0, 0.1, seed=31)
+    o = O.Oracle(P.LesParams(nx=nx, ny=ny, nz=nz))
+    G2, G4 = o.table("G2"), o.table("G4")
+    dz = prm.lz / nz
+    delta = (prm.lx / nx * prm.ly / ny * dz) ** (1 / 3)
+    npl = nx * ny
+    print(f"features at {grid}", flush=True)
+    with pkg.LesGpu(prm) as g:
+        g.set_ic(u, v, w)
+        g.step(7)
+        npts = npl * (nz + 1)
+        X = torch.zeros((npts, 12), dtype=torch.float32, device="cuda")
+        Y = torch.zeros((npts, 2), dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+        n = g.sgs_samples(X.data_ptr(), Y.data_ptr(), npts, 1)
+        Fg = X[:n].cpu().numpy().astype(np.float64)
+        grads = {name: g.download(name) for name in FO.GRADS}
+    p0 = npts - n
+    assert p0 % npl == 0 and n > 0
+    Fo = FO.features(grads, G2, G4, nz, dz, delta).reshape(npts, 12)[p0:]
+    # levels from the band's top to nz (the lid, where S ~ 0, is left out)
+    keep = slice(0, n - npl)
+    Fg, Fo = Fg[keep], Fo[keep]
+    names = ["log|S|", "RR/SS", "S3", "SR2", "log r2", "RR/SS 2d", "S3 2d", "SR2 2d", "log r4", "RR/SS 4d",
+             "S3 4d", "log z/delta"]
+    for i, nm in enumerate(names):
+        err = np.abs(Fg[:, i] - Fo[:, i])
+        scale = max(np.abs(Fo[:, i]).max(), 1.0)
+        print(f"  f{i:2d} {nm:12s} max |err| {err.max():.2e}  99.9% {np.quantile(err, 0.999):.2e}  "
+              f"rel L2 {np.linalg.norm(err) / (np.linalg.norm(Fo[:, i]) + 1e-30):.2e}", flush=True)
+        # FP32 filtering and invariants: errors of a few float32 ulps of the
+        # feature's scale; ratios at points where S:S is small amplify them
+        assert np.quantile(err, 0.999) < 1e-3 * scale, (nm, np.quantile(err, 0.999))
+        assert np.linalg.norm(err) <= 1e-4 * np.linalg.norm(Fo[:, i]) + 1e-6 * np.sqrt(len(err)), nm
