@@ -196,19 +196,36 @@ def run_variant(pkg, P, ic, precision, steps, warmup, nx, ny, nz, kernel_stats=T
     return res
 
 
-def matched_error(pkg, P, ic, nx, ny, nz, steps, variants, seed=1234, window=100):
+def constant_weights(W):
+    """The surrogate with its last layer zeroed: it predicts the training
+    mean of Cs^2 and beta everywhere (the constant-coefficient model the
+    surrogate replaces above the near-wall band), with the same band,
+    fallback and judge schedule."""
+    c = np.array(W, dtype=np.float32, copy=True)
+    n_w4 = 64 * 12 + 64 + 2 * (64 * 64 + 64)  # offset of W4 in the compact layout
+    c[n_w4:n_w4 + 2 * 64 + 2] = 0.0           # W4 and b4
+    return c
+
+
+def matched_error(pkg, P, ic, nx, ny, nz, steps, variants, seed=1234, window=None):
     """Run the FP64 step and each of `variants` from the same initial field
     for `steps` steps, with the device running averages over the last
-    `window` steps; report for each the largest deviation of the
-    mean-velocity profile (relative) and of the resolved-stress profile
-    (relative to its peak) from the FP64 profiles.  The profiles are the
-    reference's post-processing of its running averages, reduced on the
-    device (les_stat_profiles): mean u on the uvp levels, and the resolved
-    shear stress R13 + C13 (temporal plus dispersive part, i.e. the
-    space-time covariance of u on the w node and w)."""
-    precs = {"fp64": P.PREC_FP64, "mixed": P.PREC_MIXED, "learned": P.PREC_LEARNED}
+    `window` steps (default: the last fifth, at least 100); report for each
+    the largest and the rms deviation of the mean-velocity profile
+    (relative) and of the resolved-stress profile (relative to its peak)
+    from the FP64 profiles.  The profiles are the reference's
+    post-processing of its running averages, reduced on the device
+    (les_stat_profiles): mean u on the uvp levels, and the resolved shear
+    stress R13 + C13 (temporal plus dispersive part, i.e. the space-time
+    covariance of u on the w node and w).  Variant 'constant' is the learned
+    mode with constant_weights: what the surrogate gains over it is the
+    difference of their deviations."""
+    precs = {"fp64": P.PREC_FP64, "mixed": P.PREC_MIXED, "learned": P.PREC_LEARNED, "constant": P.PREC_LEARNED}
+    if window is None:
+        window = max(100, steps // 5)
     window = min(window, steps)
     prof = {}
+    t_all = time.perf_counter()
     for name in ["fp64"] + list(variants):
         prm = P.LesParams(nx=nx, ny=ny, nz=nz, dt=DT, precision=precs[name], judge_every=50)
         u, v, w = ic.log_law_ic(nx, ny, nz, prm.lz, prm.z0, 0.10, seed=seed)
@@ -216,19 +233,31 @@ def matched_error(pkg, P, ic, nx, ny, nz, steps, variants, seed=1234, window=100
             g.set_ic(u, v, w)
             if name == "learned":
                 g.mlp_set_weights(load_weights())
-            g.step(steps - window)
+            elif name == "constant":
+                g.mlp_set_weights(constant_weights(load_weights()))
+            left = steps - window
+            while left > 0:
+                n = min(left, 1000)
+                g.step(n)
+                left -= n
+                g.sync()
+                print(f"# profiles: {name} step {g.jt_total} ({time.perf_counter() - t_all:.0f} s)", file=sys.stderr,
+                      flush=True)
             g.stat_init()
-            for _ in range(window):
+            for i in range(window):
                 g.step(1)
                 g.stat_accumulate()
+                if (i + 1) % 1000 == 0:
+                    g.sync()
+                    print(f"# profiles: {name} step {g.jt_total}", file=sys.stderr, flush=True)
             if g.check_nan():
                 prof[name] = None
                 continue
             pr = g.stat_profiles()
             prof[name] = (pr["u"], pr["R13"] + pr["C13"])
-            if name == "learned":
+            if name in ("learned", "constant"):
                 js = g.judge_stats()
-                prof["judge"] = {"cs2_rel_l2": js[0], "beta_rel_l2": js[1], "step": int(js[3])}
+                prof["judge_" + name] = {"cs2_rel_l2": js[0], "beta_rel_l2": js[1], "step": int(js[3])}
         print(f"# profiles: {name} done", file=sys.stderr, flush=True)
     out = {"steps": steps, "dt": DT, "criterion": 0.02, "window": [steps - window + 1, steps],
            "profiles": "device running averages (les_stat_profiles): mean u, resolved R13 + C13",
@@ -242,10 +271,12 @@ def matched_error(pkg, P, ic, nx, ny, nz, steps, variants, seed=1234, window=100
             out[name] = {"error": "diverged"}
             continue
         U1, R1 = prof[name]
-        out[name] = {"mean_u_max_rel_dev": float(np.max(np.abs(U1 - U0) / np.abs(U0))),
-                     "resolved_stress_max_dev_rel_peak": float(np.max(np.abs(R1 - R0)) / np.max(np.abs(R0)))}
-        if name == "learned":
-            out[name]["judge"] = prof["judge"]
+        du, dr = np.abs(U1 - U0) / np.abs(U0), np.abs(R1 - R0) / np.max(np.abs(R0))
+        out[name] = {"mean_u_max_rel_dev": float(du.max()), "mean_u_rms_rel_dev": float(np.sqrt(np.mean(du ** 2))),
+                     "resolved_stress_max_dev_rel_peak": float(dr.max()),
+                     "resolved_stress_rms_dev_rel_peak": float(np.sqrt(np.mean(dr ** 2)))}
+        if "judge_" + name in prof:
+            out[name]["judge"] = prof["judge_" + name]
     return out
 
 
@@ -325,6 +356,8 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-configs", action="store_true", help="skip the other workloads' throughput legs")
     ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--profile-variants", default="mixed,learned",
+                    help="variants compared with FP64 in the profile run (mixed, learned, constant)")
     ap.add_argument("--profile-steps", type=int, default=500,
                     help="steps of the matched-error profile comparison (0 to skip)")
     ap.add_argument("--no-kernel-stats", action="store_true", help="skip the per-launch HIP-event pass")
@@ -425,7 +458,7 @@ def main(argv=None):
         out["judge_amortisation"] = hr["judge_amortisation"]
     if "fp64" in results and "mpts" in results["fp64"]:
         out["speedup_vs_fp64"] = hr["mpts"] / results["fp64"]["mpts"]
-    others = [n for n in ("mixed", "learned") if n in ok]
+    others = [n for n in args.profile_variants.split(",") if n in ok or n == "constant"]
     if args.profile_steps > 0 and others:
         try:
             out["matched_error"] = matched_error(pkg, P, ic, nx, ny, nz, args.profile_steps, others)

@@ -139,8 +139,8 @@ __global__ void k_lagr(LagrArgs<T> a) {
 // compute_LagScaleDep_coeff on an update step (src/compute_LagScaleDep.f90:237-292)
 // on levels 1..nlev (nlev = nz + 1: the whole column).  The procedure is
 // local in z apart from the Lagrangian gather, which reads F_* one level
-// above: with nlev < nz + 1 the levels above keep their F_* (copied into the
-// new buffer), so level nlev gathers from the last full update.  The physical
+// above: with nlev < nz + 1 the levels above keep their F_*, so level nlev
+// gathers from the last full update.  The physical
 // nz still decides the boundary semantics (halo planes, beta = 1 at nz + 1).
 template <typename T>
 static void lasd_update(Les& L, int jtt, int nlev) {
@@ -207,19 +207,24 @@ static void lasd_update(Les& L, int jtt, int nlev) {
     hipLaunchKernelGGL((k_germano<T>), dim3(nblk(npt)), dim3(LES_BLOCK), 0, L.stream, ga);
     kt_end(L, KT_POINTWISE, (double)npt * 23.0 * sizeof(T), "k_germano", sizeof(T));
   }
-  // Lagrangian interpolation, relaxation and coefficients; F_* double buffered
-  for (int f = 0; f < 4; ++f) {
-    void* tmp = L.f[LES_F_LM + f];
-    L.f[LES_F_LM + f] = (void*)L.Fold[f];
-    L.Fold[f] = (double*)tmp;
-  }
+  // Lagrangian interpolation, relaxation and coefficients.  The gather reads
+  // the previous F_* one level around the point, so the update reads one
+  // buffer and writes the other: the whole column swaps the two buffers;
+  // the near-wall band (nlev < nz + 1) updates F_* in place after copying the
+  // levels its gather reads (0..nlev+1) to the second buffer, and the levels
+  // above keep their values.
   if (nlev < L.nz + 1) {
-    const size_t off = (size_t)(nlev + 1) * L.np;
-    const size_t cnt = (size_t)(L.nz + 1 - nlev) * L.np;
+    const size_t cnt = (size_t)(nlev + 2) * L.np;
     kt_begin(L, KT_POINTWISE);
     for (int f = 0; f < 4; ++f)
-      hipMemcpyAsync(D(L, LES_F_LM + f) + off, L.Fold[f] + off, 8 * cnt, hipMemcpyDeviceToDevice, L.stream);
-    kt_end(L, KT_POINTWISE, 4.0 * 16 * cnt, "hipMemcpyAsync F_* above the band");
+      hipMemcpyAsync(L.Fold[f], D(L, LES_F_LM + f), 8 * cnt, hipMemcpyDeviceToDevice, L.stream);
+    kt_end(L, KT_POINTWISE, 4.0 * 16 * cnt, "hipMemcpyAsync F_* band");
+  } else {
+    for (int f = 0; f < 4; ++f) {
+      void* tmp = L.f[LES_F_LM + f];
+      L.f[LES_F_LM + f] = (void*)L.Fold[f];
+      L.Fold[f] = (double*)tmp;
+    }
   }
   LagrArgs<T> la;
   la.ulag = F<T>(L, LES_U_LAG); la.vlag = F<T>(L, LES_V_LAG); la.wlag = F<T>(L, LES_W_LAG);

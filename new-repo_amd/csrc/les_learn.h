@@ -123,6 +123,12 @@ static int mlp_forward_backward(Les& L, const short* X, const float* Y, long n, 
   return 0;
 }
 
+// Features and inference in one kernel (k_feat_infer) or in two
+// (k_features writing bf16 features, k_mlp_infer reading them).
+#ifndef LES_FUSED_FEAT
+#define LES_FUSED_FEAT 1
+#endif
+
 // features + inference for the current gradients (S_ij already written) on
 // levels k0..nz+1; cs/beta targets of elem size eb
 template <typename T>
@@ -170,6 +176,18 @@ static void learned_coeffs(Les& L, void* cs, void* beta, int eb, int k0) {
   fa.xmean = M->norm; fa.xstd = M->norm + 12; fa.xlim = M->hn[28];
   fa.X = M->X; fa.raw = nullptr; fa.tgt = nullptr; fa.cs = nullptr; fa.beta = nullptr; fa.elem_bytes = 4;
   fa.stride = 1; fa.nsel = nsel; fa.p0 = p0;
+  if (LES_FUSED_FEAT && cs) {
+    InferArgs ia;
+    ia.X = nullptr; ia.Wb = M->wb; ia.bias = M->w + MLP_NMAT;
+    ia.ymean0 = M->hn[24]; ia.ymean1 = M->hn[25]; ia.ystd0 = M->hn[26]; ia.ystd1 = M->hn[27];
+    ia.n = npt; ia.np = L.np; ia.nz = L.nz;
+    ia.cs = cs; ia.beta = beta; ia.elem_bytes = eb;
+    ia.raw_out = nullptr; ia.write_fields = 1; ia.p0 = p0;
+    kt_begin(L, KT_MLP);
+    hipLaunchKernelGGL(k_feat_infer, dim3(2048), dim3(256), 0, L.stream, fa, ia);
+    kt_end(L, KT_MLP, (double)nsel * (27.0 * sizeof(T) + 2.0 * eb), "k_feat_infer");
+    return;
+  }
   kt_begin(L, KT_MLP);
   hipLaunchKernelGGL(k_features, dim3(nblk(nsel)), dim3(LES_BLOCK), 0, L.stream, fa);
   kt_end(L, KT_MLP, (double)nsel * (27.0 * sizeof(T) + 32.0), "k_features");

@@ -50,14 +50,15 @@ __device__ __forceinline__ float gelu_s(float x) {
   return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(x * p));
 }
 __device__ __forceinline__ float gelu(float x) { return x * gelu_s(x); }
-// GELU of four accumulators as four bf16 operand values: two
-// v_cvt_pk_bf16_f32 (round to nearest even), each packing a pair into one
-// register (the compiler otherwise converts one value per instruction and
-// packs with v_perm).
+// GELU of four accumulators as four bf16 operand values: a float pair
+// converted as one vector (round to nearest even), which gfx950 does with
+// one v_cvt_pk_bf16_f32 per pair; the compiler sees the conversion, so it
+// schedules it (and any hazard around the MFMAs) itself.
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f2v __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ unsigned cvt_pk_bf16(float a, float b) {
-  unsigned r;
-  __asm__("v_cvt_pk_bf16_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
-  return r;
+  const f2v v = {a, b};
+  return __builtin_bit_cast(unsigned, __builtin_convertvector(v, bf16x2));
 }
 __device__ __forceinline__ s4v gelu_bf4(f4v v) {
   typedef unsigned u2v __attribute__((ext_vector_type(2)));
@@ -224,17 +225,16 @@ __device__ __forceinline__ void store_coef(const InferArgs& a, long p, double cs
   }
 }
 
-__global__ void __launch_bounds__(256) k_mlp_infer(InferArgs a) {
-  const int l = threadIdx.x & 63;
-  const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const long nwaves = ((long)gridDim.x * blockDim.x) >> 6;
-  const int r = l & 15, q = (l >> 4) * 4;
-  // Weights as A-operand fragments W[16 rb + r][16 cb + q + j], j = 0..3, in
-  // LDS in lane order (one 8-byte read per fragment): blocks 0-3 W1 (rb),
-  // 4-19 W2 (rb, cb), 20-35 W3, 36-39 W4 (cb).  Biases in LDS too: read per
-  // tile instead of occupying registers, which doubles the waves per SIMD.
-  __shared__ __align__(16) s4v sw[40 * 64];
-  __shared__ __align__(16) float sb[MLP_NBIAS];
+// Weights as A-operand fragments W[16 rb + r][16 cb + q + j], j = 0..3, in
+// LDS in lane order (one 8-byte read per fragment): blocks 0-3 W1 (rb),
+// 4-19 W2 (rb, cb), 20-35 W3, 36-39 W4 (cb).  Biases in LDS too: read per
+// tile instead of occupying registers, which doubles the waves per SIMD.
+struct MlpLds {
+  s4v sw[40 * 64];
+  float sb[MLP_NBIAS];
+};
+
+__device__ __forceinline__ void mlp_lds_fill(MlpLds& m, const short* Wb, const float* bias) {
   for (int idx = threadIdx.x; idx < 40 * 64; idx += blockDim.x) {
     const int blk = idx >> 6, ln = idx & 63;
     const int rr = ln & 15, qq = (ln >> 4) * 4;
@@ -244,20 +244,73 @@ __global__ void __launch_bounds__(256) k_mlp_infer(InferArgs a) {
     else if (blk < 36) base = MW3 + (16 * ((blk - 20) >> 2) + rr) * 64 + 16 * ((blk - 20) & 3) + qq;
     else base = MW4 + rr * 64 + 16 * (blk - 36) + qq;
     s4v v;
-    for (int j = 0; j < 4; ++j) v[j] = a.Wb[base + j];
-    sw[idx] = v;
+    for (int j = 0; j < 4; ++j) v[j] = Wb[base + j];
+    m.sw[idx] = v;
   }
-  for (int i = threadIdx.x; i < MLP_NBIAS; i += blockDim.x) sb[i] = a.bias[i];
-  __syncthreads();
-  const s4v* wl = sw + l;
+  for (int i = threadIdx.x; i < MLP_NBIAS; i += blockDim.x) m.sb[i] = bias[i];
+}
+
+// One 16-point tile: x holds features q..q+3 of point r (lane l = 16 (q / 4)
+// + r); lanes 0..15 return outputs 0..3 of point r.
+__device__ __forceinline__ f4v mlp_tile(const MlpLds& m, int l, s4v x) {
+  const int q = (l >> 4) * 4;
+  const s4v* wl = m.sw + l;
   const s4v* w1 = wl;
   const s4v* w2 = wl + 4 * 64;
   const s4v* w3 = wl + 20 * 64;
   const s4v* w4 = wl + 36 * 64;
-  const f4v* bb1 = reinterpret_cast<const f4v*>(sb + MB1 + q);
-  const f4v* bb2 = reinterpret_cast<const f4v*>(sb + MB2 + q);
-  const f4v* bb3 = reinterpret_cast<const f4v*>(sb + MB3 + q);
-  const f4v bb4 = *reinterpret_cast<const f4v*>(sb + MB4 + q);
+  const f4v* bb1 = reinterpret_cast<const f4v*>(m.sb + MB1 + q);
+  const f4v* bb2 = reinterpret_cast<const f4v*>(m.sb + MB2 + q);
+  const f4v* bb3 = reinterpret_cast<const f4v*>(m.sb + MB3 + q);
+  const f4v bb4 = *reinterpret_cast<const f4v*>(m.sb + MB4 + q);
+  s4v h1[4], h2[4], h3[4];
+  for (int rb = 0; rb < 4; ++rb) {
+    f4v acc = bb1[4 * rb];
+    acc = mfma16(w1[rb * 64], x, acc);
+    h1[rb] = gelu_bf4(acc);
+  }
+  for (int rb = 0; rb < 4; ++rb) {
+    f4v acc = bb2[4 * rb];
+    for (int cb = 0; cb < 4; ++cb) acc = mfma16(w2[(rb * 4 + cb) * 64], h1[cb], acc);
+    h2[rb] = gelu_bf4(acc);
+  }
+  for (int rb = 0; rb < 4; ++rb) {
+    f4v acc = bb3[4 * rb];
+    for (int cb = 0; cb < 4; ++cb) acc = mfma16(w3[(rb * 4 + cb) * 64], h2[cb], acc);
+    h3[rb] = gelu_bf4(acc);
+  }
+  f4v y = bb4;
+  for (int cb = 0; cb < 4; ++cb) y = mfma16(w4[cb * 64], h3[cb], y);
+  return y;
+}
+
+// The network's outputs of point p (lanes 0..15): raw outputs and/or the
+// coefficients (in-range points only), float32-valued like the dynamic
+// procedure's casts, beta >= 1/8 and 1 on the lid.
+__device__ __forceinline__ void mlp_store(const InferArgs& a, long p, f4v y, bool in_range) {
+  if (a.raw_out) {
+    a.raw_out[p * 2] = y[0];
+    a.raw_out[p * 2 + 1] = y[1];
+  }
+  if (a.write_fields && in_range) {
+    const int k = 1 + (int)(p / a.np);
+    const float c2 = y[0] * a.ystd0 + a.ymean0;
+    float b = y[1] * a.ystd1 + a.ymean1;
+    b = b > 0.125f ? b : 0.125f;
+    if (k == a.nz + 1) b = 1.0f;
+    const float cs = c2 > 1e-24f ? c2 : 1e-24f;
+    store_coef(a, p, (double)cs, (double)b);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_mlp_infer(InferArgs a) {
+  const int l = threadIdx.x & 63;
+  const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const long nwaves = ((long)gridDim.x * blockDim.x) >> 6;
+  const int r = l & 15, q = (l >> 4) * 4;
+  __shared__ __align__(16) MlpLds m;
+  mlp_lds_fill(m, a.Wb, a.bias);
+  __syncthreads();
   const long ntiles = (a.n + 15) / 16;
   for (long tile = a.p0 / 16 + wave; tile < ntiles; tile += nwaves) {
     __asm__ volatile("" ::: "memory");  // re-read the biases from LDS each tile (keeps them out of registers)
@@ -265,42 +318,10 @@ __global__ void __launch_bounds__(256) k_mlp_infer(InferArgs a) {
     s4v x = {0, 0, 0, 0};
     if (p < a.n)
       for (int j = 0; j < 4; ++j) x[j] = a.X[p * 16 + q + j];
-    s4v h1[4], h2[4], h3[4];
-    for (int rb = 0; rb < 4; ++rb) {
-      f4v acc = bb1[4 * rb];
-      acc = mfma16(w1[rb * 64], x, acc);
-      h1[rb] = gelu_bf4(acc);
-    }
-    for (int rb = 0; rb < 4; ++rb) {
-      f4v acc = bb2[4 * rb];
-      for (int cb = 0; cb < 4; ++cb) acc = mfma16(w2[(rb * 4 + cb) * 64], h1[cb], acc);
-      h2[rb] = gelu_bf4(acc);
-    }
-    for (int rb = 0; rb < 4; ++rb) {
-      f4v acc = bb3[4 * rb];
-      for (int cb = 0; cb < 4; ++cb) acc = mfma16(w3[(rb * 4 + cb) * 64], h2[cb], acc);
-      h3[rb] = gelu_bf4(acc);
-    }
-    f4v y = bb4;
-    for (int cb = 0; cb < 4; ++cb) y = mfma16(w4[cb * 64], h3[cb], y);
-    // lanes 0..15 hold outputs 0..3 of point r; the range flag of point r is
-    // element 3 of lane 48 + r's feature registers
+    const f4v y = mlp_tile(m, l, x);
+    // the range flag of point r is element 3 of lane 48 + r's feature registers
     const int flag_bits = __shfl((int)(unsigned short)x[3], 48 + r, 64);
-    if (l < 16 && p < a.n) {
-      if (a.raw_out) {
-        a.raw_out[p * 2] = y[0];
-        a.raw_out[p * 2 + 1] = y[1];
-      }
-      if (a.write_fields && bf2f((short)flag_bits) == 0.0f) {
-        const int k = 1 + (int)(p / a.np);
-        const float c2 = y[0] * a.ystd0 + a.ymean0;
-        float b = y[1] * a.ystd1 + a.ymean1;
-        b = b > 0.125f ? b : 0.125f;
-        if (k == a.nz + 1) b = 1.0f;
-        const float cs = c2 > 1e-24f ? c2 : 1e-24f;
-        store_coef(a, p, (double)cs, (double)b);
-      }
-    }
+    if (l < 16 && p < a.n) mlp_store(a, p, y, bf2f((short)flag_bits) == 0.0f);
   }
 }
 
@@ -377,15 +398,8 @@ __device__ __forceinline__ void invariants(const float* s, const float* rr, floa
   out[3] = SR2 / (mag * SS + 1e-30f);
 }
 
-__global__ void k_features(FeatArgs a) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long n = a.np * (a.nz + 1);
-  long p = a.p0 + t;
-  if (a.raw) {
-    if (t >= a.nsel) return;
-    p = a.p0 + t * a.stride;
-  }
-  if (p >= n) return;
+// the 12 raw features of point p (offset from level 1)
+__device__ __forceinline__ void feat_point(const FeatArgs& a, long p, float* f) {
   const long c = a.np + p;
   const int k = 1 + (int)(p / a.np);
   float inv[3][4];
@@ -396,7 +410,6 @@ __global__ void k_features(FeatArgs a) {
     invariants(s, r, inv[sc]);
   }
   const float z = (k == 1) ? 0.5f * a.dz : (float)(k - 1) * a.dz;
-  float f[12];
   f[0] = logf(inv[0][0] + 1e-12f);
   f[1] = inv[0][1]; f[2] = inv[0][2]; f[3] = inv[0][3];
   f[4] = logf((inv[1][0] + 1e-12f) / (inv[0][0] + 1e-12f));
@@ -404,6 +417,34 @@ __global__ void k_features(FeatArgs a) {
   f[8] = logf((inv[2][0] + 1e-12f) / (inv[0][0] + 1e-12f));
   f[9] = inv[2][1]; f[10] = inv[2][2];
   f[11] = logf(z / a.delta);
+}
+
+// normalised bf16 features of one point, slot 15 = out-of-range flag
+__device__ __forceinline__ void feat_normalise(const FeatArgs& a, const float* f, short* X) {
+  bool out = false;
+  for (int i = 0; i < 12; ++i) {
+    const float x = (f[i] - a.xmean[i]) / a.xstd[i];
+    if (!(fabsf(x) <= a.xlim)) out = true;
+    X[i] = f2bf(x);
+  }
+  X[12] = 0;
+  X[13] = 0;
+  X[14] = 0;
+  X[15] = out ? f2bf(1.0f) : (short)0;
+}
+
+__global__ void k_features(FeatArgs a) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long n = a.np * (a.nz + 1);
+  long p = a.p0 + t;
+  if (a.raw) {
+    if (t >= a.nsel) return;
+    p = a.p0 + t * a.stride;
+  }
+  if (p >= n) return;
+  const long c = a.np + p;
+  float f[12];
+  feat_point(a, p, f);
   if (a.raw) {
     for (int i = 0; i < 12; ++i) a.raw[t * 12 + i] = f[i];
     if (a.tgt) {
@@ -414,16 +455,46 @@ __global__ void k_features(FeatArgs a) {
     }
     return;
   }
-  bool out = false;
-  for (int i = 0; i < 12; ++i) {
-    const float x = (f[i] - a.xmean[i]) / a.xstd[i];
-    if (!(fabsf(x) <= a.xlim)) out = true;
-    a.X[p * 16 + i] = f2bf(x);
+  short x[16];
+  feat_normalise(a, f, x);
+  for (int i = 0; i < 16; ++i) a.X[p * 16 + i] = x[i];
+}
+
+// Features and inference in one pass: each block computes the normalised
+// features of 256 points (one per thread) into LDS, then each of its waves
+// runs the network on 64 of them (four 16-point tiles) and stores the
+// coefficients; the feature loads of the block's other waves overlap the
+// network's arithmetic.  The features never go through HBM.  Same
+// arithmetic as k_features followed by k_mlp_infer.
+__global__ void __launch_bounds__(256) k_feat_infer(FeatArgs fa, InferArgs ia) {
+  __shared__ __align__(16) MlpLds m;
+  __shared__ __align__(16) short sX[256 * 16];
+  mlp_lds_fill(m, ia.Wb, ia.bias);
+  __syncthreads();
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6, r = l & 15, q = (l >> 4) * 4;
+  const long n = fa.np * (fa.nz + 1);
+  const long nchunk = (n - fa.p0 + 255) / 256;
+  for (long ch = blockIdx.x; ch < nchunk; ch += gridDim.x) {
+    const long p0 = fa.p0 + ch * 256;
+    short* xs = sX + threadIdx.x * 16;
+    if (p0 + threadIdx.x < n) {
+      float f[12];
+      feat_point(fa, p0 + threadIdx.x, f);
+      feat_normalise(fa, f, xs);
+    } else {
+      for (int i = 0; i < 16; ++i) xs[i] = 0;
+    }
+    __syncthreads();  // every feature row of the chunk written before any wave reads them
+    for (int t = 0; t < 4; ++t) {
+      __asm__ volatile("" ::: "memory");  // re-read the biases from LDS each tile
+      const int pt = 64 * w + 16 * t + r;
+      s4v x;
+      for (int j = 0; j < 4; ++j) x[j] = sX[pt * 16 + q + j];
+      const f4v y = mlp_tile(m, l, x);
+      if (l < 16 && p0 + pt < n) mlp_store(ia, p0 + pt, y, sX[pt * 16 + 15] == 0);
+    }
+    __syncthreads();  // every wave done with the chunk before it is overwritten
   }
-  a.X[p * 16 + 12] = 0;
-  a.X[p * 16 + 13] = 0;
-  a.X[p * 16 + 14] = 0;
-  a.X[p * 16 + 15] = out ? f2bf(1.0f) : (short)0;
 }
 
 // raw samples -> normalised bf16 features [n][16] and targets [n][16]

@@ -7,11 +7,11 @@
 // algorithm, one thread per mode, with the divergence right-hand side built
 // on the fly (src/compute_pressure.f90:205-216, 256-305, 448-493, 544-619).
 struct TriArgs {
-  const Cplx<double> *Hx, *Hy, *Hz;   // y-transformed spectra, planes 1..nz
+  const Cplx<double> *Hd, *Hz;        // y-transformed spectra, planes 1..nz: i kx Hx + i ky Hy, and Hz
   const Cplx<double> *bot, *top;      // BC layer spectra
   const double *kkx, *kky;
   Cplx<double>* phat;                 // planes 0..nz+1
-  double* gam;                        // (nz+3) * ns scratch, rows k at k * ns
+  double* gam;                        // scratch: the bet each 16-level chunk starts from, rows at chunk * ns
   int lhx, ny, nz;
   long ns;
   double dz, idz2;
@@ -38,8 +38,7 @@ __global__ void __launch_bounds__(64) k_tridiag(TriArgs a) {
   const long ns = a.ns;
   double* __restrict__ ph = reinterpret_cast<double*>(a.phat) + c;   // component c of phat
   double* __restrict__ gam = a.gam;
-  const double* __restrict__ Hxo = reinterpret_cast<const double*>(a.Hx) + (1 - c);
-  const double* __restrict__ Hyo = reinterpret_cast<const double*>(a.Hy) + (1 - c);
+  const double* __restrict__ Hdc = reinterpret_cast<const double*>(a.Hd) + c;
   const double* __restrict__ Hzc = reinterpret_cast<const double*>(a.Hz) + c;
   if (i == a.lhx - 1 || j == a.ny / 2) {
     for (int k = 0; k <= nz + 1; ++k) ph[2 * ((long)k * ns + q)] = 0.0;
@@ -56,34 +55,37 @@ __global__ void __launch_bounds__(64) k_tridiag(TriArgs a) {
   double x = r1 / bet;
   ph[2 * q] = x;
   double hz = Hzc[2 * (ns + q)];
-  // rows 2..nz+1 (interior levels 1..nz): r = i kx Hx + i ky Hy + dHz/dz.
-  // Software-pipelined: the loads of the next chunk of LES_TRI_U levels are
-  // issued before the current chunk's recurrence runs.
-  double hA[3][LES_TRI_U], hB[3][LES_TRI_U];
-  auto fwd_load = [&](double (&h)[3][LES_TRI_U], int k0) {
+  // rows 2..nz+1 (interior levels 1..nz): r = (i kx Hx + i ky Hy) + dHz/dz,
+  // the horizontal divergence formed by the column pass (real part
+  // -(kx Hx.y) - ky Hy.y = -(kx Hx.y + ky Hy.y) exactly; imaginary part
+  // kx Hx.x + ky Hy.x).  Software-pipelined: the loads of the next chunk of
+  // LES_TRI_U levels are issued before the current chunk's recurrence runs.
+  double hA[2][LES_TRI_U], hB[2][LES_TRI_U];
+  auto fwd_load = [&](double (&h)[2][LES_TRI_U], int k0) {
 #pragma unroll
     for (int u = 0; u < LES_TRI_U; ++u) {
       const int k = k0 + u;
       if (k <= nz + 1) {
         const long p = (long)(k - 1) * ns + q;
-        h[0][u] = Hxo[2 * p];
-        h[1][u] = Hyo[2 * p];
-        h[2][u] = (k - 1 < nz) ? Hzc[2 * (p + ns)] : 0.0;
+        h[0][u] = Hdc[2 * p];
+        h[1][u] = (k - 1 < nz) ? Hzc[2 * (p + ns)] : 0.0;
       }
     }
   };
-  auto fwd_step = [&](const double (&h)[3][LES_TRI_U], int k0) {
+  // The factors g_k = c_{k-1} / bet_{k-1} depend on the mode only: instead of
+  // storing them, each chunk saves the bet it starts from (one value per 16
+  // levels; both lanes of the pair store the same value) and the back
+  // substitution recomputes the chunk's g_k with the same operations.
+  auto fwd_step = [&](const double (&h)[2][LES_TRI_U], int k0) {
+    gam[(long)((k0 - 2) / LES_TRI_U) * ns + q] = bet;
 #pragma unroll
     for (int u = 0; u < LES_TRI_U; ++u) {
       const int k = k0 + u;
       if (k <= nz + 1) {
-        // real part: -(kx Hx.y + ky Hy.y) + dHz.x/dz; imaginary: (kx Hx.x + ky Hy.x) + dHz.y/dz
-        const double sk = kx * h[0][u] + ky * h[1][u];
-        const double r = (c == 0 ? -sk : sk) + (h[2][u] - hz) / dz;
-        hz = h[2][u];
+        const double r = h[0][u] + (h[1][u] - hz) / dz;
+        hz = h[1][u];
         const double cprev = (k == 2) ? c1 : idz2;
         const double g = cprev / bet;
-        gam[(long)k * ns + q] = g;  // both lanes of the pair store the same value
         bet = bmid - idz2 * g;
         x = (r - idz2 * x) / bet;
         ph[2 * ((long)(k - 1) * ns + q)] = x;
@@ -99,45 +101,63 @@ __global__ void __launch_bounds__(64) k_tridiag(TriArgs a) {
     fwd_step(hB, k0 + LES_TRI_U);
   }
   // row nz+2: top layer
+  double gtop;
   {
-    const int k = nz + 2;
     const double r = reinterpret_cast<const double*>(a.top)[2 * q + c] * dz;
     const double am = -1.0, bk = 1.0;
-    const double g = idz2 / bet;
-    gam[(long)k * ns + q] = g;
-    bet = bk - am * g;
+    gtop = idz2 / bet;
+    bet = bk - am * gtop;
     x = (r - am * x) / bet;
   }
-  // back substitution: x holds x(nz+2); each lane reads back its own
-  // stores, again with the next chunk's loads issued first
-  double gA[2][LES_TRI_U], gB[2][LES_TRI_U];
-  auto bwd_load = [&](double (&b)[2][LES_TRI_U], int k1) {
+  // back substitution: x holds x(nz+2).  Row nz+1 uses g_{nz+2}; then chunk
+  // c of the forward sweep (g_k, k = 2 + 16c .. 17 + 16c) serves rows k - 1,
+  // its g_k recomputed from the saved bet; each lane reads back its own
+  // stores, with the next chunk's loads issued first.
+  {
+    double* xp = &ph[2 * ((long)nz * ns + q)];
+    x = *xp - gtop * x;
+    *xp = x;
+  }
+  const int nchunk = (nz + LES_TRI_U - 1) / LES_TRI_U;
+  double pA[LES_TRI_U], pB[LES_TRI_U], bA = 0.0, bB = 0.0;
+  auto bwd_load = [&](double (&pv)[LES_TRI_U], double& bs, int ch) {
+    const int k0 = 2 + LES_TRI_U * ch;
 #pragma unroll
     for (int u = 0; u < LES_TRI_U; ++u) {
-      const int k = k1 - u;
-      if (k >= 1) {
-        b[0][u] = gam[(long)(k + 1) * ns + q];
-        b[1][u] = ph[2 * ((long)(k - 1) * ns + q)];
+      const int kk = k0 - 1 + u;  // row
+      if (kk <= nz) pv[u] = ph[2 * ((long)(kk - 1) * ns + q)];
+    }
+    bs = gam[(long)ch * ns + q];
+  };
+  auto bwd_step = [&](const double (&pv)[LES_TRI_U], double bs, int ch) {
+    const int k0 = 2 + LES_TRI_U * ch;
+    double G[LES_TRI_U];
+    double bt = bs;
+#pragma unroll
+    for (int u = 0; u < LES_TRI_U; ++u) {
+      const int k = k0 + u;
+      if (k <= nz + 1) {
+        const double cprev = (k == 2) ? c1 : idz2;
+        G[u] = cprev / bt;
+        bt = bmid - idz2 * G[u];
+      }
+    }
+#pragma unroll
+    for (int u = LES_TRI_U - 1; u >= 0; --u) {
+      const int kk = k0 - 1 + u;  // row kk uses g_{kk+1}
+      if (kk <= nz) {
+        x = pv[u] - G[u] * x;
+        ph[2 * ((long)(kk - 1) * ns + q)] = x;
       }
     }
   };
-  auto bwd_step = [&](const double (&b)[2][LES_TRI_U], int k1) {
-#pragma unroll
-    for (int u = 0; u < LES_TRI_U; ++u) {
-      const int k = k1 - u;
-      if (k >= 1) {
-        x = b[1][u] - b[0][u] * x;
-        ph[2 * ((long)(k - 1) * ns + q)] = x;
-      }
-    }
-  };
-  bwd_load(gA, nz + 1);
-  for (int k1 = nz + 1; k1 >= 1; k1 -= 2 * LES_TRI_U) {
-    if (k1 - LES_TRI_U >= 1) bwd_load(gB, k1 - LES_TRI_U);
-    bwd_step(gA, k1);
-    if (k1 - LES_TRI_U < 1) break;
-    if (k1 - 2 * LES_TRI_U >= 1) bwd_load(gA, k1 - 2 * LES_TRI_U);
-    bwd_step(gB, k1 - LES_TRI_U);
+  bwd_load(pA, bA, nchunk - 1);
+  for (int ch = nchunk - 1; ch >= 0; ch -= 2) {
+    if (ch - 1 >= 0) bwd_load(pB, bB, ch - 1);
+    bwd_step(pA, bA, ch);
+    if (ch - 1 < 0) break;
+    if (ch - 2 >= 0) bwd_load(pA, bA, ch - 2);
+    bwd_step(pB, bB, ch - 1);
   }
   ph[2 * ((long)(nz + 1) * ns + q)] = 0.0;
 }
@@ -164,7 +184,10 @@ static void press_stag_array(Les& L, double tadv1, bool ab2 = false, double tadv
   } else {
     rows_r2c<double, double>(L, src, dst, 3, 1, nz, c / tadv1, LOAD_SCALE_DIV, L.P.dt);
   }
-  for (int f = 0; f < 3; ++f) cols<double>(L, COP_COPY, dst[f], nullptr, dst[f], nullptr, nullptr, 1, 1, 0, 1, nz);
+  // forward y transforms; the horizontal divergence i kx Hx + i ky Hy is
+  // formed by the column pass (into Hx), so the solve reads two spectra
+  cols<double>(L, COP_DIV, Hx, Hy, Hx, nullptr, nullptr, 1, 1, 0, 1, nz);
+  cols<double>(L, COP_COPY, Hz, nullptr, Hz, nullptr, nullptr, 1, 1, 0, 1, nz);
   // bottom and top layers (src/compute_pressure.f90:152-165, 196-203)
   Cplx<double>* lb = (Cplx<double>*)L.playr[0];
   Cplx<double>* lt = (Cplx<double>*)L.playr[1];
@@ -175,7 +198,7 @@ static void press_stag_array(Les& L, double tadv1, bool ab2 = false, double tadv
   cols<double>(L, COP_COPY, lt, nullptr, lt, nullptr, nullptr, 1, 1, 0, 0, 1);
   // tridiagonal solve per mode
   TriArgs t;
-  t.Hx = Hx; t.Hy = Hy; t.Hz = Hz; t.bot = lb; t.top = lt;
+  t.Hd = Hx; t.Hz = Hz; t.bot = lb; t.top = lt;
   t.kkx = L.kkx64; t.kky = L.kky64;
   t.phat = ph;
   t.gam = L.tri_gam;
@@ -183,7 +206,7 @@ static void press_stag_array(Les& L, double tadv1, bool ab2 = false, double tadv
   t.dz = L.dz; t.idz2 = 1.0 / (L.dz * L.dz);
   kt_begin(L, KT_TRIDIAG);
   hipLaunchKernelGGL(k_tridiag, dim3(nblk(2 * L.ns, 64)), dim3(64), 0, L.stream, t);
-  kt_end(L, KT_TRIDIAG, 16.0 * L.ns * (3.0 * nz + 2.0 + (nz + 2.0)), "k_tridiag");
+  kt_end(L, KT_TRIDIAG, 16.0 * L.ns * (2.0 * nz + 2.0 + (nz + 2.0)), "k_tridiag");
   // gradient: i kx p, i ky p and p back to physical space (src/compute_pressure.f90:320-357)
   cols<double>(L, COP_PGRAD, ph, nullptr, Hx, Hy, Hz, 3, 0, 1, 0, nz + 2);
   if (fused) return;  // les_step: the projection is the inverse x transform's store (press_project)

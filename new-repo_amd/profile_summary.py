@@ -106,7 +106,7 @@ def rocprof_key(name: str) -> str:
 
 
 # kernel-name patterns of the benchmark's kernel families
-FAMILY_KERNELS = {"rows_r2c": r"\bk_rows_r2c<", "rows_c2r": r"\bk_rows_c2r(_div)?<",
+FAMILY_KERNELS = {"rows_r2c": r"\bk_rows_r2c<", "rows_c2r": r"\bk_rows_c2r(_div<|_proj\(|<)",
                   "cols": r"\bk_cols(_256|_c16)?<", "rows_resample": r"\bk_rows_resample<",
                   "cols_resample": r"\bk_(cols_resample|colres_c16)<", "tridiag": r"\bk_tridiag",
                   "lagrangian": r"\bk_lagr<"}
@@ -180,42 +180,49 @@ def traffic(args):
 
 
 def report(args):
-    """Markdown kernel-trace report: top kernels, HIP-event family averages
-    of the traced process beside rocprofv3's, and the step figures of a
-    default benchmark line."""
+    """Markdown kernel-trace report: top kernels, the HIP-event per-kernel
+    averages of the traced process beside rocprofv3's (kernels keyed by name
+    and floating-point template arguments), and the step figures of a default
+    benchmark line."""
     text, tab = kernels(args.trace, args.top)
     lines = [ln for ln in pathlib.Path(args.trace_json).read_text().splitlines() if ln.startswith("{")]
     tj = json.loads(lines[-1])
-    fam = tj["kernel_families"]
+    kern = tj.get("kernels") or tj.get("variants", {}).get(args.variant, {}).get("kernels", {})
     bb = json.loads(pathlib.Path(args.bench_json).read_text().splitlines()[-1])
-    # the default run times the families of its headline variant only
-    fam_def = bb.get("kernel_families", {}) if bb.get("config", {}).get("variant") == args.variant else {}
+    lib = {}
+    for name, v in kern.items():
+        if not name.startswith("k_"):
+            continue
+        e = lib.setdefault(name.split(" [")[0], {"ms": 0.0, "calls": 0})
+        e["ms"] += v["ms"]
+        e["calls"] += v["calls"]
+    roc = {}
+    for n, c, tot, avg, pct in tab:
+        e = roc.setdefault(rocprof_key(n), {"ns": 0.0, "calls": 0})
+        e["ns"] += tot
+        e["calls"] += c
     rows = []
-    for f, pat in FAMILY_KERNELS.items():
-        if f in fam:
-            c = sum(t[1] for t in tab if re.search(pat, t[0]))
-            tot = sum(t[2] for t in tab if re.search(pat, t[0]))
-            if c:
-                shown = pat.replace("|", "\\|")  # a literal bar inside a markdown table cell
-                dflt = f"{fam_def[f]['avg_ms']:.4f}" if f in fam_def else "-"
-                rows.append(f"| {f} | `{shown}` | {fam[f]['avg_ms']:.4f} | {dflt} | {tot / c / 1e6:.4f} | {c} |")
+    for key, e in sorted(lib.items(), key=lambda kv: -kv[1]["ms"])[:args.top]:
+        r = roc.get(key)
+        ra = f"{r['ns'] / r['calls'] / 1e6:.4f}" if r else "-"
+        rows.append(f"| `{key}` | {e['ms'] / e['calls']:.4f} | {e['calls']} | {ra} | {r['calls'] if r else '-'} |")
     steps = [f"| {k} | {v['ms_per_step']:.3f} | {v['mpts']:.0f} |" for k, v in bb["variants"].items() if "mpts" in v]
     r = bb["roofline"]
     out = [f"# {args.title}", "", "Commands (repository root, on the MI355X):", "", "```"] + args.command + ["```", "",
            args.workload, "", "## Top kernels (rocprofv3 --stats)", "", text, "",
-           "## HIP-event family averages vs rocprofv3", "",
-           "bench.py times every launch of its 5-step cycle with HIP events on the library's stream (one "
-           "stream, each launch bracketed and synchronised): in the traced process, where the profiler adds its "
-           "dispatch overhead to every bracket, and in the default run without the profiler.  rocprofv3 "
-           "averages all launches of the same kernels over the whole trace, including the timed steps, where "
-           "the SGS stage runs on a second stream beside the convection.", "",
-           "| family | kernels | bench avg ms (traced) | bench avg ms (default run) | rocprof avg ms | rocprof launches |",
-           "|---|---|---|---|---|---|"] + rows + [
+           "## HIP-event per-kernel averages vs rocprofv3", "",
+           "bench.py times every launch of a 5-step cycle with HIP events on the library's stream (one stream, each "
+           "launch bracketed and synchronised).  rocprofv3 averages all launches of the same kernel over the whole "
+           "trace.  Kernels are keyed by name and floating-point template arguments (all launch sites and "
+           "variants of a kernel together).", "",
+           "| kernel | HIP-event avg ms | launches (cycle) | rocprof avg ms | rocprof launches |",
+           "|---|---|---|---|---|"] + rows + [
            "", "## Default benchmark line (`python3 bench.py`, separate run)", "",
            "| variant | ms/step | Mpt-steps/s |", "|---|---|---|"] + steps + [
-           "", f"Dominant single kernel: `{r['kernel_family']}` at {r['achieved']:.0f} GB/s algorithmic "
-           f"({r['frac']:.3f} of 8 TB/s); whole step {bb['step_roofline']['achieved_GBs']:.0f} GB/s "
-           f"({bb['step_roofline']['frac']:.3f}) at {bb['step_roofline']['bytes_per_point_step']} B per point-step.", ""]
+           "", f"Dominant single kernel: `{r['kernel']}` at {r['achieved']:.0f} GB/s algorithmic "
+           f"({r['frac']:.3f} of 8 TB/s), {r['avg_launch_ms']:.4f} ms per launch; whole step "
+           f"{bb['step_roofline']['achieved_GBs']:.0f} GB/s ({bb['step_roofline']['frac']:.3f}) at "
+           f"{bb['step_roofline']['bytes_per_point_step']} B per point-step.", ""]
     return "\n".join(out)
 
 

@@ -75,6 +75,22 @@ def loss_and_grad(v, X, T, mode="exact"):
     return loss, pack(grads), Y
 
 
+def forward(v, X, mode="bf16", chunk=1 << 16):
+    """The network's outputs (n, 2) for inputs X (n, 12), in chunks of
+    points (the forward half of loss_and_grad)."""
+    r = bf16 if mode == "bf16" else (lambda a: np.asarray(a, dtype=np.float64))
+    W1, b1, W2, b2, W3, b3, W4, b4 = unpack(v)
+    W1r, W2r, W3r, W4r = r(W1), r(W2), r(W3), r(W4)
+    out = np.empty((X.shape[0], 2))
+    for p0 in range(0, X.shape[0], chunk):
+        x = r(X[p0:p0 + chunk])
+        H1 = r(gelu(x @ W1r.T + b1))
+        H2 = r(gelu(H1 @ W2r.T + b2))
+        H3 = r(gelu(H2 @ W3r.T + b3))
+        out[p0:p0 + chunk] = H3 @ W4r.T + b4
+    return out
+
+
 def init_params(seed=0):
     rng = np.random.default_rng(seed)
     parts = []

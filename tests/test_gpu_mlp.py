@@ -163,15 +163,46 @@ def test_band_levels_follow_the_plain_procedure():
     assert not np.array_equal(cs_m[nband + 1:18], cs_l[nband + 1:18])
 
 
-def test_surrogate_update_is_the_network_on_the_step_features():
+def test_fused_inference_with_many_tiles_per_wave():
+    # les_mlp_forward runs 1024 waves; 8195 tiles of 16 points make each wave
+    # loop over 8 tiles (the persistent loop of the in-step launch, whose
+    # 8192 waves loop over ~7 tiles at 256 x 256 x 16 and ~128 at 256^3)
+    torch = pytest.importorskip("torch")
+    n = 4 * 16 * 1024 * 2 + 48
+    rng = np.random.default_rng(12)
+    X = rng.standard_normal((n, 12)).astype(np.float32)
+    v = M.init_params(9).astype(np.float32)
+    print(f"fused inference, {n} points", flush=True)
+    with _ctx() as g:
+        g.mlp_set_weights(np.concatenate([v, _identity_norm()]))
+        dX = torch.from_numpy(X).cuda()
+        dT = torch.zeros((n, 2), dtype=torch.float32, device="cuda")
+        xb = torch.empty((n, 16), dtype=torch.int16, device="cuda")
+        yn = torch.empty((n, 16), dtype=torch.float32, device="cuda")
+        out = torch.empty((n, 2), dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+        g.mlp_prepare(dX.data_ptr(), dT.data_ptr(), n, xb.data_ptr(), yn.data_ptr())
+        g.mlp_forward(xb.data_ptr(), n, out.data_ptr())
+        y = out.cpu().numpy()
+    ref = M.forward(v.astype(np.float64), X.astype(np.float64), "bf16")
+    e = _rel(y, ref)
+    print(f"  rel L2 {e:.2e}", flush=True)
+    assert e < 3e-3
+    # every tile was written (the last, partial one included)
+    assert np.all(np.isfinite(y)) and np.abs(y[-48:] - ref[-48:]).max() < 1e-2 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("grid", [(32, 32, 16), (256, 256, 16)])
+def test_surrogate_update_is_the_network_on_the_step_features(grid):
     # A surrogate coefficient update (features and inference in one kernel)
     # against the CPU forward pass of the network (bf16 roundings where the
     # device rounds) on the features les_sgs_samples reports for the same
     # state: in-range points get the network's Cs^2 and beta, out-of-range
-    # points keep their previous coefficients.
+    # points keep their previous coefficients.  At 256 x 256 x 16 every wave
+    # of the in-step inference loops over several 16-point tiles.
     torch = pytest.importorskip("torch")
     pkg, P, ic = load_pkg(), load_params(), load_ic()
-    prm = P.LesParams(nx=32, ny=32, nz=16, precision=P.PREC_LEARNED, judge_every=1000)
+    prm = P.LesParams(nx=grid[0], ny=grid[1], nz=grid[2], precision=P.PREC_LEARNED, judge_every=1000, dt=1e-4)
     u, v, w = ic.log_law_ic(prm.nx, prm.ny, prm.nz, prm.lz, prm.z0, 0.1, seed=21)
     np_, nz = prm.nx * prm.ny, prm.nz
     print("surrogate update vs CPU forward pass", flush=True)
@@ -203,7 +234,7 @@ def test_surrogate_update_is_the_network_on_the_step_features():
     frac = out.mean()
     print(f"  band of {p0 // np_} levels, {n1} surrogate points, {frac:.3f} out of range", flush=True)
     assert 0.001 < frac < 0.6
-    _, _, yn = M.loss_and_grad(params.astype(np.float64), Xn.astype(np.float64), np.zeros((n1, 2)), "bf16")
+    yn = M.forward(params.astype(np.float64), Xn.astype(np.float64), "bf16")
     cs_ref = np.maximum(yn[:, 0] * ys[0] + ym[0], 1e-24)
     be_ref = np.maximum(yn[:, 1] * ys[1] + ym[1], 0.125)
     lev = 1 + (p0 + np.arange(n1)) // np_
