@@ -2,6 +2,7 @@
 
 kernels DIR            per-kernel table from <DIR>/**/*kernel_stats.csv
 report DIR             markdown report of a kernel trace (see --help)
+sq DIR                 per-kernel instruction counts of an SQ counter pass
 traffic FETCH WRITE    HBM bytes per launch of each kernel family from two
                        counter passes (FETCH_SIZE, WRITE_SIZE, KiB per
                        dispatch), each scaled by what the same counter reads
@@ -226,6 +227,46 @@ def report(args):
     return "\n".join(out)
 
 
+def sq(args):
+    """Per-kernel instruction counts from a counter pass (SQ_INSTS_VALU,
+    _SALU, _LDS, _VMEM_RD, _VMEM_WR, SQ_WAVES) and the VALU issue fraction
+    they imply against the HIP-event launch time of a run without the
+    profiler: VALU instructions x 4 cycles (a wave64 instruction on a
+    16-lane SIMD) / (1024 SIMDs x launch time x clock)."""
+    rows = _rows(args.dir, "counter_collection.csv")
+    agg, disp = {}, {}
+    for r in rows:
+        k = rocprof_key(_col(r, "Kernel_Name"))
+        a = agg.setdefault(k, {})
+        n = _col(r, "Counter_Name").strip()
+        a[n] = a.get(n, 0.0) + float(_col(r, "Counter_Value"))
+        disp.setdefault(k, set()).add(_col(r, "Dispatch_Id", "Correlation_Id"))
+    bench = json.loads(pathlib.Path(args.bench_json).read_text().splitlines()[-1])
+    kern = {}
+    for name, v in bench.get("kernels", {}).items():
+        e = kern.setdefault(name.split(" [")[0], {"ms": 0.0, "calls": 0})
+        e["ms"] += v["ms"]
+        e["calls"] += v["calls"]
+    lines = [f"# {args.title}", "", "Commands (repository root, on the MI355X):", "", "```"] + args.command + [
+        "```", "", f"VALU issue fraction = SQ_INSTS_VALU x 4 / (1024 SIMDs x HIP-event launch time x "
+        f"{args.clock_ghz} GHz); the HIP-event times are those of the default benchmark run (no profiler).", "",
+        "| kernel | dispatches | VALU / launch | SALU / launch | LDS / launch | VMEM / launch | waves | "
+        "HIP-event ms | VALU issue fraction |", "|---|---|---|---|---|---|---|---|---|"]
+    order = sorted(agg, key=lambda k: -(kern.get(k, {}).get("ms", 0.0)))
+    for k in order:
+        if not k.startswith("k_") or k not in kern:
+            continue
+        c, nd = agg[k], len(disp[k])
+        avg = kern[k]["ms"] / kern[k]["calls"]
+        valu = c.get("SQ_INSTS_VALU", 0.0) / nd
+        util = valu * 4 / (1024 * avg * 1e-3 * args.clock_ghz * 1e9)
+        vmem = (c.get("SQ_INSTS_VMEM_RD", 0.0) + c.get("SQ_INSTS_VMEM_WR", 0.0)) / nd
+        lines.append(f"| `{k}` | {nd} | {valu:.3e} | {c.get('SQ_INSTS_SALU', 0.0) / nd:.2e} | "
+                     f"{c.get('SQ_INSTS_LDS', 0.0) / nd:.2e} | {vmem:.2e} | {c.get('SQ_WAVES', 0.0) / nd:.0f} | "
+                     f"{avg:.4f} | {util:.2f} |")
+    return "\n".join(lines) + "\n"
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -251,7 +292,20 @@ def main(argv=None):
     rp.add_argument("--top", type=int, default=30)
     rp.add_argument("--variant", default="learned", help="variant of the traced run")
     rp.add_argument("--out")
+    q = sub.add_parser("sq")
+    q.add_argument("dir")
+    q.add_argument("--bench-json", required=True, help="stdout of a default bench.py run")
+    q.add_argument("--title", required=True)
+    q.add_argument("--command", action="append", default=[])
+    q.add_argument("--clock-ghz", type=float, default=2.4)
+    q.add_argument("--out")
     a = ap.parse_args(argv)
+    if a.cmd == "sq":
+        text = sq(a)
+        if a.out:
+            pathlib.Path(a.out).write_text(text)
+        print(text)
+        return 0
     if a.cmd == "kernels":
         text, _ = kernels(a.dir, a.top)
         print(text)
