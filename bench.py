@@ -458,7 +458,7 @@ def main(argv=None):
         out["judge_amortisation"] = hr["judge_amortisation"]
     if "fp64" in results and "mpts" in results["fp64"]:
         out["speedup_vs_fp64"] = hr["mpts"] / results["fp64"]["mpts"]
-    others = [n for n in args.profile_variants.split(",") if n in ok or n == "constant"]
+    others = [n for n in args.profile_variants.split(",") if n in ("mixed", "learned", "constant")]
     if args.profile_steps > 0 and others:
         try:
             out["matched_error"] = matched_error(pkg, P, ic, nx, ny, nz, args.profile_steps, others)

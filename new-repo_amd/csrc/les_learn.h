@@ -253,11 +253,13 @@ static int lag_levels(const Les& L, int jtt) {
 template <typename T>
 static void learned_update(Les& L, int jtt) {
   const bool ready = L.mlp && L.mlp->ready;
-  const bool judge = !ready || !L.F_coeff_init || (L.P.judge_every > 0 && jtt % L.P.judge_every == 0);
+  const bool first = !L.F_coeff_init;
+  const bool judge = !ready || first || (L.P.judge_every > 0 && jtt % L.P.judge_every == 0);
   const int k0 = L.nband + 1;
   if (judge) {
     lasd_update<T>(L, jtt, L.nz + 1);
     if (ready) {
+      // plain coefficients kept aside for the judge statistics
       void* tcs = L.wk_real[57];
       void* tbe = L.wk_real[58];
       const size_t bytes = sizeof(T) * (size_t)L.nf;
@@ -265,9 +267,26 @@ static void learned_update(Les& L, int jtt) {
       hipMemcpyAsync(tcs, L.f[LES_CS_OPT2], bytes, hipMemcpyDeviceToDevice, L.stream);
       hipMemcpyAsync(tbe, L.f[LES_BETA], bytes, hipMemcpyDeviceToDevice, L.stream);
       kt_end(L, KT_MLP, 4.0 * bytes, "hipMemcpyAsync Cs^2, beta (judge)");
-      learned_coeffs<T>(L, tcs, tbe, sizeof(T), k0);
-      L.judge[0] = reduce_rel(L, tcs, L.f[LES_CS_OPT2], sizeof(T), sizeof(T), k0, 0.0);
-      L.judge[1] = reduce_rel(L, tbe, L.f[LES_BETA], sizeof(T), sizeof(T), k0, 8.0);
+      // The first update's F_* are fresh (initialised from this step's M:M), so
+      // its plain coefficients act everywhere and the surrogate is only
+      // judged.  At later judges the F_* above the band have been relaxed once
+      // in judge_every steps (the surrogate sets those levels in between), so
+      // the plain values there only judge the surrogate and refresh the
+      // fallback: above the band the surrogate acts on in-range points and
+      // out-of-range points take the plain value, limited to the training
+      // range.  (Applying the plain values of those lagging F_* on a judge
+      // step diverged after ~7500 steps at 256^3 with dt = 1e-4, with the
+      // trained surrogate and with a constant one alike; applying the
+      // surrogate there did not.)
+      void* cs = first ? tcs : L.f[LES_CS_OPT2];
+      void* be = first ? tbe : L.f[LES_BETA];
+      learned_coeffs<T>(L, cs, be, sizeof(T), k0);
+      const void* sur_cs = first ? tcs : L.f[LES_CS_OPT2];
+      const void* sur_be = first ? tbe : L.f[LES_BETA];
+      const void* pl_cs = first ? L.f[LES_CS_OPT2] : tcs;
+      const void* pl_be = first ? L.f[LES_BETA] : tbe;
+      L.judge[0] = reduce_rel(L, sur_cs, pl_cs, sizeof(T), sizeof(T), k0, 0.0);
+      L.judge[1] = reduce_rel(L, sur_be, pl_be, sizeof(T), sizeof(T), k0, 8.0);
       L.judge[2] = (double)L.nband;
       L.judge[3] = (double)jtt;
     }

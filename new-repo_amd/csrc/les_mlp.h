@@ -284,22 +284,43 @@ __device__ __forceinline__ f4v mlp_tile(const MlpLds& m, int l, s4v x) {
   return y;
 }
 
+// The surrogate's coefficients stay in the range of its training targets
+// (the plain procedure's values clipped to these bounds; beyond them the
+// plain values sit where |S| is at round-off level or F_LM is clipped).
+#define MLP_CS2_MAX 1.0f
+#define MLP_BETA_MAX 8.0f
+
 // The network's outputs of point p (lanes 0..15): raw outputs and/or the
-// coefficients (in-range points only), float32-valued like the dynamic
-// procedure's casts, beta >= 1/8 and 1 on the lid.
+// coefficients, float32-valued like the dynamic procedure's casts.  In-range
+// points get the network's Cs^2 in [1e-24, MLP_CS2_MAX] and beta in [1/8,
+// MLP_BETA_MAX] (1 on the lid); out-of-range points keep their previous
+// coefficients, limited to the same range: a value the plain procedure gave
+// where |S| was at round-off level must not act for many steps where |S| has
+// grown since (it would make nu_t = Cs^2 delta^2 |S| exceed the explicit
+// step's diffusion limit).
 __device__ __forceinline__ void mlp_store(const InferArgs& a, long p, f4v y, bool in_range) {
   if (a.raw_out) {
     a.raw_out[p * 2] = y[0];
     a.raw_out[p * 2 + 1] = y[1];
   }
-  if (a.write_fields && in_range) {
+  if (!a.write_fields) return;
+  if (in_range) {
     const int k = 1 + (int)(p / a.np);
     const float c2 = y[0] * a.ystd0 + a.ymean0;
     float b = y[1] * a.ystd1 + a.ymean1;
     b = b > 0.125f ? b : 0.125f;
+    b = b < MLP_BETA_MAX ? b : MLP_BETA_MAX;
     if (k == a.nz + 1) b = 1.0f;
-    const float cs = c2 > 1e-24f ? c2 : 1e-24f;
+    float cs = c2 > 1e-24f ? c2 : 1e-24f;
+    cs = cs < MLP_CS2_MAX ? cs : MLP_CS2_MAX;
     store_coef(a, p, (double)cs, (double)b);
+  } else {
+    const long c = a.np + p;
+    const double cs = a.elem_bytes == 8 ? ((const double*)a.cs)[c] : (double)((const float*)a.cs)[c];
+    const double b = a.elem_bytes == 8 ? ((const double*)a.beta)[c] : (double)((const float*)a.beta)[c];
+    if (cs > (double)MLP_CS2_MAX || b > (double)MLP_BETA_MAX)
+      store_coef(a, p, cs < (double)MLP_CS2_MAX ? cs : (double)MLP_CS2_MAX,
+                 b < (double)MLP_BETA_MAX ? b : (double)MLP_BETA_MAX);
   }
 }
 
