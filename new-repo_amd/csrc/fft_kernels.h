@@ -34,6 +34,9 @@ struct RowR2CArgs {
   const TI* ab_r[3];  // LOAD_AB2_SCALE_DIV: rhs and rhs_f of field f
   const TI* ab_f[3];
   T ab_dt, ab_t1, ab_t2;
+  // k_rows_r2c_nyq: the FP64 Nyquist sums of the raw input (row sums A,
+  // column partials P of each band of 8 rows)
+  double *nyq_a, *nyq_p;
 };
 
 // LDS layout of the row transforms: the register-twiddle 256-point
@@ -50,14 +53,23 @@ __host__ __device__ constexpr int row_line(int n) {
 
 // One wavefront per row pair: load, transform and unpack without any
 // block-level barrier.  LDS: one line of a.ls points per wavefront.
-template <typename TI, typename T, typename FFT>
-__device__ __forceinline__ void rows_r2c_body(const RowR2CArgs<TI, T>& a, Cplx<T>* z, int lane, long job, FFT&& fft) {
+template <bool NYQ = false, typename TI, typename T, typename FFT>
+__device__ __forceinline__ void rows_r2c_body(const RowR2CArgs<TI, T>& a, Cplx<T>* z, int lane, long job, FFT&& fft,
+                                              double* part = nullptr) {
   const int nx = a.nx, lhx = a.lhx;
   const int npairs = a.ny / 2;
   const int f = blockIdx.y;
   const int k = a.k0 + (int)(job / npairs), jp = (int)(job % npairs);
   const TI* __restrict__ src = a.src[f] + ((long)k * a.ny + 2 * jp) * nx;
+  double s0 = 0.0, s1 = 0.0;
   for (int n = lane; n < nx; n += LES_WAVE) {
+    if constexpr (NYQ) {
+      // alternating row sums and the pair's column partial (-1)^j f, in FP64
+      const double r0 = src[n], r1 = src[n + nx];
+      s0 += (n & 1) ? -r0 : r0;
+      s1 += (n & 1) ? -r1 : r1;
+      part[n] = r0 - r1;
+    }
     T v0 = (T)src[n], v1 = (T)src[n + nx];
     if (a.mode == LOAD_AB2_SCALE_DIV) {
       const long o = ((long)k * a.ny + 2 * jp) * nx + n;
@@ -74,6 +86,17 @@ __device__ __forceinline__ void rows_r2c_body(const RowR2CArgs<TI, T>& a, Cplx<T
       v1 = a.scale * v1;
     }
     z[n] = {v0, v1};
+  }
+  if constexpr (NYQ) {
+    for (int o = LES_WAVE / 2; o > 0; o >>= 1) {
+      s0 += __shfl_xor(s0, o, LES_WAVE);
+      s1 += __shfl_xor(s1, o, LES_WAVE);
+    }
+    if (lane == 0) {
+      double* A = a.nyq_a + (long)k * a.ny + 2 * jp;
+      A[0] = s0 / (double)nx;
+      A[1] = s1 / (double)nx;
+    }
   }
   wave_sync();
   fft(z, false);
@@ -103,6 +126,42 @@ k_rows_r2c(RowR2CArgs<TI, T> a, FftPlan pl, const Cplx<T>* __restrict__ tw) {
   } else {
     rows_r2c_body(a, z, lane, job, [&](Cplx<T>* p, bool inv) { wave_fft(p, pl, tw, inv, lane); });
   }
+}
+
+// k_rows_r2c with the first pass of the mixed mode's FP64 Nyquist removal
+// folded into the load (the sums k_nyq_reduce forms, from the values this
+// kernel reads anyway): A_j = sum_i (-1)^i f_ij / nx of both rows (the same
+// lane-strided sums and butterfly as k_nyq_reduce), and per block of 4 row
+// pairs the band's column partial sum_j (-1)^j f_ij, the 4 pairs added in
+// order after a block barrier.  The host launches it only when ny % 8 == 0,
+// so every block holds 4 row pairs of one plane and no wavefront leaves
+// early.  LDS: the transform lines, then 4 nx doubles of pair partials.
+// The FP64 sums beside the FP32 transform's registers need 84 VGPRs (5
+// waves per SIMD); held to 64 the kernel spills.
+#ifndef LES_NYQ_WAVES
+#define LES_NYQ_WAVES 5
+#endif
+template <typename T>
+__global__ void __launch_bounds__(LES_BLOCK) __attribute__((amdgpu_waves_per_eu(LES_NYQ_WAVES)))
+k_rows_r2c_nyq(RowR2CArgs<double, T> a, FftPlan pl, const Cplx<T>* __restrict__ tw) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int lane = threadIdx.x & (LES_WAVE - 1), wave = threadIdx.x / LES_WAVE;
+  Cplx<T>* z = reinterpret_cast<Cplx<T>*>(smem_raw) + wave * a.ls;
+  double* part = reinterpret_cast<double*>(reinterpret_cast<Cplx<T>*>(smem_raw) + (LES_BLOCK / LES_WAVE) * a.ls);
+  const long job = (long)blockIdx.x * (blockDim.x / LES_WAVE) + wave;
+  if (pl.n == 256) {
+    TwReg<T, 256> tr;
+    tw_load(tr, tw + pl.twf, lane);
+    rows_r2c_body<true>(a, z, lane, job, [&](Cplx<T>* p, bool inv) { wave_fft_t<T, 256, row_layout<T>(256)>(p, tr, inv, lane); },
+                        part + wave * a.nx);
+  } else {
+    rows_r2c_body<true>(a, z, lane, job, [&](Cplx<T>* p, bool inv) { wave_fft(p, pl, tw, inv, lane); }, part + wave * a.nx);
+  }
+  __syncthreads();
+  const int nx = a.nx;
+  double* __restrict__ P = a.nyq_p + (long)blockIdx.x * nx;
+  for (int i = threadIdx.x; i < nx; i += LES_BLOCK)
+    P[i] = ((part[i] + part[nx + i]) + part[2 * nx + i]) + part[3 * nx + i];
 }
 
 // ------------------------------------------------------------------ row c2r

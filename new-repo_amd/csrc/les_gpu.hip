@@ -9,11 +9,13 @@
 // replaced by strided column passes and by index arithmetic inside kernels.
 #include <hip/hip_runtime.h>
 
+#include <cassert>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "les_state.h"
@@ -167,6 +169,28 @@ static void rows_r2c(Les& L, const TI* const* src, Cplx<T>* const* dst, int nf, 
          "k_rows_r2c", sizeof(TI), sizeof(T), mode == LOAD_AB2_SCALE_DIV ? "AB2 predictor" : nullptr);
 }
 
+// The grad_velocity_filt forward transform of one velocity component in the
+// mixed mode, with the FP64 Nyquist sums of field slot fld (0 u, 1 v, 2 w)
+// formed on the way (k_rows_r2c_nyq).  Needs ny % 8 == 0.
+static void rows_r2c_nyq(Les& L, const double* src, Cplx<float>* dst, int npl, float scale, int fld) {
+  RowR2CArgs<double, float> a;
+  a.src[0] = src; a.dst[0] = dst;
+  a.nx = L.nx; a.ny = L.ny; a.lhx = L.lhx; a.k0 = 0; a.nplanes = npl;
+  a.mode = LOAD_SCALE; a.scale = scale; a.div = 1.0f;
+  a.ls = row_line<float>(L.nx);
+  a.nyq_a = L.nyq_rows + (long)fld * L.ny * L.nzg;
+  a.nyq_p = L.nyq_part + (long)fld * (L.ny / 4) * L.nx * L.nzg;
+  const long jobs = (long)npl * (L.ny / 2);
+  assert(L.ny % 8 == 0 && jobs % kWavesPerBlock == 0);
+  const size_t sh = (size_t)kWavesPerBlock * a.ls * sizeof(Cplx<float>) + (size_t)kWavesPerBlock * L.nx * sizeof(double);
+  kt_begin(L, KT_ROWS_R2C);
+  hipLaunchKernelGGL(k_rows_r2c_nyq<float>, dim3((unsigned)(jobs / kWavesPerBlock), 1), dim3(LES_BLOCK), sh, L.stream, a, L.px,
+                     twpool<float>(L));
+  // the field read, its half spectrum written; A (1/nx) and P (1/(8 nx))
+  // of a field not counted, as in k_nyq_reduce
+  kt_end(L, KT_ROWS_R2C, (double)npl * L.ny * (L.nx * 8.0 + (double)L.lhx * sizeof(Cplx<float>)), "k_rows_r2c_nyq", 8, 4);
+}
+
 template <typename T, typename TO>
 static void rows_c2r(Les& L, const Cplx<T>* const* src, TO* const* dst, int nf, int k0, int npl) {
   RowC2RArgs<T, TO> a;
@@ -286,6 +310,15 @@ static double* D(Les& L, int id) { return (double*)L.f[id]; }
 
 static unsigned nblk(long n, int b = LES_BLOCK) { return (unsigned)((n + b - 1) / b); }
 
+// Mixed mode with ny % 8 == 0: grad_filt's forward transforms of u, v, w
+// also form the sums of the FP64 Nyquist removal (k_rows_r2c_nyq), which
+// then starts at its second pass.  Every caller of the removal runs
+// grad_filt on u, v and w first, on the same stream.
+#ifndef LES_NYQ_SUMS
+#define LES_NYQ_SUMS 1
+#endif
+static bool nyq_sums_fused(const Les& L) { return LES_NYQ_SUMS && L.mixed && L.ny % 8 == 0; }
+
 // ------------------------------------------------------------------ operators
 // grad_velocity_filt for one velocity component (FP64 path: the filtered
 // field is written back as in src/compute_derivatives.f90:134).
@@ -298,7 +331,12 @@ static void grad_filt(Les& L, int fid, int dx_id, int dy_id) {
   const double* src[1] = {D(L, fid)};
   Cplx<T>* dst[1] = {s0};
   T c = (T)(1.0 / (double)(L.nx * L.ny));
-  rows_r2c<double, T>(L, src, dst, 1, 0, L.nzg, c);
+  if constexpr (std::is_same<T, float>::value) {
+    if (nyq_sums_fused(L)) rows_r2c_nyq(L, src[0], s0, L.nzg, c, fid - LES_U);
+    else rows_r2c<double, T>(L, src, dst, 1, 0, L.nzg, c);
+  } else {
+    rows_r2c<double, T>(L, src, dst, 1, 0, L.nzg, c);
+  }
   cols<T>(L, COP_GRAD, s0, nullptr, s1, s2, s3, L.mixed ? 2 : 3, 1, 1, 0, L.nzg);
   const Cplx<T>* cs[2] = {s1, s2};
   T* outs[2] = {F<T>(L, dx_id), F<T>(L, dy_id)};
@@ -310,7 +348,16 @@ static void grad_filt(Les& L, int fid, int dx_id, int dy_id) {
   }
 }
 
-static void nyquist_remove_fp64(Les& L);
+static void nyquist_remove_fp64(Les& L, bool with_ddz = false);
+// mixed modes: the Nyquist removal's last pass also applies the boundary
+// conditions and forms the vertical derivatives (k_nyq_apply_ddz)
+#ifndef LES_NYQ_DDZ
+#define LES_NYQ_DDZ 1
+#endif
+// column segments per block of k_nyq_apply_ddz (64 columns each)
+#ifndef LES_NYQ_SEGS
+#define LES_NYQ_SEGS 8
+#endif
 
 template <typename T>
 static void vel_grad_bc_ddz(Les& L);
@@ -327,6 +374,10 @@ static void compute_vel_grad(Les& L) {
 // mode, the boundary conditions and the vertical derivatives.
 template <typename T>
 static void vel_grad_bc_ddz(Les& L) {
+  if (L.mixed && LES_NYQ_DDZ) {
+    nyquist_remove_fp64(L, true);
+    return;
+  }
   if (L.mixed) nyquist_remove_fp64(L);
   BcDdzArgs<T> a;
   a.u = D(L, LES_U); a.v = D(L, LES_V); a.w = D(L, LES_W);
@@ -606,9 +657,118 @@ __global__ void __launch_bounds__(LES_BLOCK) k_nyq_apply(NyqArgs a) {
   }
 }
 
+// Pass 3 fused with the boundary conditions and the vertical derivatives of
+// compute_vel_grad (k_bc_ddz), so that u, v, w are read and written once.
+// The staggered differences need the Nyquist-free value of a neighbouring
+// plane, so a thread walks a segment of one column upwards carrying the
+// previous and next planes in registers.  A block holds 64 columns (one
+// wavefront) times SEGS segments of the same columns; the raw values just
+// outside a segment (planes k0-1 and k1, written by the neighbouring
+// segments) are read before a block barrier and every store comes after it.
+// Each output location has one writer and the same arithmetic as the two
+// unfused passes: plane 0 of u, v takes the wall-law ghost u(1) - lbc dz,
+// plane nz+1 the top condition, w(1) = 0 and w(nz+1) = ubc_w.
+// The plane index is wavefront-uniform, and so is the row j when nx is a
+// multiple of 64 (ROWU): each wavefront then stages its segment's A_jk and
+// C_k in LDS (read back as broadcasts), leaving two vector loads per point
+// (the raw value and B_ik).  LDS: 2 (len + 2) doubles per wavefront.
+template <typename T, int SEGS, bool ROWU>
+__global__ void __launch_bounds__(64 * SEGS) k_nyq_apply_ddz(NyqArgs a, BcDdzArgs<T> b) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int nx = a.nx, ny = a.ny, nzg = a.nzg, nz = b.nz, fld = blockIdx.y;
+  const long np = (long)nx * ny;
+  const long q = (long)blockIdx.x * LES_WAVE + (threadIdx.x & (LES_WAVE - 1));
+  const int seg = __builtin_amdgcn_readfirstlane(threadIdx.x / LES_WAVE), len = (nzg + SEGS - 1) / SEGS;
+  const int k0 = seg * len, k1 = min(k0 + len, nzg);
+  const bool live = q < np && k0 < k1;
+  const int j = ROWU ? __builtin_amdgcn_readfirstlane((int)(q / nx)) : (int)(q / nx), i = (int)(q % nx);
+  const double si = (i & 1) ? -1.0 : 1.0, sj = (j & 1) ? -1.0 : 1.0;
+  const double* __restrict__ A = a.A + (long)fld * ny * nzg + j;
+  const double* __restrict__ B = a.B + (long)fld * nx * nzg + i;
+  const double* __restrict__ C = a.C + (long)fld * nzg;
+  double* f = a.f[fld] + q;  // f[k * np]: plane k of this column, updated in place
+  // planes k0-1 .. k1 of A_j and C, at index k - k0 + 1
+  double* sA = reinterpret_cast<double*>(smem_raw) + (long)seg * 2 * (len + 2);
+  double* sC = sA + (len + 2);
+  if (ROWU) {
+    for (int t = threadIdx.x & (LES_WAVE - 1); t < len + 2; t += LES_WAVE) {
+      const int kk = k0 - 1 + t;
+      if (kk >= 0 && kk < nzg) {
+        sA[t] = A[(long)kk * ny];
+        sC[t] = C[kk];
+      }
+    }
+  }
+  // the Nyquist-free value of plane k from its raw value (k_nyq_apply)
+  auto filt = [&](int k, double raw) {
+    const double av = ROWU ? sA[k - k0 + 1] : A[(long)k * ny], cv = ROWU ? sC[k - k0 + 1] : C[k];
+    return raw - (si * av + sj * B[(long)k * nx] - si * sj * cv);
+  };
+  double below = 0.0, above = 0.0;
+  if (live && k0 > 0) below = f[(long)(k0 - 1) * np];
+  if (live && k1 < nzg) above = f[(long)k1 * np];
+  __syncthreads();
+  if (!live) return;
+  const double cz = b.cz;
+  T* d = fld == 0 ? b.dudz + q : fld == 1 ? b.dvdz + q : b.dwdz + q;
+  const double lbc = fld < 2 ? (fld == 0 ? b.lbc_u[q] : b.lbc_v[q]) : 0.0;
+  const int neu = fld == 0 ? b.ubc_u_neu : b.ubc_v_neu;
+  const double tbc = fld == 0 ? b.ubc_u : b.ubc_v;
+  // plane k from the Nyquist-free values of planes k-1, k, k+1
+  auto emit = [&](int k, double prev, double cur, double nxt) {
+    const long c = (long)k * np;
+    if (fld < 2) {
+      if (k == 0) {
+        f[c] = nxt - lbc * b.dz;
+      } else if (k <= nz) {
+        const double fm = k == 1 ? cur - lbc * b.dz : prev;
+        f[c] = cur;
+        d[c] = (T)(cz * (cur - fm));
+      } else {
+        const double fk = neu ? prev + tbc * b.dz : 2.0 * tbc - prev;
+        f[c] = fk;
+        d[c] = (T)(cz * (fk - prev));
+      }
+    } else {
+      if (k == 0) {
+        f[c] = cur;
+      } else if (k <= nz) {
+        const double wk = k == 1 ? 0.0 : cur;
+        const double wp = k == nz ? b.ubc_w : nxt;
+        f[c] = wk;
+        d[c] = (T)(cz * (wp - wk));
+      } else {
+        f[c] = b.ubc_w;
+        d[c] = (T)0;
+      }
+    }
+  };
+  double prev = k0 > 0 ? filt(k0 - 1, below) : 0.0;
+  double cur = filt(k0, f[(long)k0 * np]);
+  // four planes at a time: their raw values are loaded (this thread's own
+  // planes, stored later by this thread, or the value read before the
+  // barrier) before the four stores
+  for (int kb = k0; kb < k1; kb += 4) {
+    double nxt[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const int kk = kb + 1 + m;
+      nxt[m] = kk < k1 ? filt(kk, f[(long)kk * np]) : (kk == k1 && kk < nzg) ? filt(kk, above) : 0.0;
+    }
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      if (kb + m < k1) {
+        emit(kb + m, prev, cur, nxt[m]);
+        prev = cur;
+        cur = nxt[m];
+      }
+    }
+  }
+}
+
 // Nyquist rows and columns of u, v, w removed in FP64 (three passes, the
 // three fields together).
-static void nyquist_remove_fp64(Les& L) {
+static void nyquist_remove_fp64(Les& L, bool with_ddz) {
   NyqArgs a;
   a.f[0] = D(L, LES_U); a.f[1] = D(L, LES_V); a.f[2] = D(L, LES_W);
   a.A = L.nyq_rows;
@@ -619,18 +779,43 @@ static void nyquist_remove_fp64(Les& L) {
   const long nrows = (long)L.ny * L.nzg;
   const int rpw = L.ny % 16 == 0 ? 4 : 1;
   const unsigned nblocks = (unsigned)(nrows / (4 * rpw));
+  // bands of column partials per plane: 8 rows each when the forward
+  // transforms formed the sums, else 4 rpw rows
+  const bool fused_sums = nyq_sums_fused(L);
+  const int nb = fused_sums ? L.ny / 8 : L.ny / (4 * rpw);
   // algorithmic bytes: the reduction reads u, v, w; the apply pass reads
   // and writes them; the row sums A and column partials are not counted
   // (1/nx and 4/nx of a field)
+  if (!fused_sums) {
+    kt_begin(L, KT_POINTWISE);
+    const size_t shp = sizeof(double) * (LES_BLOCK / LES_WAVE) * (size_t)L.nx;
+    if (rpw == 4) hipLaunchKernelGGL(k_nyq_reduce<4>, dim3(nblocks, 3), dim3(LES_BLOCK), shp, L.stream, a);
+    else hipLaunchKernelGGL(k_nyq_reduce<1>, dim3(nblocks, 3), dim3(LES_BLOCK), shp, L.stream, a);
+    kt_end(L, KT_POINTWISE, 24.0 * L.nf, "k_nyq_reduce");
+  }
   kt_begin(L, KT_POINTWISE);
-  const size_t shp = sizeof(double) * (LES_BLOCK / LES_WAVE) * (size_t)L.nx;
-  if (rpw == 4) hipLaunchKernelGGL(k_nyq_reduce<4>, dim3(nblocks, 3), dim3(LES_BLOCK), shp, L.stream, a);
-  else hipLaunchKernelGGL(k_nyq_reduce<1>, dim3(nblocks, 3), dim3(LES_BLOCK), shp, L.stream, a);
-  kt_end(L, KT_POINTWISE, 24.0 * L.nf, "k_nyq_reduce");
+  hipLaunchKernelGGL(k_nyq_bc, dim3((unsigned)L.nzg, 3), dim3(LES_BLOCK), 0, L.stream, a, nb);
+  kt_end(L, KT_POINTWISE, 3.0 * 8.0 * (double)L.nzg * ((double)nb * L.nx + L.nx), "k_nyq_bc");
   kt_begin(L, KT_POINTWISE);
-  hipLaunchKernelGGL(k_nyq_bc, dim3((unsigned)L.nzg, 3), dim3(LES_BLOCK), 0, L.stream, a, L.ny / (4 * rpw));
-  kt_end(L, KT_POINTWISE, 3.0 * 8.0 * (double)L.nzg * ((double)L.ny / (4 * rpw) * L.nx + L.nx), "k_nyq_bc");
-  kt_begin(L, KT_POINTWISE);
+  if (with_ddz) {
+    BcDdzArgs<float> b;
+    b.u = D(L, LES_U); b.v = D(L, LES_V); b.w = D(L, LES_W);
+    b.dudz = F<float>(L, LES_DUDZ); b.dvdz = F<float>(L, LES_DVDZ); b.dwdz = F<float>(L, LES_DWDZ);
+    b.lbc_u = L.layer[LES_LBC_U]; b.lbc_v = L.layer[LES_LBC_V];
+    b.np = L.np; b.nz = L.nz; b.dz = L.dz; b.cz = 1.0 / L.dz;
+    b.ubc_u_neu = L.P.ubc_u_neu; b.ubc_v_neu = L.P.ubc_v_neu;
+    b.ubc_u = L.P.ubc_u; b.ubc_v = L.P.ubc_v; b.ubc_w = L.P.ubc_w;
+    const unsigned ncb = (unsigned)((L.np + LES_WAVE - 1) / LES_WAVE);
+    constexpr int segs = LES_NYQ_SEGS;
+    const size_t sh = sizeof(double) * 2 * segs * (size_t)((L.nzg + segs - 1) / segs + 2);
+    if (L.nx % LES_WAVE == 0)
+      hipLaunchKernelGGL((k_nyq_apply_ddz<float, segs, true>), dim3(ncb, 3), dim3(64 * segs), sh, L.stream, a, b);
+    else
+      hipLaunchKernelGGL((k_nyq_apply_ddz<float, segs, false>), dim3(ncb, 3), dim3(64 * segs), 0, L.stream, a, b);
+    // u, v, w read and written, their vertical derivatives written
+    kt_end(L, KT_POINTWISE, 48.0 * L.nf + 12.0 * L.np * (L.nz + 1), "k_nyq_apply_ddz");
+    return;
+  }
   if (rpw == 4) hipLaunchKernelGGL(k_nyq_apply<4>, dim3(nblocks, 3), dim3(LES_BLOCK), 0, L.stream, a);
   else hipLaunchKernelGGL(k_nyq_apply<1>, dim3(nblocks, 3), dim3(LES_BLOCK), 0, L.stream, a);
   kt_end(L, KT_POINTWISE, 48.0 * L.nf, "k_nyq_apply");

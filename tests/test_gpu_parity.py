@@ -124,6 +124,58 @@ def test_operators_match_oracle_individually():
         _compare(g, o, prm.nz, ["p", "dpdx", "dpdy"], tol=0.0)
 
 
+# The mixed mode removes the Nyquist modes of u, v, w in FP64 by direct sums
+# and, in the same pass, applies the wall and lid conditions and forms the
+# vertical derivatives; the FP64 mode does the same with its FP64 transforms
+# and a separate boundary pass.  Both from the same fields and wall-law
+# layers: u, v, w (every plane, ghosts included) agree to FP64 round-off,
+# dudz, dvdz, dwdz to FP32 rounding of the same FP64 differences.  12x24x3
+# leaves a partial wavefront of columns and one-plane segments; the lid
+# conditions are Dirichlet there (u = 0.3, v = -0.2, w = 0.05), Neumann in
+# the others.
+@pytest.mark.parametrize("grid", [(12, 24, 3), (32, 32, 16), (256, 256, 16)])
+def test_mixed_velocity_gradients_match_fp64(grid):
+    pkg, P = load_pkg(), load_params()
+    nx, ny, nz = grid
+    kw = dict(ubc_u_neu=0, ubc_v_neu=0, ubc_u=0.3, ubc_v=-0.2, ubc_w=0.05) if nz == 3 else {}
+    prm, u, v, w = _setup(nx, ny, nz, seed=13, **kw)
+    rng = np.random.default_rng(nx + nz)
+    # full fields, ghost planes included, plus white noise: every Nyquist
+    # row and column of every plane carries energy
+    full = []
+    for a in (u, v, w):
+        f = np.zeros((nz + 2, ny, nx))
+        f[1:nz + 1] = a
+        full.append(f + 0.05 * rng.standard_normal(f.shape))
+    u, v, w = full
+    lbc_u, lbc_v = 0.5 + 0.1 * rng.standard_normal((ny, nx)), 0.1 * rng.standard_normal((ny, nx))
+    names = ["u", "v", "w", "dudx", "dudy", "dudz", "dvdx", "dvdy", "dvdz", "dwdx", "dwdy", "dwdz"]
+    out = {}
+    for prec in (P.PREC_FP64, P.PREC_MIXED):
+        prm.precision = prec
+        print(f"velocity gradients {grid} precision {prec}", flush=True)
+        with pkg.LesGpu(prm) as g:
+            g.upload("u", u); g.upload("v", v); g.upload("w", w)
+            g.set_layer("lbc_u", lbc_u); g.set_layer("lbc_v", lbc_v)
+            g.compute_vel_grad()
+            out[prec] = {n: g.download(n) for n in names}
+    a, b = out[P.PREC_MIXED], out[P.PREC_FP64]
+    for n in names:
+        lev = slice(0, nz + 2) if n in ("u", "v", "w") else slice(1, nz + 2)
+        x, y = a[n][lev], b[n][lev]
+        err = np.abs(x - y).max() / np.abs(y).max()
+        print(f"  {n:5s} max |mixed - fp64| / max |fp64| = {err:.2e}", flush=True)
+        if n in ("u", "v", "w"):
+            assert err < 1e-14, n
+        elif n.endswith("dz"):
+            assert err < 2e-7, n   # FP64 difference rounded once to FP32
+        else:
+            assert err < 1e-5, n   # FP32 spectral derivative
+    # the lid and wall values themselves
+    assert np.array_equal(a["w"][1], np.zeros((ny, nx)))
+    assert np.all(a["w"][nz + 1] == prm.ubc_w)
+
+
 # 256x256 runs the FP32 transforms of the production lengths (256, 384) on
 # register twiddles; a wrong twiddle there would give O(1) errors
 @pytest.mark.parametrize("grid", [(32, 32, 16), (256, 256, 4)])
