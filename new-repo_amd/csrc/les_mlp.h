@@ -1,8 +1,9 @@
 // Learned SGS surrogate: a per-point MLP 12 -> 64 -> 64 -> 64 -> 2 (GELU)
 // that predicts the Lagrangian scale-dependent model's Cs^2 and beta from
 // strain/rotation invariants at the grid and test-filter scales.  BF16
-// weights and activations, FP32 accumulation on MFMA
-// (v_mfma_f32_16x16x16_bf16).  No counterpart in the reference; the plain
+// weights and activations, FP32 accumulation on MFMA (inference:
+// v_mfma_f32_16x16x32_bf16, the first layer's 16 padded inputs and the
+// training GEMMs v_mfma_f32_16x16x16_bf16).  No counterpart in the reference; the plain
 // dynamic procedure (les_lasd.h) generates its training targets, serves as
 // the fallback for out-of-range inputs and judges it periodically.
 // Included by les_gpu.hip.
@@ -37,6 +38,16 @@ __device__ __forceinline__ float bf2f(short b) { return __uint_as_float(((unsign
 __device__ __forceinline__ f4v mfma16(s4v a, s4v b, f4v c) {
   return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
+// gfx950's double-K bf16 MFMA: 8 operand values per lane.  A and B hold
+// their K values in the same (lane group, slot) places, so any fixed
+// assignment of logical inputs to those places, used for both operands,
+// gives the same dot products.
+typedef short s8v __attribute__((ext_vector_type(8)));
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ f4v mfma32(s8v a, s8v b, f4v c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8v, a), __builtin_bit_cast(bf8v, b), c, 0, 0, 0);
+}
+__device__ __forceinline__ s8v cat8(s4v a, s4v b) { return __builtin_shufflevector(a, b, 0, 1, 2, 3, 4, 5, 6, 7); }
 
 // GELU, tanh form: 0.5 x (1 + tanh(y)), y = c (x + 0.044715 x^3), evaluated
 // as x s with s = 0.5 (1 + tanh(y)) = 1 / (1 + exp(-2y)) = 1 / (1 + 2^(x (k1 +
@@ -225,26 +236,39 @@ __device__ __forceinline__ void store_coef(const InferArgs& a, long p, double cs
   }
 }
 
-// Weights as A-operand fragments W[16 rb + r][16 cb + q + j], j = 0..3, in
-// LDS in lane order (one 8-byte read per fragment): blocks 0-3 W1 (rb),
-// 4-19 W2 (rb, cb), 20-35 W3, 36-39 W4 (cb).  Biases in LDS too: read per
-// tile instead of occupying registers, which doubles the waves per SIMD.
+// Weights as A-operand fragments in LDS in lane order, read once per tile
+// and fragment.  Layer 1 (12 inputs padded to 16) on the 16x16x16 MFMA:
+// sw1[rb] holds W1[16 rb + r][q + j], j = 0..3, lane l = 16 (q / 4) + r.
+// Layers 2-4 on the 16x16x32 MFMA: input chunk c (units 32c .. 32c+31) of
+// output block rb is the pair of 16-unit activation blocks 2c, 2c+1 that
+// the previous layer leaves in lane l = 16 g + n as units 16 b + 4 g + j
+// (j = 0..3); so slot s of lane group g stands for unit
+// u(c, g, s) = 32 c + 16 (s / 4) + 4 g + s % 4 in both operands, and sw
+// holds W[16 rb + r][u(c, g, s)]: blocks 0-7 W2 (rb, c), 8-15 W3, 16-17 W4
+// (c).  Biases in LDS too: read per tile instead of occupying registers.
 struct MlpLds {
-  s4v sw[40 * 64];
+  s4v sw1[4 * 64];
+  s8v sw[18 * 64];
   float sb[MLP_NBIAS];
 };
 
 __device__ __forceinline__ void mlp_lds_fill(MlpLds& m, const short* Wb, const float* bias) {
-  for (int idx = threadIdx.x; idx < 40 * 64; idx += blockDim.x) {
-    const int blk = idx >> 6, ln = idx & 63;
-    const int rr = ln & 15, qq = (ln >> 4) * 4;
-    int base;
-    if (blk < 4) base = MW1 + (16 * blk + rr) * 16 + qq;
-    else if (blk < 20) base = MW2 + (16 * ((blk - 4) >> 2) + rr) * 64 + 16 * ((blk - 4) & 3) + qq;
-    else if (blk < 36) base = MW3 + (16 * ((blk - 20) >> 2) + rr) * 64 + 16 * ((blk - 20) & 3) + qq;
-    else base = MW4 + rr * 64 + 16 * (blk - 36) + qq;
+  for (int idx = threadIdx.x; idx < 4 * 64; idx += blockDim.x) {
+    const int rb = idx >> 6, ln = idx & 63;
+    const int base = MW1 + (16 * rb + (ln & 15)) * 16 + (ln >> 4) * 4;
     s4v v;
     for (int j = 0; j < 4; ++j) v[j] = Wb[base + j];
+    m.sw1[idx] = v;
+  }
+  for (int idx = threadIdx.x; idx < 18 * 64; idx += blockDim.x) {
+    const int blk = idx >> 6, ln = idx & 63;
+    const int r = ln & 15, g = ln >> 4;
+    int row, c, mat;
+    if (blk < 8) { mat = MW2; row = 16 * (blk >> 1) + r; c = blk & 1; }
+    else if (blk < 16) { mat = MW3; row = 16 * ((blk - 8) >> 1) + r; c = (blk - 8) & 1; }
+    else { mat = MW4; row = r; c = blk - 16; }
+    s8v v;
+    for (int sl = 0; sl < 8; ++sl) v[sl] = Wb[mat + row * 64 + 32 * c + 16 * (sl >> 2) + 4 * g + (sl & 3)];
     m.sw[idx] = v;
   }
   for (int i = threadIdx.x; i < MLP_NBIAS; i += blockDim.x) m.sb[i] = bias[i];
@@ -254,11 +278,10 @@ __device__ __forceinline__ void mlp_lds_fill(MlpLds& m, const short* Wb, const f
 // + r); lanes 0..15 return outputs 0..3 of point r.
 __device__ __forceinline__ f4v mlp_tile(const MlpLds& m, int l, s4v x) {
   const int q = (l >> 4) * 4;
-  const s4v* wl = m.sw + l;
-  const s4v* w1 = wl;
-  const s4v* w2 = wl + 4 * 64;
-  const s4v* w3 = wl + 20 * 64;
-  const s4v* w4 = wl + 36 * 64;
+  const s4v* w1 = m.sw1 + l;
+  const s8v* w2 = m.sw + l;
+  const s8v* w3 = m.sw + 8 * 64 + l;
+  const s8v* w4 = m.sw + 16 * 64 + l;
   const f4v* bb1 = reinterpret_cast<const f4v*>(m.sb + MB1 + q);
   const f4v* bb2 = reinterpret_cast<const f4v*>(m.sb + MB2 + q);
   const f4v* bb3 = reinterpret_cast<const f4v*>(m.sb + MB3 + q);
@@ -271,16 +294,16 @@ __device__ __forceinline__ f4v mlp_tile(const MlpLds& m, int l, s4v x) {
   }
   for (int rb = 0; rb < 4; ++rb) {
     f4v acc = bb2[4 * rb];
-    for (int cb = 0; cb < 4; ++cb) acc = mfma16(w2[(rb * 4 + cb) * 64], h1[cb], acc);
+    for (int c = 0; c < 2; ++c) acc = mfma32(w2[(rb * 2 + c) * 64], cat8(h1[2 * c], h1[2 * c + 1]), acc);
     h2[rb] = gelu_bf4(acc);
   }
   for (int rb = 0; rb < 4; ++rb) {
     f4v acc = bb3[4 * rb];
-    for (int cb = 0; cb < 4; ++cb) acc = mfma16(w3[(rb * 4 + cb) * 64], h2[cb], acc);
+    for (int c = 0; c < 2; ++c) acc = mfma32(w3[(rb * 2 + c) * 64], cat8(h2[2 * c], h2[2 * c + 1]), acc);
     h3[rb] = gelu_bf4(acc);
   }
   f4v y = bb4;
-  for (int cb = 0; cb < 4; ++cb) y = mfma16(w4[cb * 64], h3[cb], y);
+  for (int c = 0; c < 2; ++c) y = mfma32(w4[c * 64], cat8(h3[2 * c], h3[2 * c + 1]), y);
   return y;
 }
 

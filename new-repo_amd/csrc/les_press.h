@@ -28,22 +28,40 @@ struct TriArgs {
 #ifndef LES_TRI_U
 #define LES_TRI_U 16
 #endif
+// The sweep takes all of a wavefront's time and its registers allow one
+// wavefront per SIMD, so only the modes that are solved get lanes: the
+// (lhx-1) (ny-1) modes off the Nyquist row and column, 2 lanes each (at
+// 256^2: 1020 wavefronts on 1024 SIMDs, one round; with every mode's lanes
+// it took 1032, and the last few ran in a second round).  The first
+// tri_zero_blocks blocks only zero the Nyquist modes' phat and finish at
+// once.
+__host__ __device__ inline long tri_nwork(int lhx, int ny) { return (long)(lhx - 1) * (ny - 1); }
+__host__ __device__ inline long tri_zero_blocks(int lhx, int ny) { return ((long)lhx * ny - tri_nwork(lhx, ny) + 63) / 64; }
 __global__ void __launch_bounds__(64) k_tridiag(TriArgs a) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long q = t >> 1;
-  const int c = (int)(t & 1);
-  if (q >= a.ns) return;
-  const int i = (int)(q % a.lhx), j = (int)(q / a.lhx);
   const int nz = a.nz;
   const long ns = a.ns;
+  const long nzb = tri_zero_blocks(a.lhx, a.ny);
+  if (blockIdx.x < nzb) {
+    // Nyquist modes, one lane each: the row j = ny/2 (every i), then the
+    // column i = lhx-1 of the other rows
+    const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= (long)a.lhx + a.ny - 1) return;
+    const int jn = (int)(m - a.lhx);
+    const long q = m < a.lhx ? (long)(a.ny / 2) * a.lhx + m : (long)(jn < a.ny / 2 ? jn : jn + 1) * a.lhx + a.lhx - 1;
+    for (int k = 0; k <= nz + 1; ++k) a.phat[(long)k * ns + q] = Cplx<double>{0.0, 0.0};
+    return;
+  }
+  const long t = (long)(blockIdx.x - nzb) * blockDim.x + threadIdx.x;
+  const long w = t >> 1;
+  const int c = (int)(t & 1);
+  if (w >= tri_nwork(a.lhx, a.ny)) return;
+  const int i = (int)(w % (a.lhx - 1)), jw = (int)(w / (a.lhx - 1));
+  const int j = jw < a.ny / 2 ? jw : jw + 1;
+  const long q = (long)j * a.lhx + i;
   double* __restrict__ ph = reinterpret_cast<double*>(a.phat) + c;   // component c of phat
   double* __restrict__ gam = a.gam;
   const double* __restrict__ Hdc = reinterpret_cast<const double*>(a.Hd) + c;
   const double* __restrict__ Hzc = reinterpret_cast<const double*>(a.Hz) + c;
-  if (i == a.lhx - 1 || j == a.ny / 2) {
-    for (int k = 0; k <= nz + 1; ++k) ph[2 * ((long)k * ns + q)] = 0.0;
-    return;
-  }
   const double dz = a.dz, idz2 = a.idz2;
   const double kx = a.kkx[q], ky = a.kky[q];
   const double bmid = -((kx * kx + ky * ky) + 2.0 / (dz * dz));
@@ -205,7 +223,8 @@ static void press_stag_array(Les& L, double tadv1, bool ab2 = false, double tadv
   t.lhx = L.lhx; t.ny = L.ny; t.nz = nz; t.ns = L.ns;
   t.dz = L.dz; t.idz2 = 1.0 / (L.dz * L.dz);
   kt_begin(L, KT_TRIDIAG);
-  hipLaunchKernelGGL(k_tridiag, dim3(nblk(2 * L.ns, 64)), dim3(64), 0, L.stream, t);
+  const long tri_blocks = tri_zero_blocks(L.lhx, L.ny) + (2 * tri_nwork(L.lhx, L.ny) + 63) / 64;
+  hipLaunchKernelGGL(k_tridiag, dim3((unsigned)tri_blocks), dim3(64), 0, L.stream, t);
   kt_end(L, KT_TRIDIAG, 16.0 * L.ns * (2.0 * nz + 2.0 + (nz + 2.0)), "k_tridiag");
   // gradient: i kx p, i ky p and p back to physical space (src/compute_pressure.f90:320-357)
   cols<double>(L, COP_PGRAD, ph, nullptr, Hx, Hy, Hz, 3, 0, 1, 0, nz + 2);

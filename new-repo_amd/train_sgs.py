@@ -8,10 +8,11 @@
    (les_mlp_train_step); torch only provides device buffers and shuffling.
 3. Save the compact weight vector (les_mlp_layout) as .npy.
 
-Usage: python new-repo_amd/train_sgs.py --grid 256,256,256 --updates 9 --gap-cycles 9 --stride 3 \
-           --epochs 20 --out new-repo_amd/weights/sgs_mlp.npy
-(samples after the dynamic updates at steps 25, 75, ..., 475 of a run from
-initial-field seed 4321; the last one is held out)
+Usage: python new-repo_amd/train_sgs.py --grid 256,256,256 --warm-cycles 400 --updates 16 --gap-cycles 99 \
+           --stride 4 --epochs 20 --out new-repo_amd/weights/sgs_mlp.npy
+(samples after the dynamic updates at steps 2005, 2505, ..., 10005 of a run
+from initial-field seed 4321, i.e. of the developed flow; the last one is
+held out)
 """
 from __future__ import annotations
 
