@@ -58,7 +58,7 @@ WEIGHTS = ROOT / "new-repo_amd" / "weights" / "sgs_mlp.npy"
 # HBM traffic per launch of a kernel family, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE passes scaled by a copy kernel of known size (written by
 # new-repo_amd/profile_summary.py)
-TRAFFIC = ROOT / "profiles" / "pmc_traffic.json"
+PROFILES = ROOT / "profiles"  # pmc_traffic_<variant>.json: counter traffic per kernel
 
 
 def load_weights():
@@ -80,10 +80,11 @@ def copy_calibration(mib):
 
 
 def load_traffic(variant, name, grid, table="families"):
-    if not TRAFFIC.exists():
+    path = PROFILES / f"pmc_traffic_{variant}.json"
+    if not path.exists():
         return None
     try:
-        t = json.loads(TRAFFIC.read_text())
+        t = json.loads(path.read_text())
     except ValueError:
         return None
     if t.get("variant") == variant and t.get("grid") == list(grid):
@@ -365,8 +366,9 @@ def main(argv=None):
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--profile-variants", default="mixed,learned",
                     help="variants compared with FP64 in the profile run (mixed, learned, constant, fp64_perturbed)")
-    ap.add_argument("--profile-steps", type=int, default=500,
-                    help="steps of the matched-error profile comparison (0 to skip)")
+    ap.add_argument("--profile-steps", type=int, default=3000,
+                    help="steps of the matched-error profile comparison (0 to skip); the running averages cover "
+                         "the last fifth")
     ap.add_argument("--no-kernel-stats", action="store_true", help="skip the per-launch HIP-event pass")
     ap.add_argument("--serial", action="store_true",
                     help="issue each step on one stream (default: the SGS stage on a second stream)")
@@ -431,8 +433,8 @@ def main(argv=None):
             # of those same launches
             roof["traffic_algorithmic_per_launch"] = tr["algorithmic_bytes_per_launch"]
             roof["traffic_ratio"] = tr["traffic_over_algorithmic"]
-            roof["traffic_source"] = ("profiles/pmc_traffic.json: rocprofv3 FETCH_SIZE and WRITE_SIZE passes, each "
-                                      "scaled by a 256 MiB copy kernel in the same process")
+            roof["traffic_source"] = (f"profiles/pmc_traffic_{head}.json: rocprofv3 FETCH_SIZE and WRITE_SIZE passes, "
+                                      "each scaled by a 256 MiB copy kernel in the same process")
     step_key = "fp64" if head == "fp64" else "mixed"
     step_bytes = STEP_BYTES[step_key] * nx * ny * nz
     out = {

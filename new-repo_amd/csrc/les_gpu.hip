@@ -188,7 +188,7 @@ static void rows_r2c_nyq(Les& L, const double* src, Cplx<float>* dst, int npl, f
                      twpool<float>(L));
   // the field read, its half spectrum written; A (1/nx) and P (1/(8 nx))
   // of a field not counted, as in k_nyq_reduce
-  kt_end(L, KT_ROWS_R2C, (double)npl * L.ny * (L.nx * 8.0 + (double)L.lhx * sizeof(Cplx<float>)), "k_rows_r2c_nyq", 8, 4);
+  kt_end(L, KT_ROWS_R2C, (double)npl * L.ny * (L.nx * 8.0 + (double)L.lhx * sizeof(Cplx<float>)), "k_rows_r2c_nyq", 4);
 }
 
 template <typename T, typename TO>
@@ -477,6 +477,11 @@ static void sgs_stress(Les& L, int jt, int jtt) {
   bool update = lasd && jtt >= L.P.dyn_init && (jtt % L.P.cs_count) == 0;
   bool dyn_update = dyns && jtt >= L.P.dyn_init && (jt % L.P.cs_count) == 0;
   if (acc) a.acc_kmax = lag_levels(L, jtt);
+  // levels whose u_lag sums a launch updates (0..acc_kmax, at most nz+1)
+  auto lag_bytes = [&]() {
+    if (!a.accumulate || a.acc_kmax < 1) return 0.0;
+    return (double)L.np * ((a.acc_kmax < L.nz + 1 ? a.acc_kmax : L.nz + 1) + 1.0) * (3 * 8.0 + 6.0 * sizeof(T));
+  };
   if (L.P.sgs_model == 0) {
     CsFillArgs<T> c;
     c.cs = F<T>(L, LES_CS_OPT2); c.val = (T)(L.P.cs0 * L.P.cs0); c.np = L.np; c.nz = L.nz;
@@ -491,8 +496,8 @@ static void sgs_stress(Les& L, int jt, int jtt) {
     a.write_sij = 1;
     kt_begin(L, KT_POINTWISE);
     hipLaunchKernelGGL((k_sij<T>), dim3(nblk(L.np * (L.nz + 1))), dim3(LES_BLOCK), 0, L.stream, a);
-    kt_end(L, KT_POINTWISE, (double)L.np * (L.nz + 1) * (16.0 * sizeof(T) + (a.accumulate ? 3 * 8.0 + 6.0 * sizeof(T) : 0.0)),
-           "k_sij", sizeof(T), 0, a.accumulate ? "with u_lag sums" : nullptr);
+    kt_end(L, KT_POINTWISE, (double)L.np * (L.nz + 1) * 16.0 * sizeof(T) + lag_bytes(), "k_sij", sizeof(T), 0,
+           a.accumulate ? "with u_lag sums" : nullptr);
     if constexpr (sizeof(T) == 4) {
       if (L.P.precision == 2) learned_update<T>(L, jtt);
       else lasd_update<T>(L, jtt, L.nz + 1);
@@ -512,8 +517,8 @@ static void sgs_stress(Les& L, int jt, int jtt) {
   a.write_sij = 0;
   kt_begin(L, KT_POINTWISE);
   hipLaunchKernelGGL((k_sgs<T>), dim3(nblk(L.np * (L.nz + 1))), dim3(LES_BLOCK), 0, L.stream, a);
-  kt_end(L, KT_POINTWISE, (double)L.np * (L.nz + 1) * (17.0 * sizeof(T) + (a.accumulate ? 3 * 8.0 + 6.0 * sizeof(T) : 0.0)),
-         "k_sgs", sizeof(T), 0, a.accumulate ? "with u_lag sums" : nullptr);
+  kt_end(L, KT_POINTWISE, (double)L.np * (L.nz + 1) * 17.0 * sizeof(T) + lag_bytes(), "k_sgs", sizeof(T), 0,
+         a.accumulate ? "with u_lag sums" : nullptr);
 }
 
 #include "les_lasd.h"
@@ -813,7 +818,7 @@ static void nyquist_remove_fp64(Les& L, bool with_ddz) {
     else
       hipLaunchKernelGGL((k_nyq_apply_ddz<float, segs, false>), dim3(ncb, 3), dim3(64 * segs), 0, L.stream, a, b);
     // u, v, w read and written, their vertical derivatives written
-    kt_end(L, KT_POINTWISE, 48.0 * L.nf + 12.0 * L.np * (L.nz + 1), "k_nyq_apply_ddz");
+    kt_end(L, KT_POINTWISE, 48.0 * L.nf + 12.0 * L.np * (L.nz + 1), "k_nyq_apply_ddz", 4);
     return;
   }
   if (rpw == 4) hipLaunchKernelGGL(k_nyq_apply<4>, dim3(nblocks, 3), dim3(LES_BLOCK), 0, L.stream, a);

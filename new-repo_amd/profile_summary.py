@@ -15,7 +15,7 @@ Usage:
   python new-repo_amd/profile_summary.py kernels out/trace
   python new-repo_amd/profile_summary.py traffic out/pmc_fetch out/pmc_write \\
       --bench-json out/pmc_fetch.json --variant learned --copy-mib 256 \\
-      --command "..." --out profiles/pmc_traffic.json
+      --command "..." --out profiles/pmc_traffic_learned.json
 """
 from __future__ import annotations
 
@@ -60,14 +60,22 @@ def kernels(d: str, top: int = 25):
 
 
 def _per_dispatch(d: str, counter: str):
+    """(kernel name, counter value) per dispatch, in dispatch order."""
     rows = _rows(d, "counter_collection.csv")
     vals = {}
     for r in rows:
         if _col(r, "Counter_Name").strip() != counter:
             continue
-        key = (_col(r, "Dispatch_Id", "Correlation_Id"), _col(r, "Kernel_Name"))
+        key = (int(_col(r, "Dispatch_Id", "Correlation_Id")), _col(r, "Kernel_Name"))
         vals[key] = vals.get(key, 0.0) + float(_col(r, "Counter_Value"))
-    return [(k[1], v) for k, v in vals.items()]
+    return [(k[1], vals[k]) for k in sorted(vals)]
+
+
+def _last(vals: list, n: int) -> list:
+    """The last n dispatches: the HIP-event cycle that closes the counter
+    run (its warm-up steps are profiled too, and may run other work, such
+    as the first, full-column dynamic update)."""
+    return vals[-n:] if n > 0 else vals
 
 
 def rocprof_key(name: str) -> str:
@@ -137,11 +145,11 @@ def traffic(args):
     for name, pat in FAMILY_KERNELS.items():
         if name not in fam:
             continue
-        f = [x for n, x in fd if re.search(pat, n)]
-        w = [x for n, x in wd if re.search(pat, n)]
+        calls = fam[name]["calls"]
+        f = _last([x for n, x in fd if re.search(pat, n)], calls)
+        w = _last([x for n, x in wd if re.search(pat, n)], calls)
         if not f or not w:
             continue
-        calls = fam[name]["calls"]
         tb = (sum(f) * sf + sum(w) * sw) * 1024.0
         out["families"][name] = {
             "kernel_pattern": pat, "dispatches": [len(f), len(w)], "calls": calls,
@@ -163,8 +171,8 @@ def traffic(args):
         e["ms"] += v["ms"]
     out["kernels"] = {}
     for key, e in sorted(lib.items(), key=lambda kv: -kv[1]["ms"]):
-        f = [x for n, x in fd if rocprof_key(n) == key]
-        w = [x for n, x in wd if rocprof_key(n) == key]
+        f = _last([x for n, x in fd if rocprof_key(n) == key], e["calls"])
+        w = _last([x for n, x in wd if rocprof_key(n) == key], e["calls"])
         if not f or not w:
             continue
         fb, wb = sum(f) * sf * 1024.0, sum(w) * sw * 1024.0
@@ -234,19 +242,28 @@ def sq(args):
     profiler: VALU instructions x 4 cycles (a wave64 instruction on a
     16-lane SIMD) / (1024 SIMDs x launch time x clock)."""
     rows = _rows(args.dir, "counter_collection.csv")
-    agg, disp = {}, {}
-    for r in rows:
-        k = rocprof_key(_col(r, "Kernel_Name"))
-        a = agg.setdefault(k, {})
-        n = _col(r, "Counter_Name").strip()
-        a[n] = a.get(n, 0.0) + float(_col(r, "Counter_Value"))
-        disp.setdefault(k, set()).add(_col(r, "Dispatch_Id", "Correlation_Id"))
     bench = json.loads(pathlib.Path(args.bench_json).read_text().splitlines()[-1])
     kern = {}
     for name, v in bench.get("kernels", {}).items():
         e = kern.setdefault(name.split(" [")[0], {"ms": 0.0, "calls": 0})
         e["ms"] += v["ms"]
         e["calls"] += v["calls"]
+    # the counter run's last `calls` dispatches of each kernel: its HIP-event
+    # cycle, the same steps of the cycle as the benchmark's
+    ids = {}
+    for r in rows:
+        ids.setdefault(rocprof_key(_col(r, "Kernel_Name")), set()).add(int(_col(r, "Dispatch_Id", "Correlation_Id")))
+    keep = {k: set(_last(sorted(v), kern.get(k, {}).get("calls", 0))) for k, v in ids.items()}
+    agg, disp = {}, {}
+    for r in rows:
+        k = rocprof_key(_col(r, "Kernel_Name"))
+        did = int(_col(r, "Dispatch_Id", "Correlation_Id"))
+        if did not in keep[k]:
+            continue
+        a = agg.setdefault(k, {})
+        n = _col(r, "Counter_Name").strip()
+        a[n] = a.get(n, 0.0) + float(_col(r, "Counter_Value"))
+        disp.setdefault(k, set()).add(did)
     lines = [f"# {args.title}", "", "Commands (repository root, on the MI355X):", "", "```"] + args.command + [
         "```", "", f"VALU issue fraction = SQ_INSTS_VALU x 4 / (1024 SIMDs x HIP-event launch time x "
         f"{args.clock_ghz} GHz); the HIP-event times are those of the default benchmark run (no profiler).", "",
