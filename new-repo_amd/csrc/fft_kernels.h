@@ -567,6 +567,36 @@ __global__ void __launch_bounds__(LES_BLOCK) k_cols_c16(ColArgs<T> a, const Cplx
   Cplx<T> X0[16], X1[16];
 #pragma unroll
   for (int t = 0; t < 16; ++t) X0[t] = valid ? a.in[0][base + (long)(ar + 16 * t) * a.lhx] : Cplx<T>{(T)0, (T)0};
+  if constexpr (NIN == 2 && OP == COP_DIV && NOUT == 1) {
+    // divergence: i kx X0 is formed before the second input is loaded (the
+    // transforms' barriers keep its loads after), so only one input line
+    // and the partial sum are live at a time; the same operations in the
+    // same order as c16_op's COP_DIV
+    if (a.fwd) c16_fft(X0, twf, false, ex, c, ar);
+    const Cplx<T> zero = {(T)0, (T)0};
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+      const int j = ar + 16 * t;
+      const bool nyq = (kx == a.lhx - 1) || (j == a.ny / 2);
+      X0[t] = c_ik(a.kkx[(long)j * a.lhx + kx], nyq ? zero : X0[t]);
+    }
+#pragma unroll
+    for (int t = 0; t < 16; ++t) X1[t] = valid ? a.in[1][base + (long)(ar + 16 * t) * a.lhx] : zero;
+    if (a.fwd) c16_fft(X1, twf, false, ex, c, ar);
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+      const int j = ar + 16 * t;
+      const bool nyq = (kx == a.lhx - 1) || (j == a.ny / 2);
+      X0[t] = c_add(X0[t], c_ik(a.kky[(long)j * a.lhx + kx], nyq ? zero : X1[t]));
+    }
+    if (a.inv) c16_fft(X0, twf, true, ex, c, ar);
+    if (valid) {
+      Cplx<T>* __restrict__ dst = a.out[0] + base;
+#pragma unroll
+      for (int t = 0; t < 16; ++t) dst[(long)(ar + 16 * t) * a.lhx] = X0[t];
+    }
+    return;
+  }
   if constexpr (NIN > 1) {
 #pragma unroll
     for (int t = 0; t < 16; ++t) X1[t] = valid ? a.in[1][base + (long)(ar + 16 * t) * a.lhx] : Cplx<T>{(T)0, (T)0};
