@@ -131,6 +131,21 @@ static int mlp_forward_backward(Les& L, const short* X, const float* Y, long n, 
 
 // features + inference for the current gradients (S_ij already written) on
 // levels k0..nz+1; cs/beta targets of elem size eb
+// Limit of the surrogate's coefficients: nu_t = Cs^2 delta^2 |S| at most
+// LES_MLP_DIFF_LIMIT h^2 / dt, h the smallest grid spacing (the explicit
+// step's diffusion number; the AB2 step of a diffusion term needs it below
+// about 1/4 per direction).  Without it a surrogate value near its bound of
+// 1 at a point of large |S| can make the run blow up within a few steps.
+#ifndef LES_MLP_DIFF_LIMIT
+#define LES_MLP_DIFF_LIMIT 0.2
+#endif
+static void mlp_diffusion_cap(const Les& L, InferArgs& ia) {
+  const double h = std::min(L.dx, std::min(L.dy, L.dz));
+  ia.s_mean = L.mlp->hn[0];
+  ia.s_std = L.mlp->hn[12];
+  ia.nu_cap = (float)(LES_MLP_DIFF_LIMIT * h * h / (L.P.dt * L.delta * L.delta));
+}
+
 template <typename T>
 static void learned_coeffs(Les& L, void* cs, void* beta, int eb, int k0) {
   MlpState* M = L.mlp;
@@ -183,6 +198,7 @@ static void learned_coeffs(Les& L, void* cs, void* beta, int eb, int k0) {
     ia.n = npt; ia.np = L.np; ia.nz = L.nz;
     ia.cs = cs; ia.beta = beta; ia.elem_bytes = eb;
     ia.raw_out = nullptr; ia.write_fields = 1; ia.p0 = p0;
+    mlp_diffusion_cap(L, ia);
     kt_begin(L, KT_MLP);
     hipLaunchKernelGGL(k_feat_infer, dim3(2048), dim3(256), 0, L.stream, fa, ia);
     kt_end(L, KT_MLP, (double)nsel * (27.0 * sizeof(T) + 2.0 * eb), "k_feat_infer");
@@ -199,6 +215,7 @@ static void learned_coeffs(Les& L, void* cs, void* beta, int eb, int k0) {
     ia.n = npt; ia.np = L.np; ia.nz = L.nz;
     ia.cs = cs; ia.beta = beta; ia.elem_bytes = eb;
     ia.raw_out = nullptr; ia.write_fields = 1; ia.p0 = p0;
+    mlp_diffusion_cap(L, ia);
     hipLaunchKernelGGL(k_mlp_infer, dim3(2048), dim3(256), 0, L.stream, ia);
     kt_end(L, KT_MLP, (double)nsel * (32.0 + 2.0 * eb), "k_mlp_infer");
   }
@@ -429,6 +446,7 @@ int les_mlp_forward(const void* d_x, long n, void* d_out) {
   ia.n = n; ia.np = L->np; ia.nz = L->nz;
   ia.cs = nullptr; ia.beta = nullptr; ia.elem_bytes = 4;
   ia.raw_out = (float*)d_out; ia.write_fields = 0; ia.p0 = 0;
+  ia.s_mean = 0.0f; ia.s_std = 1.0f; ia.nu_cap = 0.0f;
   hipLaunchKernelGGL(k_mlp_infer, dim3(256), dim3(256), 0, L->stream, ia);
   LES_CHECK(hipGetLastError());
   LES_CHECK(hipStreamSynchronize(L->stream));

@@ -223,6 +223,10 @@ struct InferArgs {
   float* raw_out;      // optional: normalised outputs [n][2]
   int write_fields;
   long p0;             // first point (multiple of 16): levels below are left alone
+  // explicit-diffusion limit of the coefficients the surrogate sets or keeps:
+  // Cs^2 <= nu_cap / |S|, |S| = exp(feature 0 x s_std + s_mean) (feature 0 is
+  // log(|S| + 1e-12)); nu_cap <= 0: no limit
+  float s_mean, s_std, nu_cap;
 };
 
 __device__ __forceinline__ void store_coef(const InferArgs& a, long p, double cs, double beta) {
@@ -319,14 +323,22 @@ __device__ __forceinline__ f4v mlp_tile(const MlpLds& m, int l, s4v x) {
 // MLP_BETA_MAX] (1 on the lid); out-of-range points keep their previous
 // coefficients, limited to the same range: a value the plain procedure gave
 // where |S| was at round-off level must not act for many steps where |S| has
-// grown since (it would make nu_t = Cs^2 delta^2 |S| exceed the explicit
-// step's diffusion limit).
-__device__ __forceinline__ void mlp_store(const InferArgs& a, long p, f4v y, bool in_range) {
+// grown since.  Both are also held to the explicit step's diffusion limit
+// Cs^2 |S| <= nu_cap (nu_t = Cs^2 delta^2 |S| <= c dz^2 / dt), with |S| the
+// point's grid-scale strain from its feature 0 (x0, normalised); the plain
+// procedure's own coefficients are not limited.
+__device__ __forceinline__ void mlp_store(const InferArgs& a, long p, f4v y, bool in_range, float x0) {
   if (a.raw_out) {
     a.raw_out[p * 2] = y[0];
     a.raw_out[p * 2 + 1] = y[1];
   }
   if (!a.write_fields) return;
+  float cap = MLP_CS2_MAX;
+  if (a.nu_cap > 0.0f) {
+    const float smag = __expf(x0 * a.s_std + a.s_mean);
+    const float lim = a.nu_cap / smag;
+    cap = lim < cap ? lim : cap;
+  }
   if (in_range) {
     const int k = 1 + (int)(p / a.np);
     const float c2 = y[0] * a.ystd0 + a.ymean0;
@@ -335,15 +347,14 @@ __device__ __forceinline__ void mlp_store(const InferArgs& a, long p, f4v y, boo
     b = b < MLP_BETA_MAX ? b : MLP_BETA_MAX;
     if (k == a.nz + 1) b = 1.0f;
     float cs = c2 > 1e-24f ? c2 : 1e-24f;
-    cs = cs < MLP_CS2_MAX ? cs : MLP_CS2_MAX;
+    cs = cs < cap ? cs : cap;
     store_coef(a, p, (double)cs, (double)b);
   } else {
     const long c = a.np + p;
     const double cs = a.elem_bytes == 8 ? ((const double*)a.cs)[c] : (double)((const float*)a.cs)[c];
     const double b = a.elem_bytes == 8 ? ((const double*)a.beta)[c] : (double)((const float*)a.beta)[c];
-    if (cs > (double)MLP_CS2_MAX || b > (double)MLP_BETA_MAX)
-      store_coef(a, p, cs < (double)MLP_CS2_MAX ? cs : (double)MLP_CS2_MAX,
-                 b < (double)MLP_BETA_MAX ? b : (double)MLP_BETA_MAX);
+    if (cs > (double)cap || b > (double)MLP_BETA_MAX)
+      store_coef(a, p, cs < (double)cap ? cs : (double)cap, b < (double)MLP_BETA_MAX ? b : (double)MLP_BETA_MAX);
   }
 }
 
@@ -365,7 +376,7 @@ __global__ void __launch_bounds__(256) k_mlp_infer(InferArgs a) {
     const f4v y = mlp_tile(m, l, x);
     // the range flag of point r is element 3 of lane 48 + r's feature registers
     const int flag_bits = __shfl((int)(unsigned short)x[3], 48 + r, 64);
-    if (l < 16 && p < a.n) mlp_store(a, p, y, bf2f((short)flag_bits) == 0.0f);
+    if (l < 16 && p < a.n) mlp_store(a, p, y, bf2f((short)flag_bits) == 0.0f, bf2f(x[0]));
   }
 }
 
@@ -535,7 +546,7 @@ __global__ void __launch_bounds__(256) k_feat_infer(FeatArgs fa, InferArgs ia) {
       s4v x;
       for (int j = 0; j < 4; ++j) x[j] = sX[pt * 16 + q + j];
       const f4v y = mlp_tile(m, l, x);
-      if (l < 16 && p0 + pt < n) mlp_store(ia, p0 + pt, y, sX[pt * 16 + 15] == 0);
+      if (l < 16 && p0 + pt < n) mlp_store(ia, p0 + pt, y, sX[pt * 16 + 15] == 0, bf2f(sX[pt * 16]));
     }
     __syncthreads();  // every wave done with the chunk before it is overwritten
   }

@@ -235,6 +235,12 @@ def test_surrogate_update_is_the_network_on_the_step_features(grid):
     print(f"  band of {p0 // np_} levels, {n1} surrogate points, {frac:.3f} out of range", flush=True)
     assert 0.001 < frac < 0.6
     yn = M.forward(params.astype(np.float64), Xn.astype(np.float64), "bf16")
+    # the explicit-diffusion limit Cs^2 |S| <= 0.2 h^2 / (dt delta^2) does not
+    # bind in this state (it bounds Cs^2 by more than the targets' range)
+    h = min(prm.lx / prm.nx, prm.ly / prm.ny, prm.lz / prm.nz)
+    delta = (prm.lx / prm.nx * prm.ly / prm.ny * prm.lz / prm.nz) ** (1 / 3)
+    smag = np.exp(M.bf16(Xn[:, 0].astype(np.float64)) * xs[0] + xm[0])
+    assert np.all(0.2 * h * h / (prm.dt * delta * delta) / smag > 1.0)
     cs_ref = np.clip(yn[:, 0] * ys[0] + ym[0], 1e-24, 1.0)   # the training targets' range
     be_ref = np.clip(yn[:, 1] * ys[1] + ym[1], 0.125, 8.0)
     lev = 1 + (p0 + np.arange(n1)) // np_
