@@ -372,9 +372,13 @@ def main(argv=None):
     ap.add_argument("--no-kernel-stats", action="store_true", help="skip the per-launch HIP-event pass")
     ap.add_argument("--serial", action="store_true",
                     help="issue each step on one stream (default: the SGS stage on a second stream)")
+    ap.add_argument("--weights", default=None, help="surrogate weights (.npy) instead of the committed ones")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first launch a copy kernel of MIB MiB (counter calibration)")
     args = ap.parse_args(argv)
+    if args.weights:
+        global WEIGHTS
+        WEIGHTS = pathlib.Path(args.weights)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): the whole domain lives on a single MI355X "
               f"and no multi-GPU path is built.", file=sys.stderr)

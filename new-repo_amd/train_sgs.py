@@ -94,6 +94,23 @@ def collect_samples(g, prm, n_updates: int, warm_cycles: int, stride: int, max_p
     return X[:filled], Y[:filled]
 
 
+def resample_by_strain(X, Y, power: float, seed: int = 0):
+    """Draw len(X) samples with replacement, each with probability
+    proportional to |S|^power (|S| from feature 0, log(|S| + 1e-12)): the
+    least-squares fit then weights a point by its share of the SGS stress
+    (|S|^2) or dissipation (|S|^3) instead of counting every point alike.
+    Inverse-CDF sampling on the device."""
+    import torch
+    logw = power * X[:, 0].double()
+    w = torch.exp(logw - logw.max())
+    cdf = torch.cumsum(w, 0)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    u = torch.rand(X.shape[0], device="cuda", dtype=torch.float64, generator=gen) * cdf[-1]
+    idx = torch.searchsorted(cdf, u).clamp_max(X.shape[0] - 1)
+    return X[idx], Y[idx]
+
+
 def train(g, X, Y, epochs: int = 3, batch: int = 65536, lr: float = 2e-3, seed: int = 0, xlim: float = 6.0,
           log=print):
     """Train on device samples; returns the compact weight vector and history."""
@@ -173,6 +190,8 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=1234, help="weight initialisation and shuffling seed")
     ap.add_argument("--ic-seed", type=int, default=4321,
                     help="seed of the initial field of the sampled run (distinct from the benchmark's 1234)")
+    ap.add_argument("--strain-power", type=float, default=0.0,
+                    help="resample the training points with probability proportional to |S|^p (0: as sampled)")
     ap.add_argument("--out", default=str(PKG_DIR / "weights" / "sgs_mlp.npy"))
     args = ap.parse_args(argv)
     pkg = _pkg()
@@ -189,13 +208,15 @@ def main(argv=None):
         print(f"samples: {X.shape[0]} from {args.updates + 1} dynamic updates ({time.time() - t0:.1f}s)", flush=True)
         per = X.shape[0] // (args.updates + 1)
         Xtr, Ytr, Xte, Yte = X[:per * args.updates], Y[:per * args.updates], X[per * args.updates:], Y[per * args.updates:]
+        if args.strain_power:
+            Xtr, Ytr = resample_by_strain(Xtr, Ytr, args.strain_power, args.seed)
         weights, hist = train(g, Xtr, Ytr, args.epochs, args.batch, args.lr, args.seed)
         ev = evaluate(g, Xte, Yte)
     out = pathlib.Path(args.out)
     out.parent.mkdir(parents=True, exist_ok=True)
     np.save(out, weights.astype(np.float32))
     meta = {"grid": [nx, ny, nz], "dt": args.dt, "ic_seed": args.ic_seed, "warm_cycles": args.warm_cycles, "gap_cycles": args.gap_cycles,
-            "stride": args.stride, "cs2_max": CS2_MAX, "beta_max": BETA_MAX,
+            "stride": args.stride, "strain_power": args.strain_power, "cs2_max": CS2_MAX, "beta_max": BETA_MAX,
             "train_samples": int(Xtr.shape[0]), "test_samples": int(Xte.shape[0]),
             "epochs": args.epochs, "batch": args.batch, "lr": args.lr, "held_out": ev,
             "final_loss": hist[-1][2] if hist else None, "seconds": time.time() - t0}
