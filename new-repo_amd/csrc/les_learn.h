@@ -41,6 +41,9 @@ static int mlp_alloc(Les& L) {
   return 0;
 }
 
+// K chunk of one wavefront in the weight-gradient GEMMs
+#define MLP_KCHUNK 128
+
 static int mlp_train_alloc(Les& L, long n) {
   MlpState* M = L.mlp;
   if (n <= M->cap) return 0;
@@ -50,29 +53,52 @@ static int mlp_train_alloc(Les& L, long n) {
   M->z1 = (float*)dalloc(L, 4 * 64 * c); M->z2 = (float*)dalloc(L, 4 * 64 * c); M->z3 = (float*)dalloc(L, 4 * 64 * c);
   M->c = (float*)dalloc(L, 4 * 64 * c); M->dh = (float*)dalloc(L, 4 * 64 * c);
   M->dyf = (float*)dalloc(L, 4 * 16 * c); M->dzf = (float*)dalloc(L, 4 * 64 * c);
-  M->part = (float*)dalloc(L, 4 * (size_t)(c / 512 + 2) * 64 * 64);
+  M->part = (float*)dalloc(L, 4 * (size_t)(c / MLP_KCHUNK + 2) * 64 * 64);
   M->lossp = (float*)dalloc(L, 4 * (size_t)(16 * c / 256 + 2));
   M->cap = c;
   if (L.alloc_failed) { les_set_error("out of memory (MLP training)"); return 3; }
   return 0;
 }
 
-// C[M x N] = A[M x K] B[K x N], split-K for long K
+// C[M x N] = A[M x K] B[K x N]: the weight gradients (K the batch) as
+// per-wavefront K chunks summed afterwards, the activation products as
+// 16-row strips; shapes outside those (not used by training) on the
+// one-tile kernel.
 static void gemm(Les& L, const short* A, long sam, long sak, const short* B, long sbk, long sbn, float* C, int M, int N,
                  int K) {
-  int kchunk = K, nsplit = 1;
-  if (K > 1024) {
-    kchunk = 512;
-    nsplit = (K + kchunk - 1) / kchunk;
-  }
   GemmArgs g;
   g.A = A; g.sam = sam; g.sak = sak; g.B = B; g.sbk = sbk; g.sbn = sbn;
-  g.C = nsplit > 1 ? L.mlp->part : C;
-  g.M = M; g.N = N; g.K = K; g.kchunk = kchunk;
-  hipLaunchKernelGGL(k_gemm_bf16, dim3((M / 16) * (N / 16), nsplit), dim3(64), 0, L.stream, g);
-  if (nsplit > 1) {
+  g.C = C;
+  g.M = M; g.N = N; g.K = K; g.kchunk = K;
+  const int mt = M / 16, nt = N / 16;
+  if (K > 1024 && (mt == 1 || mt == 4) && (nt == 1 || nt == 4)) {
+    g.kchunk = MLP_KCHUNK;
+    g.C = L.mlp->part;
+    const int nsplit = (K + MLP_KCHUNK - 1) / MLP_KCHUNK;
+    const dim3 grid((unsigned)((nsplit + 3) / 4));
+    if (mt == 4 && nt == 4) hipLaunchKernelGGL((k_gemm_kchunk<4, 4>), grid, dim3(256), 0, L.stream, g);
+    else if (mt == 4) hipLaunchKernelGGL((k_gemm_kchunk<4, 1>), grid, dim3(256), 0, L.stream, g);
+    else if (nt == 4) hipLaunchKernelGGL((k_gemm_kchunk<1, 4>), grid, dim3(256), 0, L.stream, g);
+    else hipLaunchKernelGGL((k_gemm_kchunk<1, 1>), grid, dim3(256), 0, L.stream, g);
     const long n = (long)M * N;
     hipLaunchKernelGGL(k_sum_splits, dim3(nblk(n)), dim3(LES_BLOCK), 0, L.stream, L.mlp->part, C, n, nsplit, 0);
+  } else if (K <= 1024 && (nt == 1 || nt == 4)) {
+    const dim3 grid((unsigned)((mt + 3) / 4));
+    if (nt == 4) hipLaunchKernelGGL((k_gemm_strip<4>), grid, dim3(256), 0, L.stream, g);
+    else hipLaunchKernelGGL((k_gemm_strip<1>), grid, dim3(256), 0, L.stream, g);
+  } else {
+    int kchunk = K, nsplit = 1;
+    if (K > 1024) {
+      kchunk = 512;
+      nsplit = (K + kchunk - 1) / kchunk;
+    }
+    g.kchunk = kchunk;
+    g.C = nsplit > 1 ? L.mlp->part : C;
+    hipLaunchKernelGGL(k_gemm_bf16, dim3((M / 16) * (N / 16), nsplit), dim3(64), 0, L.stream, g);
+    if (nsplit > 1) {
+      const long n = (long)M * N;
+      hipLaunchKernelGGL(k_sum_splits, dim3(nblk(n)), dim3(LES_BLOCK), 0, L.stream, L.mlp->part, C, n, nsplit, 0);
+    }
   }
 }
 

@@ -119,6 +119,77 @@ __global__ void __launch_bounds__(64) k_gemm_bf16(GemmArgs g) {
   for (int j = 0; j < 4; ++j) C[(long)(m0 + q + j) * g.N + n0 + r] = acc[j];
 }
 
+// Four consecutive-in-k bf16 operand values: one 8-byte load where they are
+// adjacent and aligned, else four 2-byte loads.
+__device__ __forceinline__ s4v load4(const short* p, long stride) {
+  if (stride == 1 && ((uintptr_t)p & 7) == 0) return *reinterpret_cast<const s4v*>(p);
+  s4v v;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) v[j] = p[(long)j * stride];
+  return v;
+}
+
+// The training GEMMs with few columns and short K (activations times a
+// weight matrix): each wavefront computes a 16-row strip and all NT 16-wide
+// column tiles, reusing its A fragment across them; four wavefronts per
+// block.  Same lane layout as k_gemm_bf16.
+template <int NT>
+__global__ void __launch_bounds__(256) k_gemm_strip(GemmArgs g) {
+  const int l = threadIdx.x & 63;
+  const long m0 = ((long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
+  if (m0 >= g.M) return;
+  const int r = l & 15, q = (l >> 4) * 4;
+  f4v acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) acc[t] = f4v{0.f, 0.f, 0.f, 0.f};
+  for (int k = 0; k < g.K; k += 16) {
+    const s4v a = load4(g.A + (m0 + r) * g.sam + (long)(k + q) * g.sak, g.sak);
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+      acc[t] = mfma16(a, load4(g.B + (long)(k + q) * g.sbk + (long)(16 * t + r) * g.sbn, g.sbk), acc[t]);
+  }
+#pragma unroll
+  for (int t = 0; t < NT; ++t)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) g.C[(m0 + q + j) * g.N + 16 * t + r] = acc[t][j];
+}
+
+// The weight-gradient GEMMs (M, N <= 64, K = the batch): wavefront w sums
+// the K chunk [w kchunk, (w + 1) kchunk) for every one of the MT x NT
+// output tiles into slab w of C (summed by k_sum_splits).
+template <int MT, int NT>
+__global__ void __launch_bounds__(256) k_gemm_kchunk(GemmArgs g) {
+  const int l = threadIdx.x & 63;
+  const long w = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const long k0 = w * g.kchunk;
+  if (k0 >= g.K) return;
+  const long k1 = k0 + g.kchunk < g.K ? k0 + g.kchunk : g.K;
+  const int r = l & 15, q = (l >> 4) * 4;
+  f4v acc[MT][NT];
+#pragma unroll
+  for (int i = 0; i < MT; ++i)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[i][t] = f4v{0.f, 0.f, 0.f, 0.f};
+  for (long k = k0; k < k1; k += 16) {
+    s4v a[MT], b[NT];
+#pragma unroll
+    for (int i = 0; i < MT; ++i) a[i] = load4(g.A + (long)(16 * i + r) * g.sam + (k + q) * g.sak, g.sak);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) b[t] = load4(g.B + (k + q) * g.sbk + (long)(16 * t + r) * g.sbn, g.sbk);
+#pragma unroll
+    for (int i = 0; i < MT; ++i)
+#pragma unroll
+      for (int t = 0; t < NT; ++t) acc[i][t] = mfma16(a[i], b[t], acc[i][t]);
+  }
+  float* C = g.C + w * g.M * g.N;
+#pragma unroll
+  for (int i = 0; i < MT; ++i)
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) C[(long)(16 * i + q + j) * g.N + 16 * t + r] = acc[i][t][j];
+}
+
 __global__ void k_sum_splits(const float* part, float* out, long n, int nsplit, int accumulate) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
