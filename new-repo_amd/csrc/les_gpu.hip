@@ -970,6 +970,9 @@ static void free_all(Les* L) {
   for (int i = 0; i < 6; ++i) hipEventDestroy(L->ph_ev[i]);
   hipEventDestroy(L->ev_fork);
   hipEventDestroy(L->ev_join);
+  hipEventDestroy(L->ev_feat0);
+  hipEventDestroy(L->ev_feat1);
+  hipStreamDestroy(L->stream3);
   hipStreamDestroy(L->stream2);
   hipStreamDestroy(L->stream);
 }
@@ -1032,8 +1035,17 @@ int les_init(const les_params* p) {
     les_set_error("stream");
     return 5;
   }
+  if (hipStreamCreate(&L->stream3) != hipSuccess) {
+    hipStreamDestroy(L->stream2);
+    hipStreamDestroy(L->stream);
+    delete L;
+    les_set_error("stream");
+    return 5;
+  }
   hipEventCreateWithFlags(&L->ev_fork, hipEventDisableTiming);
   hipEventCreateWithFlags(&L->ev_join, hipEventDisableTiming);
+  hipEventCreateWithFlags(&L->ev_feat0, hipEventDisableTiming);
+  hipEventCreateWithFlags(&L->ev_feat1, hipEventDisableTiming);
   {
     int dev = 0;
     L->ncu = 0;

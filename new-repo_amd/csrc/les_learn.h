@@ -10,6 +10,12 @@ struct MlpState {
   float* norm;     // xmean[12] xstd[12]
   float hn[29];    // host: xmean[12] xstd[12] ymean[2] ystd[2] xlim
   short* X;        // step features [npts][16]
+  // the surrogate's own workspaces (its chain may run beside the band's
+  // plain update, which uses the shared ones): R_ij, the 2-delta and
+  // 4-delta filtered S_ij and R_ij (fr[0..2], fr[3..11], fr[12..20]) and
+  // the transforms' spectra
+  float* fr[21];
+  void* fs[9];
   double* red;     // reduction partials
   long redn;
   int step;
@@ -34,6 +40,8 @@ static int mlp_alloc(Les& L) {
   M->norm = (float*)dalloc(L, 4 * 32);
   const long npt = L.np * (L.nz + 1);
   M->X = (short*)dalloc(L, 2 * 16 * (size_t)npt);
+  for (int i = 0; i < 21; ++i) M->fr[i] = (float*)dalloc(L, 4 * (size_t)L.nf);
+  for (int i = 0; i < 9; ++i) M->fs[i] = dalloc(L, 2 * 4 * (size_t)L.nsp);
   M->redn = 2 * (long)nblk(npt) + 64;
   M->red = (double*)dalloc(L, 8 * (size_t)M->redn);
   L.mlp = M;
@@ -181,7 +189,7 @@ static void learned_coeffs(Les& L, void* cs, void* beta, int eb, int k0) {
   const long p0 = (long)(k0 - 1) * L.np;
   const long nsel = (long)nlev * L.np;
   SgsArgs<T> a = sgs_args<T>(L);
-  T* rr[3] = {(T*)L.wk_real[0], (T*)L.wk_real[1], (T*)L.wk_real[2]};
+  T* rr[3] = {(T*)M->fr[0], (T*)M->fr[1], (T*)M->fr[2]};
   kt_begin(L, KT_MLP);
   hipLaunchKernelGGL((k_rij<T>), dim3(nblk(npt)), dim3(LES_BLOCK), 0, L.stream, a, rr[0], rr[1], rr[2]);
   kt_end(L, KT_MLP, (double)npt * 9.0 * sizeof(T), "k_rij", sizeof(T));
@@ -189,22 +197,22 @@ static void learned_coeffs(Les& L, void* cs, void* beta, int eb, int k0) {
                      F<T>(L, LES_S23), F<T>(L, LES_S33), rr[0], rr[1], rr[2]};
   T* f2[9];
   T* f4[9];
-  for (int i = 0; i < 9; ++i) { f2[i] = (T*)L.wk_real[15 + i]; f4[i] = (T*)L.wk_real[36 + i]; }
+  for (int i = 0; i < 9; ++i) { f2[i] = (T*)M->fr[3 + i]; f4[i] = (T*)M->fr[12 + i]; }
   const T* g2 = sizeof(T) == 8 ? (const T*)L.g2_64 : (const T*)L.g2_32;
   const T* g4 = sizeof(T) == 8 ? (const T*)L.g4_64 : (const T*)L.g4_32;
   for (int f0 = 0; f0 < 9; f0 += 3) {
     const T* src[3];
     Cplx<T>* sp[3];
-    for (int t = 0; t < 3; ++t) { src[t] = in9[f0 + t]; sp[t] = (Cplx<T>*)L.wk_spec[t]; }
+    for (int t = 0; t < 3; ++t) { src[t] = in9[f0 + t]; sp[t] = (Cplx<T>*)M->fs[t]; }
     rows_r2c<T, T>(L, src, sp, 3, k0, nlev, (T)1);
     for (int t = 0; t < 3; ++t)
-      cols<T>(L, COP_FILT2, sp[t], nullptr, (Cplx<T>*)L.wk_spec[3 + 2 * t], (Cplx<T>*)L.wk_spec[4 + 2 * t], nullptr, 2, 1,
-              1, k0, nlev, g2, g4);
+      cols<T>(L, COP_FILT2, sp[t], nullptr, (Cplx<T>*)M->fs[3 + 2 * t], (Cplx<T>*)M->fs[4 + 2 * t], nullptr, 2, 1, 1, k0,
+              nlev, g2, g4);
     const Cplx<T>* cs6[6];
     T* outs[6];
     for (int t = 0; t < 3; ++t) {
-      cs6[2 * t] = (Cplx<T>*)L.wk_spec[3 + 2 * t];
-      cs6[2 * t + 1] = (Cplx<T>*)L.wk_spec[4 + 2 * t];
+      cs6[2 * t] = (Cplx<T>*)M->fs[3 + 2 * t];
+      cs6[2 * t + 1] = (Cplx<T>*)M->fs[4 + 2 * t];
       outs[2 * t] = f2[f0 + t];
       outs[2 * t + 1] = f4[f0 + t];
     }
@@ -285,6 +293,9 @@ static int lag_levels(const Les& L, int jtt) {
   return L.nband > 0 ? (L.nband > 2 ? L.nband : 2) : -1;
 }
 
+#ifndef LES_FEAT_STREAM
+#define LES_FEAT_STREAM 1
+#endif
 // Coefficient update of the learned mode on a dynamic-procedure step.
 // The first update, judge steps and every step until weights are loaded run
 // the plain procedure on the whole column; the surrogate is evaluated beside
@@ -335,8 +346,24 @@ static void learned_update(Les& L, int jtt) {
     }
     return;
   }
-  if (L.nband > 0) lasd_update<T>(L, jtt, L.nband);  // also clears the u_lag sums
-  learned_coeffs<T>(L, L.f[LES_CS_OPT2], L.f[LES_BETA], sizeof(T), k0);
+  if (LES_FEAT_STREAM && L.overlap && !L.timing) {
+    // The surrogate's chain (its own workspaces; coefficients of levels
+    // k0..nz+1) and the band's plain update (levels 1..nband, F_* and the
+    // shared workspaces) read the same S_ij and write disjoint data: the
+    // first runs on a third stream beside the second.
+    hipEventRecord(L.ev_feat0, L.stream);
+    hipStreamWaitEvent(L.stream3, L.ev_feat0, 0);
+    hipStream_t s = L.stream;
+    L.stream = L.stream3;
+    learned_coeffs<T>(L, L.f[LES_CS_OPT2], L.f[LES_BETA], sizeof(T), k0);
+    hipEventRecord(L.ev_feat1, L.stream3);
+    L.stream = s;
+    if (L.nband > 0) lasd_update<T>(L, jtt, L.nband);  // also clears the u_lag sums
+    hipStreamWaitEvent(L.stream, L.ev_feat1, 0);
+  } else {
+    if (L.nband > 0) lasd_update<T>(L, jtt, L.nband);  // also clears the u_lag sums
+    learned_coeffs<T>(L, L.f[LES_CS_OPT2], L.f[LES_BETA], sizeof(T), k0);
+  }
   if (L.nband == 0) {
     const size_t tb = sizeof(T) * (size_t)L.nf;
     kt_begin(L, KT_POINTWISE);
@@ -504,17 +531,17 @@ int les_sgs_samples(void* d_x, void* d_y, long max_samples, long* n_written, int
   learned_coeffs<float>(*L, nullptr, nullptr, 4, k0);  // fills the filtered fields (feature output unused)
   M->hn[28] = save;
   FeatArgs fa;
-  const float* rr[3] = {(const float*)L->wk_real[0], (const float*)L->wk_real[1], (const float*)L->wk_real[2]};
+  const float* rr[3] = {M->fr[0], M->fr[1], M->fr[2]};
   const int sid[6] = {LES_S11, LES_S12, LES_S13, LES_S22, LES_S23, LES_S33};
   for (int i = 0; i < 6; ++i) {
     fa.s[0][i] = (const float*)L->f[sid[i]];
-    fa.s[1][i] = (const float*)L->wk_real[15 + i];
-    fa.s[2][i] = (const float*)L->wk_real[36 + i];
+    fa.s[1][i] = (const float*)L->mlp->fr[3 + i];
+    fa.s[2][i] = (const float*)L->mlp->fr[12 + i];
   }
   for (int i = 0; i < 3; ++i) {
     fa.r[0][i] = rr[i];
-    fa.r[1][i] = (const float*)L->wk_real[21 + i];
-    fa.r[2][i] = (const float*)L->wk_real[42 + i];
+    fa.r[1][i] = (const float*)L->mlp->fr[9 + i];
+    fa.r[2][i] = (const float*)L->mlp->fr[18 + i];
   }
   fa.np = L->np; fa.nz = L->nz; fa.dz = (float)L->dz; fa.delta = (float)L->delta;
   fa.xmean = M->norm; fa.xstd = M->norm + 12; fa.xlim = 1e30f;

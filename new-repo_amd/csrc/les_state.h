@@ -28,6 +28,8 @@ struct Les {
   hipStream_t stream;
   hipStream_t stream2;   // second stream of the step: the SGS stage beside the convection
   hipEvent_t ev_fork, ev_join;
+  hipStream_t stream3;   // learned mode: the surrogate's chain beside the band's plain update
+  hipEvent_t ev_feat0, ev_feat1;
   int overlap;           // les_step on two streams (les_set_overlap)
   int mixed;             // flow kernels in FP32
 
