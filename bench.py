@@ -61,6 +61,9 @@ WEIGHTS = ROOT / "new-repo_amd" / "weights" / "sgs_mlp.npy"
 PROFILES = ROOT / "profiles"  # pmc_traffic_<variant>.json: counter traffic per kernel
 
 
+LEARNED_ZMIN = 4.0  # the library's default band: z / delta < 4
+
+
 def load_weights():
     return np.load(WEIGHTS, allow_pickle=False) if WEIGHTS.exists() else None
 
@@ -140,7 +143,7 @@ def judge_amortisation(g, prm, first, steps, ms_measured):
 
 
 def run_variant(pkg, P, ic, precision, steps, warmup, nx, ny, nz, kernel_stats=True, seed=1234, serial=False):
-    prm = P.LesParams(nx=nx, ny=ny, nz=nz, dt=DT, precision=precision, judge_every=50)
+    prm = P.LesParams(nx=nx, ny=ny, nz=nz, dt=DT, precision=precision, judge_every=50, learned_zmin=LEARNED_ZMIN)
     prm.validate()
     u, v, w = ic.log_law_ic(nx, ny, nz, prm.lz, prm.z0, 0.10, seed=seed)
     res = {}
@@ -232,7 +235,7 @@ def matched_error(pkg, P, ic, nx, ny, nz, steps, variants, seed=1234, window=Non
     prof = {}
     t_all = time.perf_counter()
     for name in ["fp64"] + list(variants):
-        prm = P.LesParams(nx=nx, ny=ny, nz=nz, dt=DT, precision=precs[name], judge_every=50)
+        prm = P.LesParams(nx=nx, ny=ny, nz=nz, dt=DT, precision=precs[name], judge_every=50, learned_zmin=LEARNED_ZMIN)
         u, v, w = ic.log_law_ic(nx, ny, nz, prm.lz, prm.z0, 0.10, seed=seed)
         if name == "fp64_perturbed":
             rng = np.random.default_rng(seed + 1)
@@ -373,12 +376,16 @@ def main(argv=None):
     ap.add_argument("--serial", action="store_true",
                     help="issue each step on one stream (default: the SGS stage on a second stream)")
     ap.add_argument("--weights", default=None, help="surrogate weights (.npy) instead of the committed ones")
+    ap.add_argument("--learned-zmin", type=float, default=None,
+                    help="height z / delta below which the learned mode keeps the plain procedure (default: the library's)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first launch a copy kernel of MIB MiB (counter calibration)")
     args = ap.parse_args(argv)
+    global WEIGHTS, LEARNED_ZMIN
     if args.weights:
-        global WEIGHTS
         WEIGHTS = pathlib.Path(args.weights)
+    if args.learned_zmin is not None:
+        LEARNED_ZMIN = args.learned_zmin
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): the whole domain lives on a single MI355X "
               f"and no multi-GPU path is built.", file=sys.stderr)
