@@ -61,7 +61,11 @@ WEIGHTS = ROOT / "new-repo_amd" / "weights" / "sgs_mlp.npy"
 PROFILES = ROOT / "profiles"  # pmc_traffic_<variant>.json: counter traffic per kernel
 
 
-LEARNED_ZMIN = 4.0  # the library's default band: z / delta < 4
+LEARNED_ZMIN = None  # the learned mode's near-wall band: the library's default unless given
+
+
+def zmin_kw():
+    return {} if LEARNED_ZMIN is None else {"learned_zmin": LEARNED_ZMIN}
 
 
 def load_weights():
@@ -143,7 +147,7 @@ def judge_amortisation(g, prm, first, steps, ms_measured):
 
 
 def run_variant(pkg, P, ic, precision, steps, warmup, nx, ny, nz, kernel_stats=True, seed=1234, serial=False):
-    prm = P.LesParams(nx=nx, ny=ny, nz=nz, dt=DT, precision=precision, judge_every=50, learned_zmin=LEARNED_ZMIN)
+    prm = P.LesParams(nx=nx, ny=ny, nz=nz, dt=DT, precision=precision, judge_every=50, **zmin_kw())
     prm.validate()
     u, v, w = ic.log_law_ic(nx, ny, nz, prm.lz, prm.z0, 0.10, seed=seed)
     res = {}
@@ -235,7 +239,7 @@ def matched_error(pkg, P, ic, nx, ny, nz, steps, variants, seed=1234, window=Non
     prof = {}
     t_all = time.perf_counter()
     for name in ["fp64"] + list(variants):
-        prm = P.LesParams(nx=nx, ny=ny, nz=nz, dt=DT, precision=precs[name], judge_every=50, learned_zmin=LEARNED_ZMIN)
+        prm = P.LesParams(nx=nx, ny=ny, nz=nz, dt=DT, precision=precs[name], judge_every=50, **zmin_kw())
         u, v, w = ic.log_law_ic(nx, ny, nz, prm.lz, prm.z0, 0.10, seed=seed)
         if name == "fp64_perturbed":
             rng = np.random.default_rng(seed + 1)
@@ -463,7 +467,9 @@ def main(argv=None):
         "data": "synthetic: seeded log-law initial field (u* = 1, z0 = 1e-4, +-10% perturbation in z <= 0.5)",
         "config": {"workload": f"LES time step, {head}, LASD cs_count=5, dt={DT}", "grid": [nx, ny, nz], "variant": head,
                    "parallelism": "single GPU, whole domain resident",
-                   "schedule": "one stream" if args.serial else "two streams (SGS stage beside the convection)"},
+                   "schedule": "one stream" if args.serial else "two streams (SGS stage beside the convection)",
+                   **({"learned_zmin": P.LesParams(**zmin_kw()).learned_zmin, "judge_every": 50}
+                      if head == "learned" else {})},
         "roofline": roof,
         "step_roofline": {"bytes_per_point_step": STEP_BYTES[step_key],
                           "achieved_GBs": step_bytes / (hr["ms_per_step"] * 1e-3) / 1e9,

@@ -49,7 +49,9 @@ typedef struct les_params {
   double lx, ly, lz, dt, z0, pfx, pfy, cs0, coriol, ug, vg, ubc_u, ubc_v, ubc_w;
   double re;       /* Reynolds number (physical_param Re), used by sgs_model dns */
   double learned_zmin; /* learned SGS: levels with z / delta < learned_zmin (the
-                          near-wall band) keep the plain procedure on every update */
+                          near-wall band) keep the plain procedure on every update
+                          (16 in the parameter modules: the band whose long-run
+                          mean profiles stay inside the FP64 noise floor) */
 } les_params;
 
 /* Field ids (system_variables, compute_sgs, compute_LagScaleDep globals). */

@@ -46,7 +46,7 @@ module parameters_IO
   real(rprec), public :: dudz0 = 0._rprec, z_turb = 0.5_rprec, vnf = 1._rprec
   ! les_gpu_param (device variant)
   integer, public :: precision = 0, halo_mode = 0, judge_every = 50
-  real(rprec), public :: learned_zmin = 4._rprec
+  real(rprec), public :: learned_zmin = 16._rprec
   character(8), public :: dealias = 'PADD'
   ! derived
   real(rprec), public :: dx, dy, dz

@@ -61,7 +61,7 @@ class LesParams:
     vg: float = 0.0
     precision: int = PREC_FP64
     judge_every: int = 50       # learned SGS: plain LASD judge cadence
-    learned_zmin: float = 4.0   # learned SGS: plain LASD on levels with z / delta below this
+    learned_zmin: float = 16.0  # learned SGS: plain LASD on levels with z / delta below this
     re: float = 1.0e5           # physical_param Re (the dns stress uses 2/Re)
 
     # ---- vectors shared with the CPU oracle (orc_init argument order)

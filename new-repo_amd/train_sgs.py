@@ -216,7 +216,8 @@ def main(argv=None):
     out.parent.mkdir(parents=True, exist_ok=True)
     np.save(out, weights.astype(np.float32))
     meta = {"grid": [nx, ny, nz], "dt": args.dt, "ic_seed": args.ic_seed, "warm_cycles": args.warm_cycles, "gap_cycles": args.gap_cycles,
-            "stride": args.stride, "strain_power": args.strain_power, "cs2_max": CS2_MAX, "beta_max": BETA_MAX,
+            "stride": args.stride, "strain_power": args.strain_power, "learned_zmin": prm.learned_zmin,
+            "cs2_max": CS2_MAX, "beta_max": BETA_MAX,
             "train_samples": int(Xtr.shape[0]), "test_samples": int(Xte.shape[0]),
             "epochs": args.epochs, "batch": args.batch, "lr": args.lr, "held_out": ev,
             "final_loss": hist[-1][2] if hist else None, "seconds": time.time() - t0}

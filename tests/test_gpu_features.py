@@ -23,7 +23,7 @@ def test_features_match_the_restatement(grid):
     torch = pytest.importorskip("torch")
     pkg, P, ic, O = load_pkg(), load_params(), load_ic(), load_oracle()
     nx, ny, nz = grid
-    prm = P.LesParams(nx=nx, ny=ny, nz=nz, precision=P.PREC_LEARNED, dt=1e-4)
+    prm = P.LesParams(nx=nx, ny=ny, nz=nz, precision=P.PREC_LEARNED, dt=1e-4, learned_zmin=4.0)
     u, v, w = ic.log_law_ic(nx, ny, nz, prm.lz, prm.z0, 0.1, seed=31)
     o = O.Oracle(P.LesParams(nx=nx, ny=ny, nz=nz))
     G2, G4 = o.table("G2"), o.table("G4")

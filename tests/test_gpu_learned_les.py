@@ -6,11 +6,14 @@ band, fallback and judge schedule) are both run for 3000 steps from the
 FP64 run's initial field, and their time-averaged mean-velocity and
 resolved-stress profiles over steps 2401-3000 (the reference's
 post-processing of its running averages, reduced on the device) are
-compared with the FP64 run's.
+compared with the FP64 run's.  The band here is z / delta < 4, where the
+surrogate covers 95% of the levels; with the default band (z / delta < 16)
+both variants stay near the FP64 noise floor (3000 steps: surrogate 0.25% /
+2.4%, constant 0.29% / 3.0%).
 
 Measured (MI355X): surrogate 0.31% / 3.7%, constant 0.88% / 14.5% (largest
 deviation of mean u, relative; of R13 + C13, relative to its peak).  Over
-10000 steps (window 8001-10000) the figures are 2.2% / 11.7% against
+10000 steps (window 8001-10000) the figures are 2.2-2.7% / 12-13% against
 5.0% / 38%, with two FP64 runs whose initial fields differ by 1e-9
 relative apart by 0.57% / 3.0% (the noise floor of the window)."""
 import numpy as np
@@ -55,7 +58,7 @@ def test_surrogate_beats_constant_coefficients_in_the_les():
     ref = None
     for name, prec, wts in (("fp64", P.PREC_FP64, None), ("learned", P.PREC_LEARNED, weights),
                             ("constant", P.PREC_LEARNED, const)):
-        prm = P.LesParams(nx=nx, ny=ny, nz=nz, dt=1e-4, precision=prec, judge_every=50)
+        prm = P.LesParams(nx=nx, ny=ny, nz=nz, dt=1e-4, precision=prec, judge_every=50, learned_zmin=4.0)
         u, v, w = ic.log_law_ic(nx, ny, nz, prm.lz, prm.z0, 0.10, seed=1234)
         print(f"{name}: {STEPS} steps at 256^3", flush=True)
         U, R = _profiles(pkg, prm, u, v, w, wts)

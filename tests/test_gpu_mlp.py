@@ -93,7 +93,7 @@ def test_learned_step_with_judge_beats_constant_coefficient():
     torch = pytest.importorskip("torch")
     pkg, P, ic = load_pkg(), load_params(), load_ic()
     tr = _train_mod()
-    prm = P.LesParams(nx=32, ny=32, nz=16, precision=P.PREC_LEARNED, judge_every=10)
+    prm = P.LesParams(nx=32, ny=32, nz=16, precision=P.PREC_LEARNED, judge_every=10, learned_zmin=4.0)
     u, v, w = ic.log_law_ic(prm.nx, prm.ny, prm.nz, prm.lz, prm.z0, 0.1, seed=8)
     print("learned step: sample, train, run with judge", flush=True)
     with pkg.LesGpu(prm) as g:
@@ -202,7 +202,8 @@ def test_surrogate_update_is_the_network_on_the_step_features(grid):
     # of the in-step inference loops over several 16-point tiles.
     torch = pytest.importorskip("torch")
     pkg, P, ic = load_pkg(), load_params(), load_ic()
-    prm = P.LesParams(nx=grid[0], ny=grid[1], nz=grid[2], precision=P.PREC_LEARNED, judge_every=1000, dt=1e-4)
+    prm = P.LesParams(nx=grid[0], ny=grid[1], nz=grid[2], precision=P.PREC_LEARNED, judge_every=1000, dt=1e-4,
+                      learned_zmin=4.0)
     u, v, w = ic.log_law_ic(prm.nx, prm.ny, prm.nz, prm.lz, prm.z0, 0.1, seed=21)
     np_, nz = prm.nx * prm.ny, prm.nz
     print("surrogate update vs CPU forward pass", flush=True)
