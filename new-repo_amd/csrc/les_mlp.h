@@ -407,7 +407,7 @@ __device__ __forceinline__ void mlp_store(const InferArgs& a, long p, f4v y, boo
   float cap = MLP_CS2_MAX;
   if (a.nu_cap > 0.0f) {
     const float smag = __expf(x0 * a.s_std + a.s_mean);
-    const float lim = a.nu_cap / smag;
+    const float lim = a.nu_cap * __builtin_amdgcn_rcpf(smag);
     cap = lim < cap ? lim : cap;
   }
   if (in_range) {
@@ -500,7 +500,12 @@ struct FeatArgs {
   long p0;      // first point (offset from level 1): levels below are skipped
 };
 
+// The surrogate's inputs are FP32 approximations of the features (checked
+// against an FP64 restatement to a relative L2 of 1e-4): multiply-adds are
+// contracted here, and divisions and logarithms use the hardware
+// reciprocal and log2, unlike the FP64 paths of the step.
 __device__ __forceinline__ void invariants(const float* s, const float* rr, float* out) {
+#pragma clang fp contract(fast)
   // S symmetric, R antisymmetric (R21 = -R12, ...)
   const float S[3][3] = {{s[0], s[1], s[2]}, {s[1], s[3], s[4]}, {s[2], s[4], s[5]}};
   const float R[3][3] = {{0.f, rr[0], rr[1]}, {-rr[0], 0.f, rr[2]}, {-rr[1], -rr[2], 0.f}};
@@ -519,9 +524,9 @@ __device__ __forceinline__ void invariants(const float* s, const float* rr, floa
     for (int j = 0; j < 3; ++j) { S3 += S2[i][j] * S[j][i]; SR2 += S[i][j] * R2[j][i]; }
   const float mag = sqrtf(2.f * SS);
   out[0] = mag;
-  out[1] = RR / (SS + 1e-20f);
-  out[2] = S3 / (mag * mag * mag + 1e-30f);
-  out[3] = SR2 / (mag * SS + 1e-30f);
+  out[1] = RR * __builtin_amdgcn_rcpf(SS + 1e-20f);
+  out[2] = S3 * __builtin_amdgcn_rcpf(mag * mag * mag + 1e-30f);
+  out[3] = SR2 * __builtin_amdgcn_rcpf(mag * SS + 1e-30f);
 }
 
 // the 12 raw features of point p (offset from level 1)
@@ -536,20 +541,21 @@ __device__ __forceinline__ void feat_point(const FeatArgs& a, long p, float* f) 
     invariants(s, r, inv[sc]);
   }
   const float z = (k == 1) ? 0.5f * a.dz : (float)(k - 1) * a.dz;
-  f[0] = logf(inv[0][0] + 1e-12f);
+  const float l0 = __logf(inv[0][0] + 1e-12f);
+  f[0] = l0;
   f[1] = inv[0][1]; f[2] = inv[0][2]; f[3] = inv[0][3];
-  f[4] = logf((inv[1][0] + 1e-12f) / (inv[0][0] + 1e-12f));
+  f[4] = __logf(inv[1][0] + 1e-12f) - l0;
   f[5] = inv[1][1]; f[6] = inv[1][2]; f[7] = inv[1][3];
-  f[8] = logf((inv[2][0] + 1e-12f) / (inv[0][0] + 1e-12f));
+  f[8] = __logf(inv[2][0] + 1e-12f) - l0;
   f[9] = inv[2][1]; f[10] = inv[2][2];
-  f[11] = logf(z / a.delta);
+  f[11] = __logf(z * __builtin_amdgcn_rcpf(a.delta));
 }
 
 // normalised bf16 features of one point, slot 15 = out-of-range flag
 __device__ __forceinline__ void feat_normalise(const FeatArgs& a, const float* f, short* X) {
   bool out = false;
   for (int i = 0; i < 12; ++i) {
-    const float x = (f[i] - a.xmean[i]) / a.xstd[i];
+    const float x = (f[i] - a.xmean[i]) * __builtin_amdgcn_rcpf(a.xstd[i]);
     if (!(fabsf(x) <= a.xlim)) out = true;
     X[i] = f2bf(x);
   }
