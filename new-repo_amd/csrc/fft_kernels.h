@@ -14,6 +14,7 @@
 
 #define LES_BLOCK 256
 #define LES_MAX_FIELDS 6
+#define LES_COL_JOBS 3
 
 // ------------------------------------------------------------------ row r2c
 // Load modes of the real input.
@@ -232,8 +233,10 @@ enum { COP_GRAD = 0, COP_DIV = 1, COP_FILT2 = 2, COP_FILT1 = 3, COP_COPY = 4, CO
 
 template <typename T>
 struct ColArgs {
-  const Cplx<T>* in[2];
-  Cplx<T>* out[3];
+  // per job (blockIdx.y of the register-resident kernel; job 0 elsewhere):
+  // the same op on up to LES_COL_JOBS independent input/output sets
+  const Cplx<T>* in[LES_COL_JOBS][2];
+  Cplx<T>* out[LES_COL_JOBS][3];
   const T* kkx;
   const T* kky;
   const T* g0;
@@ -290,7 +293,7 @@ __device__ __forceinline__ void cols_body(const ColArgs<T>& a, unsigned char* sm
   if (nc > CB) nc = CB;
   const long pbase = (long)k * ny * a.lhx + c0;
   for (int t = 0; t < a.nin; ++t) {
-    const Cplx<T>* __restrict__ src = a.in[t] + pbase;
+    const Cplx<T>* __restrict__ src = a.in[0][t] + pbase;
     Cplx<T>* X = t == 0 ? X0 : X1;
     for (int idx = threadIdx.x; idx < CB * ny; idx += blockDim.x) {
       const int j = idx / CB, c = idx % CB;
@@ -315,7 +318,7 @@ __device__ __forceinline__ void cols_body(const ColArgs<T>& a, unsigned char* sm
       if (a.inv) fft(y, true);
     }
     __syncthreads();
-    Cplx<T>* __restrict__ dst = a.out[o] + pbase;
+    Cplx<T>* __restrict__ dst = a.out[0][o] + pbase;
     for (int idx = threadIdx.x; idx < CB * ny; idx += blockDim.x) {
       const int j = idx / CB, c = idx % CB;
       if (c < nc) dst[(long)j * a.lhx + c] = Y[c * LS + j];
@@ -355,7 +358,7 @@ __device__ __forceinline__ void cols_body_pipe(const ColArgs<T>& a, unsigned cha
 #pragma unroll
         for (int e = 0; e < E; ++e) {
           const int idx = threadIdx.x + e * LES_BLOCK, j = idx / CB, c = idx % CB;
-          pre[in][e] = c < nc ? a.in[in][pb + (long)j * a.lhx + c] : Cplx<T>{(T)0, (T)0};
+          pre[in][e] = c < nc ? a.in[0][in][pb + (long)j * a.lhx + c] : Cplx<T>{(T)0, (T)0};
         }
       }
     }
@@ -396,7 +399,7 @@ __device__ __forceinline__ void cols_body_pipe(const ColArgs<T>& a, unsigned cha
         if (a.inv) fft(y, true);
       }
       block_sync_lds();
-      Cplx<T>* __restrict__ dst = a.out[o] + pbase;
+      Cplx<T>* __restrict__ dst = a.out[0][o] + pbase;
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         const int idx = threadIdx.x + e * LES_BLOCK, j = idx / CB, c = idx % CB;
@@ -558,6 +561,7 @@ __global__ void __launch_bounds__(LES_BLOCK) k_cols_c16(ColArgs<T> a, const Cplx
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Cplx<T>* ex = reinterpret_cast<Cplx<T>*>(smem_raw);
   const int c = threadIdx.x & 15, ar = threadIdx.x >> 4;  // column in the tile, row class
+  const int job = blockIdx.y;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int k = a.k0 + tile / a.ncb;
   const int c0 = (tile % a.ncb) * 16;
@@ -566,7 +570,7 @@ __global__ void __launch_bounds__(LES_BLOCK) k_cols_c16(ColArgs<T> a, const Cplx
   const long base = (long)k * 256 * a.lhx + kx;
   Cplx<T> X0[16], X1[16];
 #pragma unroll
-  for (int t = 0; t < 16; ++t) X0[t] = valid ? a.in[0][base + (long)(ar + 16 * t) * a.lhx] : Cplx<T>{(T)0, (T)0};
+  for (int t = 0; t < 16; ++t) X0[t] = valid ? a.in[job][0][base + (long)(ar + 16 * t) * a.lhx] : Cplx<T>{(T)0, (T)0};
   if constexpr (NIN == 2 && OP == COP_DIV && NOUT == 1) {
     // divergence: i kx X0 is formed before the second input is loaded (the
     // transforms' barriers keep its loads after), so only one input line
@@ -581,7 +585,7 @@ __global__ void __launch_bounds__(LES_BLOCK) k_cols_c16(ColArgs<T> a, const Cplx
       X0[t] = c_ik(a.kkx[(long)j * a.lhx + kx], nyq ? zero : X0[t]);
     }
 #pragma unroll
-    for (int t = 0; t < 16; ++t) X1[t] = valid ? a.in[1][base + (long)(ar + 16 * t) * a.lhx] : zero;
+    for (int t = 0; t < 16; ++t) X1[t] = valid ? a.in[job][1][base + (long)(ar + 16 * t) * a.lhx] : zero;
     if (a.fwd) c16_fft(X1, twf, false, ex, c, ar);
 #pragma unroll
     for (int t = 0; t < 16; ++t) {
@@ -591,7 +595,7 @@ __global__ void __launch_bounds__(LES_BLOCK) k_cols_c16(ColArgs<T> a, const Cplx
     }
     if (a.inv) c16_fft(X0, twf, true, ex, c, ar);
     if (valid) {
-      Cplx<T>* __restrict__ dst = a.out[0] + base;
+      Cplx<T>* __restrict__ dst = a.out[job][0] + base;
 #pragma unroll
       for (int t = 0; t < 16; ++t) dst[(long)(ar + 16 * t) * a.lhx] = X0[t];
     }
@@ -599,7 +603,7 @@ __global__ void __launch_bounds__(LES_BLOCK) k_cols_c16(ColArgs<T> a, const Cplx
   }
   if constexpr (NIN > 1) {
 #pragma unroll
-    for (int t = 0; t < 16; ++t) X1[t] = valid ? a.in[1][base + (long)(ar + 16 * t) * a.lhx] : Cplx<T>{(T)0, (T)0};
+    for (int t = 0; t < 16; ++t) X1[t] = valid ? a.in[job][1][base + (long)(ar + 16 * t) * a.lhx] : Cplx<T>{(T)0, (T)0};
   }
   if (a.fwd) {
     c16_fft(X0, twf, false, ex, c, ar);
@@ -612,7 +616,7 @@ __global__ void __launch_bounds__(LES_BLOCK) k_cols_c16(ColArgs<T> a, const Cplx
     for (int t = 0; t < 16; ++t) Y[t] = c16_op<T, OP>(a, o, kx, ar + 16 * t, X0[t], NIN > 1 ? X1[t] : X0[t]);
     if (a.inv) c16_fft(Y, twf, true, ex, c, ar);
     if (valid) {
-      Cplx<T>* __restrict__ dst = a.out[o] + base;
+      Cplx<T>* __restrict__ dst = a.out[job][o] + base;
 #pragma unroll
       for (int t = 0; t < 16; ++t) dst[(long)(ar + 16 * t) * a.lhx] = Y[t];
     }

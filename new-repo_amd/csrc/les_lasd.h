@@ -176,11 +176,10 @@ static void lasd_update(Les& L, int jtt, int nlev) {
     Cplx<T>* sp[3];
     for (int t = 0; t < 3; ++t) { src[t] = in[f0 + t]; sp[t] = (Cplx<T>*)L.wk_spec[t]; }
     rows_r2c<T, T>(L, src, sp, 3, 1, nlev, (T)1);
-    for (int t = 0; t < 3; ++t) {
-      Cplx<T>* o2 = (Cplx<T>*)L.wk_spec[3 + 2 * t];
-      Cplx<T>* o4 = (Cplx<T>*)L.wk_spec[4 + 2 * t];
-      cols<T>(L, COP_FILT2, sp[t], nullptr, o2, o4, nullptr, 2, 1, 1, 1, nlev, g2, g4);
-    }
+    Cplx<T>* o2[3];
+    Cplx<T>* o4[3];
+    for (int t = 0; t < 3; ++t) { o2[t] = (Cplx<T>*)L.wk_spec[3 + 2 * t]; o4[t] = (Cplx<T>*)L.wk_spec[4 + 2 * t]; }
+    cols_jobs<T>(L, COP_FILT2, 3, sp, nullptr, o2, o4, nullptr, 2, 1, 1, 1, nlev, g2, g4);
     const Cplx<T>* cs[6];
     T* outs[6];
     for (int t = 0; t < 3; ++t) {
@@ -368,8 +367,9 @@ static void dynsmag_update(Les& L) {
       outs[t] = fl[f0 + t];
     }
     rows_r2c<T, T>(L, src, sp, 3, 1, nlev, (T)1);
-    for (int t = 0; t < 3; ++t)
-      cols<T>(L, COP_FILT1, sp[t], nullptr, (Cplx<T>*)L.wk_spec[3 + t], nullptr, nullptr, 1, 1, 1, 1, nlev, g2);
+    Cplx<T>* fo[3];
+    for (int t = 0; t < 3; ++t) fo[t] = (Cplx<T>*)L.wk_spec[3 + t];
+    cols_jobs<T>(L, COP_FILT1, 3, sp, nullptr, fo, nullptr, nullptr, 1, 1, 1, 1, nlev, g2);
     rows_c2r<T, T>(L, cs, outs, 3, 1, nlev);
   }
   GermanoArgs<T> ga;

@@ -205,9 +205,10 @@ static void learned_coeffs(Les& L, void* cs, void* beta, int eb, int k0) {
     Cplx<T>* sp[3];
     for (int t = 0; t < 3; ++t) { src[t] = in9[f0 + t]; sp[t] = (Cplx<T>*)M->fs[t]; }
     rows_r2c<T, T>(L, src, sp, 3, k0, nlev, (T)1);
-    for (int t = 0; t < 3; ++t)
-      cols<T>(L, COP_FILT2, sp[t], nullptr, (Cplx<T>*)M->fs[3 + 2 * t], (Cplx<T>*)M->fs[4 + 2 * t], nullptr, 2, 1, 1, k0,
-              nlev, g2, g4);
+    Cplx<T>* o2[3];
+    Cplx<T>* o4[3];
+    for (int t = 0; t < 3; ++t) { o2[t] = (Cplx<T>*)M->fs[3 + 2 * t]; o4[t] = (Cplx<T>*)M->fs[4 + 2 * t]; }
+    cols_jobs<T>(L, COP_FILT2, 3, sp, nullptr, o2, o4, nullptr, 2, 1, 1, k0, nlev, g2, g4);
     const Cplx<T>* cs6[6];
     T* outs[6];
     for (int t = 0; t < 3; ++t) {

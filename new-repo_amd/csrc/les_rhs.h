@@ -137,8 +137,9 @@ static void spectral_multiply(Les& L, T* const* src, TO* const* dst, int nf, int
       o[t] = dst[f0 + t];
     }
     rows_r2c<T, T>(L, s, sp, m, 1, nlev, (T)1);
-    for (int t = 0; t < m; ++t)
-      cols<T>(L, COP_FILT1, sp[t], nullptr, (Cplx<T>*)L.wk_spec[3 + t], nullptr, nullptr, 1, 1, 1, 1, nlev, g);
+    Cplx<T>* fo[3];
+    for (int t = 0; t < m; ++t) fo[t] = (Cplx<T>*)L.wk_spec[3 + t];
+    cols_jobs<T>(L, COP_FILT1, m, sp, nullptr, fo, nullptr, nullptr, 1, 1, 1, 1, nlev, g);
     rows_c2r<T, TO>(L, cs, o, m, 1, nlev);
   }
 }
@@ -430,9 +431,10 @@ static void div_stress_spectra(Les& L) {
   const T* src[5] = {F<T>(L, LES_TXX), F<T>(L, LES_TXY), F<T>(L, LES_TYY), F<T>(L, LES_TXZ), F<T>(L, LES_TYZ)};
   const T c = (T)(1.0 / (double)(L.nx * L.ny));
   rows_r2c<T, T>(L, src, sp, 5, 1, L.nz + 1, c);
-  cols<T>(L, COP_DIV, sp[0], sp[1], sp[5], nullptr, nullptr, 1, 1, 1, 1, L.nz + 1);
-  cols<T>(L, COP_DIV, sp[1], sp[2], sp[6], nullptr, nullptr, 1, 1, 1, 1, L.nz + 1);
-  cols<T>(L, COP_DIV, sp[3], sp[4], sp[7], nullptr, nullptr, 1, 1, 1, 1, L.nz + 1);
+  const Cplx<T>* a0[3] = {sp[0], sp[1], sp[3]};
+  const Cplx<T>* a1[3] = {sp[1], sp[2], sp[4]};
+  Cplx<T>* d[3] = {sp[5], sp[6], sp[7]};
+  cols_jobs<T>(L, COP_DIV, 3, a0, a1, d, nullptr, nullptr, 1, 1, 1, 1, L.nz + 1);
 }
 
 // Second half: inverse transforms with the rhs update as their store.
