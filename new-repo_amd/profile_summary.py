@@ -181,10 +181,11 @@ def traffic(args):
             "fetch_bytes_per_launch": fb / len(f), "write_bytes_per_launch": wb / len(w),
             "traffic_bytes_per_launch": fb / len(f) + wb / len(w),
             "algorithmic_bytes_per_launch": e["bytes"] / e["calls"],
-            "traffic_over_algorithmic": (fb / len(f) + wb / len(w)) / (e["bytes"] / e["calls"]),
-            "hip_event_avg_ms": e["ms"] / e["calls"],
-            "traffic_GBs": (fb / len(f) + wb / len(w)) / (e["ms"] / e["calls"] * 1e-3) / 1e9,
-            "algorithmic_GBs": e["bytes"] / (e["ms"] * 1e-3) / 1e9}
+            "traffic_over_algorithmic": (fb / len(f) + wb / len(w)) / (e["bytes"] / e["calls"])}
+    # no rates: counter collection slows every dispatch, so the times of
+    # these runs are not the kernels' (those are in the trace summaries)
+    out["note"] = ("bytes per launch only; kernel times and GB/s come from the runs without counters "
+                   "(kernel-trace summaries and the benchmark line)")
     return out
 
 
@@ -284,6 +285,60 @@ def sq(args):
     return "\n".join(lines) + "\n"
 
 
+def _cycle_counters(path, calls):
+    """Counter sums per kernel key over the last `calls[key]` dispatches of
+    each kernel in a counter directory (the HIP-event cycle)."""
+    rows = _rows(path, "counter_collection.csv")
+    ids = {}
+    for r in rows:
+        ids.setdefault(rocprof_key(_col(r, "Kernel_Name")), set()).add(int(_col(r, "Dispatch_Id", "Correlation_Id")))
+    keep = {k: set(_last(sorted(v), calls.get(k, 0))) for k, v in ids.items()}
+    agg = {}
+    for r in rows:
+        k = rocprof_key(_col(r, "Kernel_Name"))
+        if int(_col(r, "Dispatch_Id", "Correlation_Id")) not in keep[k]:
+            continue
+        a = agg.setdefault(k, {})
+        n = _col(r, "Counter_Name").strip()
+        a[n] = a.get(n, 0.0) + float(_col(r, "Counter_Value"))
+    return agg
+
+
+def stalls(args):
+    """Where the waves of each kernel spend their time, from two counter
+    passes: SQ_WAIT_ANY (waiting on memory counters, barriers or anything
+    else), SQ_WAIT_INST_ANY (ready, waiting for issue) and SQ_WAIT_INST_LDS,
+    each over SQ_WAVE_CYCLES (all in quad-cycles); and the LDS bank-conflict
+    stall cycles over the conflict-free LDS cycles (SQ_LDS_BANK_CONFLICT /
+    (SQ_LDS_IDX_ACTIVE - SQ_LDS_BANK_CONFLICT))."""
+    bench = json.loads(pathlib.Path(args.bench_json).read_text().splitlines()[-1])
+    calls, ms = {}, {}
+    for name, v in bench.get("kernels", {}).items():
+        k = name.split(" [")[0]
+        calls[k] = calls.get(k, 0) + v["calls"]
+        ms[k] = ms.get(k, 0.0) + v["ms"]
+    a = _cycle_counters(args.dir_wait, calls)
+    b = _cycle_counters(args.dir_lds, calls)
+    lines = [f"# {args.title}", "", "Commands (repository root, on the MI355X):", "", "```"] + args.command + [
+        "```", "", "Fractions of the waves' cycles (SQ_WAVE_CYCLES) over the launches of the benchmark's HIP-event "
+        "cycle. Kernels in order of their HIP-event time per step.", "",
+        "| kernel | waiting (SQ_WAIT_ANY) | ready, not issued (SQ_WAIT_INST_ANY) | waiting for LDS issue | "
+        "LDS bank-conflict stalls / LDS cycles |", "|---|---|---|---|---|"]
+    for k in sorted(a, key=lambda k: -ms.get(k, 0.0)):
+        if not k.startswith("k_") or k not in ms:
+            continue
+        c = a[k]
+        wc = c.get("SQ_WAVE_CYCLES", 0.0)
+        if wc <= 0:
+            continue
+        d = b.get(k, {})
+        act = d.get("SQ_LDS_IDX_ACTIVE", 0.0) - d.get("SQ_LDS_BANK_CONFLICT", 0.0)
+        bank = f"{d.get('SQ_LDS_BANK_CONFLICT', 0.0) / act:.2f}" if act > 0 else "–"
+        lines.append(f"| `{k}` | {c.get('SQ_WAIT_ANY', 0.0) / wc:.2f} | {c.get('SQ_WAIT_INST_ANY', 0.0) / wc:.2f} | "
+                     f"{c.get('SQ_WAIT_INST_LDS', 0.0) / wc:.3f} | {bank} |")
+    return "\n".join(lines) + "\n"
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -316,9 +371,16 @@ def main(argv=None):
     q.add_argument("--command", action="append", default=[])
     q.add_argument("--clock-ghz", type=float, default=2.4)
     q.add_argument("--out")
+    st = sub.add_parser("stalls")
+    st.add_argument("dir_wait", help="counter pass with SQ_WAVE_CYCLES, SQ_WAIT_ANY, SQ_WAIT_INST_ANY, SQ_WAIT_INST_LDS")
+    st.add_argument("dir_lds", help="counter pass with SQ_LDS_BANK_CONFLICT, SQ_LDS_IDX_ACTIVE")
+    st.add_argument("--bench-json", required=True, help="stdout of a default bench.py run")
+    st.add_argument("--title", required=True)
+    st.add_argument("--command", action="append", default=[])
+    st.add_argument("--out")
     a = ap.parse_args(argv)
-    if a.cmd == "sq":
-        text = sq(a)
+    if a.cmd in ("sq", "stalls"):
+        text = sq(a) if a.cmd == "sq" else stalls(a)
         if a.out:
             pathlib.Path(a.out).write_text(text)
         print(text)

@@ -1,7 +1,8 @@
 """Host-side logic of the measurement tools, on the CPU: the benchmark's
 refusal of more than one GPU, the constant-coefficient variant of the
-surrogate's weights, and the kernel-name keys that pair the library's
-per-kernel HIP-event statistics with rocprofv3's kernel names."""
+surrogate's weights, the kernel-name keys that pair the library's
+per-kernel HIP-event statistics with rocprofv3's kernel names, and the
+counter summaries' selection of the HIP-event cycle's dispatches."""
 import importlib.util
 
 import numpy as np
@@ -55,3 +56,30 @@ def test_counter_summaries_keep_the_last_cycle():
     vals = list(range(10))
     assert ps._last(vals, 3) == [7, 8, 9]
     assert ps._last(vals, 0) == vals
+
+
+def _counter_csv(path, rows):
+    path.mkdir(parents=True)
+    with open(path / "1_counter_collection.csv", "w") as f:
+        f.write("Dispatch_Id,Kernel_Name,Counter_Name,Counter_Value\n")
+        for did, name, counter, value in rows:
+            f.write(f'{did},"{name}",{counter},{value}\n')
+
+
+def test_stall_summary_ratios_over_the_last_cycle(tmp_path):
+    k = "void k_sgs<float>(SgsArgs<float>)"
+    # dispatch 1 belongs to the warm-up (not in the cycle), 2 and 3 to the cycle
+    wait = []
+    for did, wc in ((1, 1000.0), (2, 100.0), (3, 300.0)):
+        wait += [(did, k, "SQ_WAVE_CYCLES", wc), (did, k, "SQ_WAIT_ANY", 0.5 * wc),
+                 (did, k, "SQ_WAIT_INST_ANY", 0.25 * wc), (did, k, "SQ_WAIT_INST_LDS", 0.0)]
+    lds = [(did, k, "SQ_LDS_IDX_ACTIVE", 30.0) for did in (1, 2, 3)] + \
+          [(did, k, "SQ_LDS_BANK_CONFLICT", 10.0) for did in (1, 2, 3)]
+    _counter_csv(tmp_path / "a", wait)
+    _counter_csv(tmp_path / "b", lds)
+    bj = tmp_path / "bench.json"
+    bj.write_text('{"kernels": {"k_sgs<float>": {"calls": 2, "ms": 0.6}}}\n')
+    args = ps.argparse.Namespace(dir_wait=str(tmp_path / "a"), dir_lds=str(tmp_path / "b"), bench_json=str(bj),
+                                 title="t", command=["c"], out=None)
+    row = [ln for ln in ps.stalls(args).splitlines() if ln.startswith("| `k_sgs<float>`")]
+    assert row == ["| `k_sgs<float>` | 0.50 | 0.25 | 0.000 | 0.50 |"]
