@@ -358,6 +358,36 @@ def config_legs(pkg, P, ic):
                                                "cs_count": 2, "judge_every": 10}
         print(f"# config learned 512x512x256: {out['learned_512x512x256_coriolis']['mpts']:.0f} Mpt-steps/s",
               file=sys.stderr, flush=True)
+    out["tke_budget_256cube"] = tke_leg(pkg, P, ic)
+    return out
+
+
+def tke_leg(pkg, P, ic, n=256, calls=10):
+    """One call of the TKE budget sums (compute_tke_budget, every step past
+    start_tke when tke_flag is set) at 256^3 after two steps, FP64 and
+    mixed: milliseconds per call and the algorithmic bytes per point (the
+    cleaned pressure and its FP64 transform chain, then two passes over the
+    fields for the 61 plane sums)."""
+    out = {}
+    for name, prec, tb in (("fp64", P.PREC_FP64, 8), ("mixed", P.PREC_MIXED, 4)):
+        prm = P.LesParams(nx=n, ny=n, nz=n, dt=DT, precision=prec)
+        u, v, w = ic.log_law_ic(n, n, n, prm.lz, prm.z0, 0.10, seed=1234)
+        with pkg.LesGpu(prm) as g:
+            g.set_ic(u, v, w)
+            g.tke_init()
+            g.step(2)
+            g.tke_accumulate()
+            g.sync()
+            t0 = time.perf_counter()
+            for _ in range(calls):
+                g.tke_accumulate()
+            g.sync()
+            ms = (time.perf_counter() - t0) / calls * 1e3
+        spec = 16.0 * (n // 2 + 1) / n  # one FP64 half spectrum per point
+        bpp = 5 * 8 + (8 + spec) + 4 * spec + (3 * spec + 3 * 8) + 2 * (7 * 8 + 16 * tb)
+        out[name] = {"ms_per_call": ms, "bytes_per_point": bpp,
+                     "GBs": bpp * n * n * n / (ms * 1e-3) / 1e9, "calls": calls}
+        print(f"# tke budget {name} 256^3: {ms:.2f} ms per call", file=sys.stderr, flush=True)
     return out
 
 

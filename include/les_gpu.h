@@ -208,6 +208,29 @@ int les_stat_profiles(double* prof);
  * else status 10.  Calls on one GPU share that array: calls that overlap
  * (two host threads, or two processes' streams) race on it. */
 int les_stat_profiles_noalloc(double* prof);
+/* TKE budget (src/tke_budget1D.f90:64-440): LES_NTKE per-level sums of
+ * plane averages on the uvp nodes of levels 1..nz, in the order of the
+ * reference's ta1_profiles/<name>.out files:
+ *   u v w dudx dudy dudz dvdx dvdy dvdz dwdx dwdy dwdz uu vv ww uv uw vw
+ *   txx tyy tzz txy txz tyz cs uuw vvw www uww wtzz utxz vtyz dxx dyy dzz
+ *   dxy dxz dyz dudx2 dudy2 dudz2 dvdx2 dvdy2 dvdz2 dwdx2 dwdy2 dwdz2
+ *   dudxdwdx dudydwdy dudzdwdz p dpdx dpdy dpdz pw pu pdudx pdvdy pdwdz
+ *   pdwdx pdudz
+ * (w-node fields interpolated to the uvp node; the p family is of the
+ * cleaned pressure p - |u|^2 / 2, through grad_velocity_filt).
+ * les_tke_init allocates the sums and the cleaned-pressure workspaces
+ * (tke_budget_init, :64-195) and zeroes the sums. */
+#define LES_NTKE 61
+int les_tke_init(void);
+/* One call of compute_tke_budget (:209-390): the plane averages of the
+ * current state added to the sums.  Error 1 before les_tke_init. */
+int les_tke_accumulate(void);
+/* Host copy of the sums: LES_NTKE rows of nz levels (row-major by
+ * quantity).  The reference writes sums / tke_period every tke_period
+ * steps (avg_var, :415-436). */
+int les_tke_download(double* sums);
+/* Zeroes the sums, as after a record (:437). */
+int les_tke_reset(void);
 /* compute_running_diagnistics (src/running_diagnostics.f90:80-305) of the
  * current state: out[0] CFL, out[1] MKE, out[2] plane-mean u*, out[3] rms of
  * du/dx + dv/dy + dw/dz over levels 1..nz. */

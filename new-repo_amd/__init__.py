@@ -33,6 +33,15 @@ STATS = ["u", "v", "w", "p", "u2", "v2", "w2", "u3", "v3", "w3", "u4", "v4", "w4
 PROFILES = ["u", "v", "w", "p", "var_u", "var_v", "var_w", "txx", "tyy", "tzz", "R13", "txz", "St13",
             "R23", "tyz", "St23", "tke", "cs", "nut", "C13", "C23"]
 
+# TKE budget sums (les_tke_download rows), the names of the reference's
+# ta1_profiles/<name>.out files
+TKE = ["u", "v", "w", "dudx", "dudy", "dudz", "dvdx", "dvdy", "dvdz", "dwdx", "dwdy", "dwdz",
+       "uu", "vv", "ww", "uv", "uw", "vw", "txx", "tyy", "tzz", "txy", "txz", "tyz", "cs",
+       "uuw", "vvw", "www", "uww", "wtzz", "utxz", "vtyz", "dxx", "dyy", "dzz", "dxy", "dxz", "dyz",
+       "dudx2", "dudy2", "dudz2", "dvdx2", "dvdy2", "dvdz2", "dwdx2", "dwdy2", "dwdz2",
+       "dudxdwdx", "dudydwdy", "dudzdwdz", "p", "dpdx", "dpdy", "dpdz", "pw", "pu",
+       "pdudx", "pdvdy", "pdwdz", "pdwdx", "pdudz"]
+
 STATUS = {0: "ok", 1: "invalid option or model", 3: "out of memory", 4: "double initialisation",
           5: "HIP runtime error", 9: "NaN or singular solve", 10: "bad shape or id"}
 
@@ -97,6 +106,7 @@ def library() -> ctypes.CDLL:
             "les_mlp_forward": [vp, ctypes.c_long, vp],
             "les_stat_init": [], "les_stat_reset": [], "les_stat_accumulate": [],
             "les_stat_download": [dp, dp, ip], "les_stat_profiles": [dp], "les_stat_profiles_noalloc": [dp], "les_running_diagnostics": [dp],
+            "les_tke_init": [], "les_tke_reset": [], "les_tke_accumulate": [], "les_tke_download": [dp],
         }
         for name, args in sig.items():
             f = getattr(lib, name)
@@ -331,6 +341,21 @@ class LesGpu:
         out = np.empty((len(PROFILES), self.prm.nz))
         check(self.lib.les_stat_profiles_noalloc(_dp(out)))
         return dict(zip(PROFILES, out))
+
+    # ---- TKE budget
+    def tke_init(self) -> None:
+        check(self.lib.les_tke_init())
+
+    def tke_reset(self) -> None:
+        check(self.lib.les_tke_reset())
+
+    def tke_accumulate(self) -> None:
+        check(self.lib.les_tke_accumulate())
+
+    def tke_download(self) -> Dict[str, np.ndarray]:
+        out = np.empty((len(TKE), self.prm.nz))
+        check(self.lib.les_tke_download(_dp(out)))
+        return dict(zip(TKE, out))
 
     def running_diagnostics(self) -> Dict[str, float]:
         out = np.zeros(4)

@@ -616,6 +616,7 @@ static void sgs_stress(Les& L, int jt, int jtt) {
 #include "les_mlp.h"
 #include "les_learn.h"
 #include "les_stats.h"
+#include "les_tke.h"
 
 // ---------------------------------------------------------------- allocation-free profiles
 // Device array behind les_stat_profiles_noalloc: the per-slice partial sums
@@ -1711,6 +1712,49 @@ static void diagnostics(Les& L) {
   hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(LES_BLOCK), 0, L.stream, b);
 }
 
+// compute_tke_budget: the cleaned pressure through the FP64
+// grad_velocity_filt chain (the filtered field back into tk_pf, in place in
+// the column pass like grad_filt3), then the plane sums of the 61
+// quantities in two launches of 31 and 30, then the window sums.
+template <typename T>
+static void tke_accumulate(Les& L) {
+  TkePcleanArgs c;
+  c.u = D(L, LES_U); c.v = D(L, LES_V); c.w = D(L, LES_W); c.p = D(L, LES_P);
+  c.pf = L.tk_pf; c.np = L.np; c.nz = L.nz;
+  kt_begin(L, KT_POINTWISE);
+  hipLaunchKernelGGL(k_tke_pclean, dim3(nblk(L.np * (L.nz + 2))), dim3(LES_BLOCK), 0, L.stream, c);
+  kt_end(L, KT_POINTWISE, (double)L.np * L.nz * 5.0 * 8 + 2.0 * L.np * 8, "k_tke_pclean", 8);
+  Cplx<double>* s0 = (Cplx<double>*)L.tk_spec[0];
+  Cplx<double>* s1 = (Cplx<double>*)L.tk_spec[1];
+  Cplx<double>* s2 = (Cplx<double>*)L.tk_spec[2];
+  const double* src[1] = {L.tk_pf};
+  Cplx<double>* dst[1] = {s0};
+  rows_r2c<double, double>(L, src, dst, 1, 0, L.nzg, 1.0 / (double)(L.nx * L.ny));
+  cols<double>(L, COP_GRAD, s0, nullptr, s1, s2, s0, 3, 1, 1, 0, L.nzg);
+  const Cplx<double>* cs[3] = {s1, s2, s0};
+  double* outs[3] = {L.tk_px, L.tk_py, L.tk_pf};
+  rows_c2r<double, double>(L, cs, outs, 3, 0, L.nzg);
+  TkeArgs<T> a;
+  a.u = D(L, LES_U); a.v = D(L, LES_V); a.w = D(L, LES_W);
+  a.dudx = F<T>(L, LES_DUDX); a.dudy = F<T>(L, LES_DUDY); a.dudz = F<T>(L, LES_DUDZ);
+  a.dvdx = F<T>(L, LES_DVDX); a.dvdy = F<T>(L, LES_DVDY); a.dvdz = F<T>(L, LES_DVDZ);
+  a.dwdx = F<T>(L, LES_DWDX); a.dwdy = F<T>(L, LES_DWDY); a.dwdz = F<T>(L, LES_DWDZ);
+  a.txx = F<T>(L, LES_TXX); a.txy = F<T>(L, LES_TXY); a.txz = F<T>(L, LES_TXZ);
+  a.tyy = F<T>(L, LES_TYY); a.tyz = F<T>(L, LES_TYZ); a.tzz = F<T>(L, LES_TZZ);
+  a.cs = F<T>(L, LES_CS_OPT2);
+  a.pf = L.tk_pf; a.px = L.tk_px; a.py = L.tk_py;
+  a.part = L.tk_part; a.np = L.np; a.nz = L.nz; a.cz = 1.0 / L.dz;
+  const double rd = (double)L.np * L.nz * (4.0 * 8 + 19.0 * sizeof(T) + 3.0 * 8);
+  kt_begin(L, KT_POINTWISE);
+  hipLaunchKernelGGL((k_tke_planes<T, 0, 31>), dim3(LES_TKE_SLICES, L.nz), dim3(LES_BLOCK), 0, L.stream, a);
+  kt_end(L, KT_POINTWISE, rd, "k_tke_planes", sizeof(T), 0, "0-30");
+  kt_begin(L, KT_POINTWISE);
+  hipLaunchKernelGGL((k_tke_planes<T, 31, TK_N>), dim3(LES_TKE_SLICES, L.nz), dim3(LES_BLOCK), 0, L.stream, a);
+  kt_end(L, KT_POINTWISE, rd, "k_tke_planes", sizeof(T), 0, "31-60");
+  hipLaunchKernelGGL(k_tke_add, dim3(nblk((long)TK_N * L.nz)), dim3(LES_BLOCK), 0, L.stream, (const double*)L.tk_part,
+                     L.tk_sum, L.nz, (double)(L.nx * L.ny));
+}
+
 static void stat_zero(Les& L) {
   const long n = (long)LES_NSTAT * L.np * L.nz;
   hipLaunchKernelGGL(k_fill_zero, dim3(nblk(n)), dim3(LES_BLOCK), 0, L.stream, L.st_sum, n);
@@ -1735,6 +1779,54 @@ int les_stat_init(void) {
   }
   stat_zero(*L);
   LES_CHECK(hipGetLastError());
+  return 0;
+}
+
+int les_tke_init(void) {
+  Les* L = g_les;
+  if (!L) return 1;
+  if (!L->tk_sum) {
+    L->tk_sum = (double*)dalloc(*L, sizeof(double) * LES_NTKE * (size_t)L->nz);
+    L->tk_part = (double*)dalloc(*L, sizeof(double) * (size_t)L->nz * LES_TKE_SLICES * LES_NTKE);
+    L->tk_pf = (double*)dalloc(*L, sizeof(double) * L->nf);
+    L->tk_px = (double*)dalloc(*L, sizeof(double) * L->nf);
+    L->tk_py = (double*)dalloc(*L, sizeof(double) * L->nf);
+    for (int i = 0; i < 3; ++i) L->tk_spec[i] = (double*)dalloc(*L, 2 * sizeof(double) * L->nsp);
+    if (!L->tk_sum || !L->tk_part || !L->tk_pf || !L->tk_px || !L->tk_py || !L->tk_spec[0] || !L->tk_spec[1] ||
+        !L->tk_spec[2]) {
+      les_set_error("Out of memory when initialising the tke budget");
+      return 3;
+    }
+  }
+  return les_tke_reset();
+}
+
+int les_tke_reset(void) {
+  Les* L = g_les;
+  if (!L) return 1;
+  if (!L->tk_sum) { les_set_error("tke budget not initialised (les_tke_init)"); return 1; }
+  hipLaunchKernelGGL(k_fill_zero, dim3(nblk((long)LES_NTKE * L->nz)), dim3(LES_BLOCK), 0, L->stream, L->tk_sum,
+                     (long)LES_NTKE * L->nz);
+  LES_CHECK(hipGetLastError());
+  return 0;
+}
+
+int les_tke_accumulate(void) {
+  Les* L = g_les;
+  if (!L) return 1;
+  if (!L->tk_sum) { les_set_error("tke budget not initialised (les_tke_init)"); return 1; }
+  if (L->mixed) tke_accumulate<float>(*L); else tke_accumulate<double>(*L);
+  LES_CHECK(hipGetLastError());
+  return 0;
+}
+
+int les_tke_download(double* sums) {
+  Les* L = g_les;
+  if (!L) return 1;
+  if (!L->tk_sum) { les_set_error("tke budget not initialised (les_tke_init)"); return 1; }
+  if (!sums) { les_set_error("les_tke_download: null output"); return 10; }
+  LES_CHECK(hipStreamSynchronize(L->stream));
+  LES_CHECK(hipMemcpy(sums, L->tk_sum, sizeof(double) * LES_NTKE * (size_t)L->nz, hipMemcpyDeviceToHost));
   return 0;
 }
 

@@ -84,6 +84,13 @@ struct Les {
   double* st_part;        // profile partial sums
   double* st_prof;        // LES_NPROF x nz profiles
   int st_iters;           // accumulated steps
+  // TKE budget (allocated by les_tke_init)
+  double* tk_sum;         // LES_NTKE x nz window sums
+  double* tk_part;        // plane partial sums
+  double* tk_pf;          // cleaned pressure, then filtered (nx, ny, nzg)
+  double* tk_px;          // its x derivative
+  double* tk_py;          // its y derivative
+  double* tk_spec[3];     // FP64 spectra of the chain
   double* dg_part;        // diagnostics partials
   double* dg_out;         // cfl, mke, u*, rms div
 

@@ -23,6 +23,7 @@ module LES_engine
   use compute_wall_law
   use running_diagnositcs
   use mod_IO_statistics
+  use tke_budget1D
   use mod_IO_main
   implicit none
   private
@@ -53,6 +54,7 @@ contains
        call read_parameters_restart(jt_total)
        tt = jt_total*dt
     end if
+    if (tke_flag) call tke_budget_init()
     call les_check(les_init(cparams), 'les_init')
     call running_diagnostics_init()
     if (output) call io_init()
@@ -107,6 +109,7 @@ contains
        call compute_running_diagnistics(jt, jt_total, tt, .true.)
        if (mod(jt, NaN_check) == 0) call check_NaN(jt)
        if (stat_flag .and. jt_total > start_stats) call io_stat_averages(jt, jt_total)
+       if (tke_flag .and. jt_total > start_tke) call compute_tke_budget(jt)
        if (output .and. mod(jt, step_out) == 0) call output_loop(jt, jt_total)
        if (mod(jt_total, checkpoint) == 0) call checkpoint_out(jt_total)
     end do
@@ -118,6 +121,7 @@ contains
     if (mod(jt_total, checkpoint) /= 0) call checkpoint_out(jt_total)
     call write_parameters_restart(jt_total)
     if (stat_flag) call io_stat_finalize()
+    if (tke_flag) call tke_budget_finalize()
     if (output) call io_finalize()
 
     write(*,'(A,F10.4,A)') ' wall time for time loop: ', time_loop, 's'

@@ -141,13 +141,26 @@ module les_gpu_abi
        import :: c_int, c_double
        real(c_double), intent(out) :: prof(*)
      end function les_stat_profiles
+     integer(c_int) function les_tke_init() bind(C, name="les_tke_init")
+       import :: c_int
+     end function les_tke_init
+     integer(c_int) function les_tke_reset() bind(C, name="les_tke_reset")
+       import :: c_int
+     end function les_tke_reset
+     integer(c_int) function les_tke_accumulate() bind(C, name="les_tke_accumulate")
+       import :: c_int
+     end function les_tke_accumulate
+     integer(c_int) function les_tke_download(sums) bind(C, name="les_tke_download")
+       import :: c_int, c_double
+       real(c_double), intent(out) :: sums(*)
+     end function les_tke_download
      integer(c_int) function les_running_diagnostics(out) bind(C, name="les_running_diagnostics")
        import :: c_int, c_double
        real(c_double), intent(out) :: out(4)
      end function les_running_diagnostics
   end interface
 
-  integer(c_int), parameter :: LES_NSTAT = 26, LES_NPROF = 21
+  integer(c_int), parameter :: LES_NSTAT = 26, LES_NPROF = 21, LES_NTKE = 61
 
 contains
 

@@ -25,6 +25,7 @@
 //                                               matlab_src/get_profile_struct.m:9-59,
 //                                               matlab_src/compute_Cij.m:1-36
 //   running diagnostics ....................... src/running_diagnostics.f90:80-305
+//   TKE budget sums ........................... src/tke_budget1D.f90:209-440
 //
 // Deliberate restatement choices (documented in the repository's design notes):
 //   * FFTs use the algorithm in fftcore.h, not FFTW; they are unnormalised like
@@ -102,6 +103,9 @@ struct Oracle {
   // running statistics: 26 sums of (nx, ny, nz) and the u* sum
   std::vector<double> st_sum, st_ustar;
   int st_iters = 0;
+  // TKE budget: 61 per-level sums of plane averages (nz each)
+  std::vector<double> tk_sum;
+  std::vector<double> tk_abs;  // the same sums of |value|: the scale of each sum's rounding
 
   size_t I(int i, int j, int k) const { return (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * (size_t)k); }
   size_t S(int kx, int j, int k) const { return (size_t)kx + (size_t)lhx * ((size_t)j + (size_t)ny * (size_t)k); }
@@ -1249,6 +1253,104 @@ static void stat_profiles(const Oracle& o, double* prof) {
   }
 }
 
+// ---------------------------------------------------------------- TKE budget
+// compute_tke_budget (src/tke_budget1D.f90:209-390), 1 rank: 61 quantities
+// on the uvp nodes of levels 1..nz, each reduced to its plane average
+// (compute_planar_average, src/utility_tools.f90:59-91: one running sum per
+// level over j, then i, divided by nx*ny) and added to its per-level sum
+// (avg_var, src/tke_budget1D.f90:397-440).  I(f) = (f(k) + f(k+1)) / 2 is
+// int_var (:448-468); with the wall law the level-1 txz, tyz and the
+// level-1 dudz + dwdx, dvdz + dwdy sums use the reference's special forms
+// (:284-313).  Integer powers as products: x**2 = x*x, x**3 = (x*x)*x.
+// The cleaned pressure p - (u*u + v*v + W*W) / 2 on levels 1..nz goes
+// through grad_velocity_filt (filtered field, d/dx, d/dy) and ddz_uv; its
+// ghost planes 0 and nz+1 are 0 here (the reference leaves them
+// uninitialised: update_ghost does nothing on one rank in z,
+// src/decomp_2d_custom.f90:364-366), which reaches dpdz at levels 1 and nz.
+enum {
+  TK_U = 0, TK_V, TK_W, TK_DUDX, TK_DUDY, TK_DUDZ, TK_DVDX, TK_DVDY, TK_DVDZ, TK_DWDX, TK_DWDY, TK_DWDZ,
+  TK_UU, TK_VV, TK_WW, TK_UV, TK_UW, TK_VW, TK_TXX, TK_TYY, TK_TZZ, TK_TXY, TK_TXZ, TK_TYZ, TK_CS,
+  TK_UUW, TK_VVW, TK_WWW, TK_UWW, TK_WTZZ, TK_UTXZ, TK_VTYZ, TK_DXX, TK_DYY, TK_DZZ, TK_DXY, TK_DXZ,
+  TK_DYZ, TK_DUDX2, TK_DUDY2, TK_DUDZ2, TK_DVDX2, TK_DVDY2, TK_DVDZ2, TK_DWDX2, TK_DWDY2, TK_DWDZ2,
+  TK_DUDXDWDX, TK_DUDYDWDY, TK_DUDZDWDZ, TK_P, TK_DPDX, TK_DPDY, TK_DPDZ, TK_PW, TK_PU, TK_PDUDX,
+  TK_PDVDY, TK_PDWDZ, TK_PDWDX, TK_PDUDZ, TK_N
+};
+static_assert(TK_N == 61, "TKE budget quantities");
+
+static void tke_reset(Oracle& o) {
+  o.tk_sum.assign((size_t)TK_N * o.nz, 0.0);
+  o.tk_abs.assign((size_t)TK_N * o.nz, 0.0);
+}
+
+// the 61 values at point q of level k; pf, px, py: the filtered cleaned
+// pressure and its x, y derivatives (all planes 0..nz+1)
+static void tke_point(const Oracle& o, int k, size_t q, const std::vector<double>& pf, const std::vector<double>& px,
+                      const std::vector<double>& py, double* x) {
+  const size_t a = o.I(0, 0, k) + q, b = a + o.np;
+  auto I = [&](const std::vector<double>& f) { return 0.5 * (f[a] + f[b]); };
+  const double u = o.u[a], v = o.v[a], W = PLANT(43) ? o.w[a] : I(o.w);
+  const double Idudz = I(o.dudz), Idvdz = I(o.dvdz), Idwdx = I(o.dwdx), Idwdy = I(o.dwdy);
+  // txz, tyz on the uvp node; the wall law keeps level 1's own values
+  const double Txz = k == 1 && !PLANT(45) ? o.txz[a] : I(o.txz);
+  const double Tyz = k == 1 ? o.tyz[a] : I(o.tyz);
+  const double S13 = k == 1 ? o.dudz[a] + 0.5 * (o.dwdx[a] + o.dwdx[b])
+                            : 0.5 * ((o.dudz[a] + o.dwdx[a]) + (o.dudz[b] + o.dwdx[b]));
+  const double S23 = k == 1 ? o.dvdz[a] + 0.5 * (o.dwdy[a] + o.dwdy[b])
+                            : 0.5 * ((o.dvdz[a] + o.dwdy[a]) + (o.dvdz[b] + o.dwdy[b]));
+  const double c = 1.0 / o.dz;
+  const double Pf = pf[a];
+  const double dpdz = PLANT(42) ? c * (pf[a] - pf[a - o.np]) : 0.5 * (c * (pf[a] - pf[a - o.np]) + c * (pf[b] - pf[a]));
+  x[TK_U] = u; x[TK_V] = v; x[TK_W] = W;
+  x[TK_DUDX] = o.dudx[a]; x[TK_DUDY] = o.dudy[a]; x[TK_DUDZ] = Idudz;
+  x[TK_DVDX] = o.dvdx[a]; x[TK_DVDY] = o.dvdy[a]; x[TK_DVDZ] = Idvdz;
+  x[TK_DWDX] = Idwdx; x[TK_DWDY] = Idwdy; x[TK_DWDZ] = o.dwdz[a];
+  x[TK_UU] = u * u; x[TK_VV] = v * v; x[TK_WW] = W * W;
+  x[TK_UV] = u * v; x[TK_UW] = W * u; x[TK_VW] = W * v;
+  x[TK_TXX] = o.txx[a]; x[TK_TYY] = o.tyy[a]; x[TK_TZZ] = o.tzz[a];
+  x[TK_TXY] = o.txy[a]; x[TK_TXZ] = I(o.txz); x[TK_TYZ] = I(o.tyz);
+  x[TK_CS] = o.cs_opt2[a];
+  x[TK_UUW] = (u * u) * W; x[TK_VVW] = (v * v) * W; x[TK_WWW] = (W * W) * W; x[TK_UWW] = PLANT(44) ? (u * u) * W : u * (W * W);
+  x[TK_WTZZ] = W * o.tzz[a]; x[TK_UTXZ] = Txz * u; x[TK_VTYZ] = Tyz * v;
+  x[TK_DXX] = o.dudx[a] * o.txx[a]; x[TK_DYY] = o.dvdy[a] * o.tyy[a]; x[TK_DZZ] = o.dwdz[a] * o.tzz[a];
+  x[TK_DXY] = (PLANT(46) ? 1.0 : 0.5) * (o.dudy[a] + o.dvdx[a]) * o.txy[a];
+  x[TK_DXZ] = (0.5 * S13) * Txz; x[TK_DYZ] = (0.5 * Tyz) * S23;
+  x[TK_DUDX2] = o.dudx[a] * o.dudx[a]; x[TK_DUDY2] = o.dudy[a] * o.dudy[a]; x[TK_DUDZ2] = Idudz * Idudz;
+  x[TK_DVDX2] = o.dvdx[a] * o.dvdx[a]; x[TK_DVDY2] = o.dvdy[a] * o.dvdy[a]; x[TK_DVDZ2] = Idvdz * Idvdz;
+  x[TK_DWDX2] = Idwdx * Idwdx; x[TK_DWDY2] = Idwdy * Idwdy; x[TK_DWDZ2] = o.dwdz[a] * o.dwdz[a];
+  x[TK_DUDXDWDX] = o.dudx[a] * Idwdx; x[TK_DUDYDWDY] = o.dudy[a] * Idwdy; x[TK_DUDZDWDZ] = Idudz * o.dwdz[a];
+  x[TK_P] = Pf; x[TK_DPDX] = px[a]; x[TK_DPDY] = py[a]; x[TK_DPDZ] = dpdz;
+  x[TK_PW] = Pf * W; x[TK_PU] = Pf * u;
+  x[TK_PDUDX] = Pf * o.dudx[a]; x[TK_PDVDY] = Pf * o.dvdy[a]; x[TK_PDWDZ] = Pf * o.dwdz[a];
+  x[TK_PDWDX] = Pf * Idwdx; x[TK_PDUDZ] = Pf * Idudz;
+}
+
+static void tke_accumulate(Oracle& o) {
+  if (o.tk_sum.empty()) tke_reset(o);
+  std::vector<double> pf(o.nf, 0.0), px(o.nf), py(o.nf);
+  for (int k = 1; k <= o.nz; ++k)
+    for (size_t q = 0; q < o.np; ++q) {
+      const size_t a = o.I(0, 0, k) + q;
+      const double h = 0.5 * (o.w[a] + o.w[a + o.np]);
+      pf[a] = o.p[a] - 0.5 * ((o.u[a] * o.u[a] + o.v[a] * o.v[a]) + (PLANT(41) ? 0.0 : h * h));
+    }
+  grad_filt(o, pf, px, py);
+  const double nxy = (double)(o.nx * o.ny);
+#pragma omp parallel for schedule(static)
+  for (int k = 1; k <= o.nz; ++k) {
+    double sum[TK_N] = {0.0}, asum[TK_N] = {0.0}, x[TK_N];
+    for (size_t q = 0; q < o.np; ++q) {
+      tke_point(o, k, q, pf, px, py, x);
+      for (int f = 0; f < TK_N; ++f) {
+        sum[f] = sum[f] + x[f];
+        asum[f] = asum[f] + std::fabs(x[f]);
+      }
+    }
+    for (int f = 0; f < TK_N; ++f) o.tk_abs[(size_t)f * o.nz + (k - 1)] += asum[f] / nxy;
+    for (int f = 0; f < TK_N; ++f)
+      o.tk_sum[(size_t)f * o.nz + (k - 1)] = (PLANT(47) ? 0.0 : o.tk_sum[(size_t)f * o.nz + (k - 1)]) + sum[f] / nxy;
+  }
+}
+
 // compute_running_diagnistics (src/running_diagnostics.f90:80-305), 1 rank:
 // CFL = dt (max|u|/dx + max|v|/dy + max|w|/dz) over levels 1..nz; MKE =
 // (U^2 + V^2 + W^2) / 2 of the domain means (w averaged to the uvp node);
@@ -1551,6 +1653,16 @@ int orc_stat_profiles(double* prof) {
   return 0;
 }
 int orc_diagnostics(double* out) { if (!g) return 1; diagnostics(*g, out); return 0; }
+// TKE budget sums (TK_N x nz, row-major by quantity)
+int orc_tke_reset(void) { if (!g) return 1; tke_reset(*g); return 0; }
+int orc_tke_accumulate(void) { if (!g) return 1; tke_accumulate(*g); return 0; }
+int orc_tke_get(double* sums, double* abs_sums) {
+  if (!g) return 1;
+  if (g->tk_sum.empty()) tke_reset(*g);
+  std::memcpy(sums, g->tk_sum.data(), sizeof(double) * g->tk_sum.size());
+  if (abs_sums) std::memcpy(abs_sums, g->tk_abs.data(), sizeof(double) * g->tk_abs.size());
+  return 0;
+}
 int orc_set_2d(int which, const double* in) {
   if (!g) return 1;
   std::vector<double>* f = which == 0 ? &g->lbc_u : which == 1 ? &g->lbc_v : which == 2 ? &g->ustar

@@ -50,6 +50,7 @@ def lib(path=None):
             "orc_stat_reset": [], "orc_stat_accumulate": [], "orc_stat_get": [d, d, i],
             "orc_stat_profiles": [d], "orc_diagnostics": [d],
             "orc_update_F": [ctypes.c_int, d, d], "orc_dynamic_mum": [d, d, d, d],
+            "orc_tke_reset": [], "orc_tke_accumulate": [], "orc_tke_get": [d, d],
         }.items():
             f = getattr(_lib, name)
             f.argtypes = args
@@ -225,6 +226,29 @@ class Oracle:
         if self.L.orc_stat_profiles(_dp(out)):
             raise ValueError("no accumulated statistics")
         return dict(zip(self.PROFILES, out))
+
+    # ---- TKE budget (src/tke_budget1D.f90): the file names of its records
+    TKE = ["u", "v", "w", "dudx", "dudy", "dudz", "dvdx", "dvdy", "dvdz", "dwdx", "dwdy", "dwdz",
+           "uu", "vv", "ww", "uv", "uw", "vw", "txx", "tyy", "tzz", "txy", "txz", "tyz", "cs",
+           "uuw", "vvw", "www", "uww", "wtzz", "utxz", "vtyz", "dxx", "dyy", "dzz", "dxy", "dxz", "dyz",
+           "dudx2", "dudy2", "dudz2", "dvdx2", "dvdy2", "dvdz2", "dwdx2", "dwdy2", "dwdz2",
+           "dudxdwdx", "dudydwdy", "dudzdwdz", "p", "dpdx", "dpdy", "dpdz", "pw", "pu",
+           "pdudx", "pdvdy", "pdwdz", "pdwdx", "pdudz"]
+
+    def tke_reset(self):
+        self.L.orc_tke_reset()
+
+    def tke_accumulate(self):
+        self.L.orc_tke_accumulate()
+
+    def tke_get(self, with_abs=False):
+        """The sums by name; with_abs: also the same sums of |value| (the
+        scale of each sum's rounding)."""
+        out = np.zeros((len(self.TKE), self.prm.nz))
+        ab = np.zeros_like(out)
+        self.L.orc_tke_get(_dp(out), _dp(ab))
+        s = dict(zip(self.TKE, out))
+        return (s, dict(zip(self.TKE, ab))) if with_abs else s
 
     def diagnostics(self):
         out = np.zeros(4)

@@ -678,3 +678,126 @@ def pin_coriolis_forcing(lib):
 PINS["coriolis_forcing"] = pin_coriolis_forcing
 PLANTED[39] = ("Coriolis term with the opposite sign", ["coriolis_forcing"])
 PLANTED[40] = ("geostrophic wind components swapped", ["coriolis_forcing"])
+
+
+# ---------------------------------------------------------------- TKE budget
+# src/tke_budget1D.f90:209-440: plane averages (over x, y) of products of the
+# fields on the uvp nodes of levels 1..nz, added to per-level sums.  On
+# single Fourier modes the averages are closed forms: <cos> = 0,
+# <cos^2> = 1/2, <cos^3> = 0, <cos sin> = 0.
+
+_TKE_FIELDS = ("u", "v", "w", "p", "dudx", "dudy", "dudz", "dvdx", "dvdy", "dvdz", "dwdx", "dwdy", "dwdz",
+               "txx", "txy", "txz", "tyy", "tyz", "tzz", "cs_opt2")
+
+
+def _tke_case(lib, fields, nx=16, ny=16, nz=6, calls=1):
+    prm, o = _mk(lib, nx, ny, nz)
+    sh = _shape(prm)
+    for n in _TKE_FIELDS:
+        o.put(n, np.broadcast_to(fields.get(n, 0.0), sh))
+    o.tke_reset()
+    for _ in range(calls):
+        o.tke_accumulate()
+    return prm, o.tke_get()
+
+
+def _small(got, scale, tol, what):
+    """|got| <= tol * scale: a vanishing average, measured on the scale of
+    the terms it averages."""
+    got = np.asarray(got, dtype=np.float64)
+    err = np.abs(got).max() / scale
+    assert np.isfinite(got).all() and err <= tol, f"{what}: {err:.3e} of {scale:.3e} > {tol}"
+
+
+def _levels(prm):
+    return np.arange(prm.nz + 2, dtype=np.float64)[:, None, None]
+
+
+def pin_tke_cleaned_pressure(lib):
+    """p - (u^2 + v^2 + W^2) / 2, filtered, and its derivatives: with u, v,
+    w uniform and p = P0 + beta k + A cos 3x the mean is P0 - (U^2 + V^2 +
+    c^2) / 2 + beta k, <p dudx> = A / 2 for dudx = cos 3x, and dpdz = beta /
+    dz between the ghost planes (which are 0: level 1 reads Pf(0) = 0, level
+    nz reads Pf(nz+1) = 0)."""
+    U, V, c, P0, beta, A = 1.5, -0.5, 0.25, 3.0, 0.125, 0.75
+    prm, o0 = _mk(lib, 16, 16, 6)
+    x, _ = _xy(prm)
+    k = _levels(prm)
+    _, t = _tke_case(lib, {"u": U, "v": V, "w": c, "p": P0 + beta * k + A * np.cos(3 * x),
+                           "dudx": np.cos(3 * x) + 0 * k})
+    nz, dz = prm.nz, prm.lz / prm.nz
+    kk = np.arange(1, nz + 1, dtype=np.float64)
+    pm = P0 - 0.5 * (U * U + V * V + c * c) + beta * kk
+    _close(t["p"], pm, 1e-13, "<p'>")
+    _close(t["pu"], U * pm, 1e-13, "<p' u>")
+    _close(t["pw"], c * pm, 1e-13, "<p' W>")
+    _close(t["pdudx"], np.full(nz, A / 2), 1e-12, "<p' du/dx>")
+    _small(t["dpdx"], 3 * A, 1e-13, "<dp'/dx>")
+    dpdz = np.full(nz, beta / dz)
+    dpdz[0] = 0.5 * (pm[0] + beta) / dz
+    dpdz[-1] = 0.5 * (beta - pm[-1]) / dz
+    _close(t["dpdz"], dpdz, 1e-12, "<dp'/dz>")
+
+
+def pin_tke_transport_and_interpolation(lib):
+    """u = U + a cos x, w(k) = (w0 + w1 k) cos x on the w nodes, so on the uvp
+    node W = b_k cos x with b_k = w0 + w1 (k + 1/2): <W^2> = b^2 / 2,
+    <u W> = a b / 2, <u^2 W> = U a b, <u W^2> = U b^2 / 2, <W^3> = 0."""
+    U, a, w0, w1 = 2.0, 0.5, 0.3, 0.2
+    prm, _ = _mk(lib, 16, 16, 6)
+    x, _ = _xy(prm)
+    k = _levels(prm)
+    _, t = _tke_case(lib, {"u": U + a * np.cos(x) + 0 * k, "w": (w0 + w1 * k) * np.cos(x)})
+    b = w0 + w1 * (np.arange(1, prm.nz + 1) + 0.5)
+    _close(t["ww"], b * b / 2, 1e-13, "<W W>")
+    _close(t["uw"], a * b / 2, 1e-13, "<u W>")
+    _close(t["uuw"], U * a * b, 1e-13, "<u u W>")
+    _close(t["uww"], U * b * b / 2, 1e-13, "<u W W>")
+    _small(t["www"], b.max() ** 3, 1e-14, "<W W W>")
+    _close(t["u"], np.full(prm.nz, U), 1e-14, "<u>")
+
+
+def pin_tke_wall_level_stresses(lib):
+    """txz(k) = t0 + t1 k, dudz(k) = g0 + g1 k, dwdx(k) = h0 + h1 k uniform
+    in the plane, u = U: the SGS transport u txz and dissipation on the uvp
+    node interpolate between k and k+1, except at level 1 (the wall law),
+    where txz(1) and dudz(1) + (dwdx(1) + dwdx(2)) / 2 are used."""
+    U, t0, t1, g0, g1, h0, h1 = 1.25, 0.5, 0.25, 2.0, -0.5, 0.125, 0.375
+    prm, _ = _mk(lib, 16, 16, 6)
+    k = _levels(prm)
+    _, t = _tke_case(lib, {"u": U + 0 * k, "txz": t0 + t1 * k, "dudz": g0 + g1 * k, "dwdx": h0 + h1 * k})
+    kk = np.arange(1, prm.nz + 1, dtype=np.float64)
+    Txz = t0 + t1 * (kk + 0.5)
+    Txz[0] = t0 + t1
+    S13 = (g0 + g1 * (kk + 0.5)) + (h0 + h1 * (kk + 0.5))
+    S13[0] = (g0 + g1) + (h0 + h1 * 1.5)
+    _close(t["utxz"], U * Txz, 1e-13, "<u txz>")
+    _close(t["dxz"], 0.5 * S13 * Txz, 1e-13, "<S13 txz>")
+    _close(t["txz"], t0 + t1 * (kk + 0.5), 1e-13, "<txz> (interpolated at every level)")
+
+
+def pin_tke_dissipation_single_modes(lib):
+    """dudx = a cos x, dudy = b sin x, dvdx = c sin x, txy = d sin x:
+    <dudx^2> = a^2 / 2, <(dudy + dvdx) / 2 txy> = (b + c) d / 4, and two
+    calls add (the window sums)."""
+    a, b, c, d = 0.75, 0.5, -0.25, 2.0
+    prm, _ = _mk(lib, 16, 16, 6)
+    x, _ = _xy(prm)
+    z = 0 * _levels(prm)
+    _, t = _tke_case(lib, {"dudx": a * np.cos(x) + z, "dudy": b * np.sin(x) + z, "dvdx": c * np.sin(x) + z,
+                           "txy": d * np.sin(x) + z}, calls=2)
+    nz = prm.nz
+    _close(t["dudx2"], np.full(nz, 2 * a * a / 2), 1e-13, "<dudx^2>")
+    _close(t["dxy"], np.full(nz, 2 * (b + c) * d / 4), 1e-13, "<S12 txy>")
+    _close(t["dudy2"], np.full(nz, 2 * b * b / 2), 1e-13, "<dudy^2>")
+
+
+PINS.update({f.__name__[4:]: f for f in (pin_tke_cleaned_pressure, pin_tke_transport_and_interpolation,
+                                          pin_tke_wall_level_stresses, pin_tke_dissipation_single_modes)})
+PLANTED[41] = ("cleaned pressure without the W^2 term", ["tke_cleaned_pressure"])
+PLANTED[42] = ("dp/dz left on the w node (no interpolation)", ["tke_cleaned_pressure"])
+PLANTED[43] = ("w not interpolated to the uvp node", ["tke_transport_and_interpolation"])
+PLANTED[44] = ("u W^2 computed as u^2 W", ["tke_transport_and_interpolation"])
+PLANTED[45] = ("wall-level txz interpolated like the other levels", ["tke_wall_level_stresses"])
+PLANTED[46] = ("S12 without the factor 1/2", ["tke_dissipation_single_modes"])
+PLANTED[47] = ("window sums overwritten instead of added", ["tke_dissipation_single_modes"])

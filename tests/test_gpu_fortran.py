@@ -4,7 +4,10 @@ in, reference-format checkpoint files out.  The checkpoint after N steps must
 equal the CPU oracle's state, and a checkpoint/restart continuation must
 reproduce an uninterrupted run bit for bit.  With stat_flag the RAV files must
 hold the oracle's running sums bit for bit, and running_diagnostics.txt the
-oracle's diagnostics to its printed precision."""
+oracle's diagnostics to its printed precision.  With tke_flag the records of
+ta1_profiles/<name>.out hold the oracle's TKE budget sums / tke_period
+(within 1e-12 of the sums of |value|: the plane sums run in another
+order)."""
 import pathlib
 import subprocess
 
@@ -19,7 +22,7 @@ DRIVER = PKG_DIR / "les_driver"
 
 NLST = """&simulation_param  p_on_y=1 p_on_z=1 nx={nx} ny={ny} nz={nz} nt={nt} dt=2.0d-4 dim_dt_flag=.false. restart_flag={restart} /
 &output_param      out_path='./' output={output} step_out={step_out} stat_flag={stat} start_stats={start} running_period={period}
-                   tke_flag=.false. start_tke=0 tke_period=100 checkpoint={ckpt} scheckpoint=100000 /
+                   tke_flag={tke} start_tke={tstart} tke_period={tperiod} checkpoint={ckpt} scheckpoint=100000 /
 &physical_param    XY_pi=.true. lx=2.0d0 ly=2.0d0 lz=1.0d0 u_scale=1.0d0 zi=1.0d0 lbc_dz0=1.0d-4 Re=1.0d5
                    pfx=1.0d0 pfy=0.0d0 coriolis_flag=.false. ug_dim=0.0d0 vg_dim=0.0d0 geostr_rot=.false. geostr_start=100000 /
 &sgs_param         Cs0=0.16d0 sgs_model='{sgs}' cs_count=5 dyn_init=1 /
@@ -28,9 +31,12 @@ NLST = """&simulation_param  p_on_y=1 p_on_z=1 nx={nx} ny={ny} nz={nz} nt={nt} d
 {extra}"""
 
 
-def _run(d, nx, ny, nz, nt, restart=False, ckpt=100000, stats=None, step_out=None, sgs="dyn_LagScaleDep", extra=""):
+def _run(d, nx, ny, nz, nt, restart=False, ckpt=100000, stats=None, step_out=None, sgs="dyn_LagScaleDep", extra="",
+         tke=None):
     stat, start, period = (".true.", *stats) if stats else (".false.", 0, 100)
+    tk, tstart, tperiod = (".true.", *tke) if tke else (".false.", 0, 100)
     (d / "parameters.nlst").write_text(NLST.format(nx=nx, ny=ny, nz=nz, nt=nt, ckpt=ckpt, sgs=sgs, extra=extra,
+                                                   tke=tk, tstart=tstart, tperiod=tperiod,
                                                    restart=".true." if restart else ".false.",
                                                    stat=stat, start=start, period=period,
                                                    output=".true." if step_out else ".false.",
@@ -144,3 +150,37 @@ def test_fortran_driver_dyn_smag_fsmthg_matches_oracle(tmp_path):
         lev = slice(0, nz) if n.startswith("rhs") else slice(0, nz + 1)
         assert np.array_equal(ck[i][lev], ref[lev]), n
     assert ck[6][1:nz + 1].max() > 0
+
+
+def test_fortran_tke_budget_records(tmp_path):
+    """tke_flag, start_tke = 2, tke_period = 5, 10 steps: record 1 holds the
+    sums of steps 3..5 / 5, record 2 those of steps 6..10 / 5, one file per
+    quantity (src/tke_budget1D.f90:209-440)."""
+    if not DRIVER.exists():
+        pytest.skip("flang driver not built")
+    P, IC, O = load_params(), load_ic(), load_oracle()
+    nx, ny, nz = 16, 16, 8
+    prm = P.LesParams(nx=nx, ny=ny, nz=nz)
+    u, v, w = IC.log_law_ic(nx, ny, nz, prm.lz, prm.z0, 0.1, seed=17)
+    IC.write_vel_files(tmp_path, u, v, w)
+    print("fortran driver: 10 steps with the tke budget", flush=True)
+    _run(tmp_path, nx, ny, nz, 10, tke=(2, 5))
+    o = O.Oracle(prm)
+    o.set_ic(u, v, w)
+    o.tke_reset()
+    recs = []
+    for s in range(1, 11):
+        o.step(1)
+        if s > 2:
+            o.tke_accumulate()
+        if s % 5 == 0:
+            recs.append(o.tke_get(with_abs=True))
+            o.tke_reset()
+    for n in O.Oracle.TKE:
+        f = tmp_path / "ta1_profiles" / f"{n}.out"
+        got = np.fromfile(f, dtype="<f8")
+        assert got.size == 2 * nz, (n, got.size)
+        got = got.reshape(2, nz)
+        for r, (sums, ab) in enumerate(recs):
+            scale = max(ab[n].max(), 1e-300) / 5.0
+            assert np.abs(got[r] - sums[n] / 5.0).max() <= 1e-12 * scale, (n, r)
