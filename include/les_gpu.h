@@ -208,6 +208,14 @@ int les_stat_profiles(double* prof);
  * else status 10.  Calls on one GPU share that array: calls that overlap
  * (two host threads, or two processes' streams) race on it. */
 int les_stat_profiles_noalloc(double* prof);
+/* The geostrophic wind (ug, vg), nondimensional, of the Coriolis forcing
+ * from the next operator or step on: the reference's module variables that
+ * compute_geostrophic_correct rotates every step from geostr_start on when
+ * geostr_rot is set (src/compute_geostrophic.f90:24-45; called at
+ * src/LES_engine.f90:198-201).  les_init takes them from les_params;
+ * les_step(n) keeps them for its n steps, so a rotating caller steps one
+ * at a time (as the Fortran host does). */
+int les_set_geostrophic(double ug, double vg);
 /* TKE budget (src/tke_budget1D.f90:64-440): LES_NTKE per-level sums of
  * plane averages on the uvp nodes of levels 1..nz, in the order of the
  * reference's ta1_profiles/<name>.out files:

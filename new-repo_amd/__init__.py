@@ -107,6 +107,7 @@ def library() -> ctypes.CDLL:
             "les_stat_init": [], "les_stat_reset": [], "les_stat_accumulate": [],
             "les_stat_download": [dp, dp, ip], "les_stat_profiles": [dp], "les_stat_profiles_noalloc": [dp], "les_running_diagnostics": [dp],
             "les_tke_init": [], "les_tke_reset": [], "les_tke_accumulate": [], "les_tke_download": [dp],
+            "les_set_geostrophic": [ctypes.c_double, ctypes.c_double],
         }
         for name, args in sig.items():
             f = getattr(lib, name)
@@ -341,6 +342,9 @@ class LesGpu:
         out = np.empty((len(PROFILES), self.prm.nz))
         check(self.lib.les_stat_profiles_noalloc(_dp(out)))
         return dict(zip(PROFILES, out))
+
+    def set_geostrophic(self, ug: float, vg: float) -> None:
+        check(self.lib.les_set_geostrophic(float(ug), float(vg)))
 
     # ---- TKE budget
     def tke_init(self) -> None:

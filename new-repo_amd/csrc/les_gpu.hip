@@ -1782,6 +1782,14 @@ int les_stat_init(void) {
   return 0;
 }
 
+int les_set_geostrophic(double ug, double vg) {
+  Les* L = g_les;
+  if (!L) return 1;
+  L->P.ug = ug;
+  L->P.vg = vg;
+  return 0;
+}
+
 int les_tke_init(void) {
   Les* L = g_les;
   if (!L) return 1;

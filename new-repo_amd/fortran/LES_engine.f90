@@ -21,6 +21,7 @@ module LES_engine
   use compute_pressure
   use compute_sgs
   use compute_wall_law
+  use compute_geostrophic
   use running_diagnositcs
   use mod_IO_statistics
   use tke_budget1D
@@ -73,6 +74,9 @@ contains
        time_tmp = wtime(.true.)
        tt = tt + dt
        jt_total = jt_total + 1
+
+       ! geostrophic wind correction
+       if (geostr_rot .and. jt_total >= geostr_start) call compute_geostrophic_correct(jt_total)
 
        ! friction velocity and wall stress at k = 1
        call obukhov(wall_model)

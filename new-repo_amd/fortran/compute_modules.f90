@@ -123,3 +123,33 @@ contains
     if (len_trim(wl_type) == 0) return
   end subroutine wallstress
 end module compute_wall_law
+
+!! src/compute_geostrophic.f90:24-45: the geostrophic wind turns with a
+!! 72 h period from geostr_start on, held at its 6 h direction after that,
+!! keeping its magnitude; the device's Coriolis forcing takes the new
+!! (ug, vg) from the next operator on.
+module compute_geostrophic
+  use iso_c_binding
+  use les_gpu_abi
+  use parameters_IO
+  implicit none
+  private
+  real(rprec), parameter :: t_period = 72.0_rprec*3600.0_rprec
+  real(rprec), parameter :: up_limit = 6.0_rprec*3600.0_rprec
+  public :: compute_geostrophic_correct
+contains
+  subroutine compute_geostrophic_correct(jt_total)
+    integer, intent(in) :: jt_total
+    real(rprec) :: ug_new, vg_new, curr_time
+    curr_time = real((jt_total - geostr_start)*dt_dim, rprec)
+    ug_new = sqrt(ug**2 + vg**2)*cos(2.0_rprec*pi*curr_time/t_period)
+    vg_new = sqrt(ug**2 + vg**2)*sin(2.0_rprec*pi*curr_time/t_period)
+    if (curr_time >= up_limit) then
+       ug_new = sqrt(ug**2 + vg**2)*cos((2.0_rprec*pi*up_limit)/t_period)
+       vg_new = sqrt(ug**2 + vg**2)*sin((2.0_rprec*pi*up_limit)/t_period)
+    end if
+    ug = ug_new
+    vg = vg_new
+    call les_check(les_set_geostrophic(ug, vg), 'compute_geostrophic_correct')
+  end subroutine compute_geostrophic_correct
+end module compute_geostrophic

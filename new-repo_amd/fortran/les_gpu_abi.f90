@@ -141,6 +141,10 @@ module les_gpu_abi
        import :: c_int, c_double
        real(c_double), intent(out) :: prof(*)
      end function les_stat_profiles
+     integer(c_int) function les_set_geostrophic(ug, vg) bind(C, name="les_set_geostrophic")
+       import :: c_int, c_double
+       real(c_double), value :: ug, vg
+     end function les_set_geostrophic
      integer(c_int) function les_tke_init() bind(C, name="les_tke_init")
        import :: c_int
      end function les_tke_init

@@ -50,6 +50,9 @@ module parameters_IO
   character(8), public :: dealias = 'PADD'
   ! derived
   real(rprec), public :: dx, dy, dz
+  ! the nondimensional geostrophic wind (rotated by compute_geostrophic_correct)
+  ! and the dimensional time step (src/parameters_IO.f90:71,83,258-271)
+  real(rprec), public :: ug = 0._rprec, vg = 0._rprec, dt_dim
   logical, public :: Lagran_init = .false.
 
   type(les_params_c), public :: cparams
@@ -85,7 +88,12 @@ contains
        lx = lx*pi; ly = ly*pi
     end if
     lx = lx/zi; ly = ly/zi; lz = lz/zi
-    if (dim_dt_flag) dt = dt*u_scale/zi
+    if (dim_dt_flag) then
+       dt_dim = dt
+       dt = dt*u_scale/zi
+    else
+       dt_dim = dt/u_scale*zi
+    end if
     dx = lx/real(nx, rprec); dy = ly/real(ny, rprec); dz = lz/real(nz, rprec)
 
     cparams%nx = nx; cparams%ny = ny; cparams%nz = nz
@@ -132,7 +140,8 @@ contains
     if (coriolis_flag) then
        cparams%coriolis = 1
        cparams%coriol = 1.067E-04*zi/u_scale
-       cparams%ug = ug_dim/u_scale; cparams%vg = vg_dim/u_scale
+       ug = ug_dim/u_scale; vg = vg_dim/u_scale
+       cparams%ug = ug; cparams%vg = vg
     end if
     cparams%lx = lx; cparams%ly = ly; cparams%lz = lz; cparams%dt = dt
     cparams%z0 = lbc_dz0/zi; cparams%pfx = pfx; cparams%pfy = pfy; cparams%cs0 = Cs0

@@ -1653,6 +1653,12 @@ int orc_stat_profiles(double* prof) {
   return 0;
 }
 int orc_diagnostics(double* out) { if (!g) return 1; diagnostics(*g, out); return 0; }
+int orc_set_geostrophic(double ug, double vg) {
+  if (!g) return 1;
+  g->P.ug = ug;
+  g->P.vg = vg;
+  return 0;
+}
 // TKE budget sums (TK_N x nz, row-major by quantity)
 int orc_tke_reset(void) { if (!g) return 1; tke_reset(*g); return 0; }
 int orc_tke_accumulate(void) { if (!g) return 1; tke_accumulate(*g); return 0; }

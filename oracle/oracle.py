@@ -51,6 +51,7 @@ def lib(path=None):
             "orc_stat_profiles": [d], "orc_diagnostics": [d],
             "orc_update_F": [ctypes.c_int, d, d], "orc_dynamic_mum": [d, d, d, d],
             "orc_tke_reset": [], "orc_tke_accumulate": [], "orc_tke_get": [d, d],
+            "orc_set_geostrophic": [ctypes.c_double, ctypes.c_double],
         }.items():
             f = getattr(_lib, name)
             f.argtypes = args
@@ -226,6 +227,9 @@ class Oracle:
         if self.L.orc_stat_profiles(_dp(out)):
             raise ValueError("no accumulated statistics")
         return dict(zip(self.PROFILES, out))
+
+    def set_geostrophic(self, ug, vg):
+        self.L.orc_set_geostrophic(float(ug), float(vg))
 
     # ---- TKE budget (src/tke_budget1D.f90): the file names of its records
     TKE = ["u", "v", "w", "dudx", "dudy", "dudz", "dvdx", "dvdy", "dvdz", "dwdx", "dwdy", "dwdz",

@@ -7,7 +7,8 @@ hold the oracle's running sums bit for bit, and running_diagnostics.txt the
 oracle's diagnostics to its printed precision.  With tke_flag the records of
 ta1_profiles/<name>.out hold the oracle's TKE budget sums / tke_period
 (within 1e-12 of the sums of |value|: the plane sums run in another
-order)."""
+order).  With geostr_rot the state equals the oracle's stepped with the
+geostrophic wind turned by the reference's formula bit for bit."""
 import pathlib
 import subprocess
 
@@ -24,7 +25,7 @@ NLST = """&simulation_param  p_on_y=1 p_on_z=1 nx={nx} ny={ny} nz={nz} nt={nt} d
 &output_param      out_path='./' output={output} step_out={step_out} stat_flag={stat} start_stats={start} running_period={period}
                    tke_flag={tke} start_tke={tstart} tke_period={tperiod} checkpoint={ckpt} scheckpoint=100000 /
 &physical_param    XY_pi=.true. lx=2.0d0 ly=2.0d0 lz=1.0d0 u_scale=1.0d0 zi=1.0d0 lbc_dz0=1.0d-4 Re=1.0d5
-                   pfx=1.0d0 pfy=0.0d0 coriolis_flag=.false. ug_dim=0.0d0 vg_dim=0.0d0 geostr_rot=.false. geostr_start=100000 /
+                   pfx=1.0d0 pfy=0.0d0 coriolis_flag={cor} ug_dim={ug} vg_dim={vg} geostr_rot={geo} geostr_start={gstart} /
 &sgs_param         Cs0=0.16d0 sgs_model='{sgs}' cs_count=5 dyn_init=1 /
 &momentum_bc       lbc_model='std' lbc_special='wall_law' ubc_model='std' ubc_special='std' wall_model='wall_filter' surface_files=.false. /
 &momentum_ic       ic_type='files' u0=0.0d0 v0=0.0d0 w0=0.0d0 u_star=1.0d0 dudz0=0.0d0 z_turb=0.5d0 vnf=1.0d0 /
@@ -32,11 +33,15 @@ NLST = """&simulation_param  p_on_y=1 p_on_z=1 nx={nx} ny={ny} nz={nz} nt={nt} d
 
 
 def _run(d, nx, ny, nz, nt, restart=False, ckpt=100000, stats=None, step_out=None, sgs="dyn_LagScaleDep", extra="",
-         tke=None):
+         tke=None, geo=None):
     stat, start, period = (".true.", *stats) if stats else (".false.", 0, 100)
     tk, tstart, tperiod = (".true.", *tke) if tke else (".false.", 0, 100)
+    # geo = (ug_dim, vg_dim, geostr_start): Coriolis forcing with a turning geostrophic wind
+    cor, ug, vg, gflag, gstart = (".true.", f"{geo[0]!r}d0", f"{geo[1]!r}d0", ".true.", geo[2]) if geo else \
+        (".false.", "0.0d0", "0.0d0", ".false.", 100000)
     (d / "parameters.nlst").write_text(NLST.format(nx=nx, ny=ny, nz=nz, nt=nt, ckpt=ckpt, sgs=sgs, extra=extra,
-                                                   tke=tk, tstart=tstart, tperiod=tperiod,
+                                                   tke=tk, tstart=tstart, tperiod=tperiod, cor=cor, ug=ug, vg=vg,
+                                                   geo=gflag, gstart=gstart,
                                                    restart=".true." if restart else ".false.",
                                                    stat=stat, start=start, period=period,
                                                    output=".true." if step_out else ".false.",
@@ -184,3 +189,43 @@ def test_fortran_tke_budget_records(tmp_path):
         for r, (sums, ab) in enumerate(recs):
             scale = max(ab[n].max(), 1e-300) / 5.0
             assert np.abs(got[r] - sums[n] / 5.0).max() <= 1e-12 * scale, (n, r)
+
+
+def _geostrophic(ug, vg, jt_total, geostr_start, dt_dim):
+    """compute_geostrophic_correct (src/compute_geostrophic.f90:24-45) with
+    the C library's sqrt, cos and sin, as the Fortran host calls them."""
+    import math
+    T, up = 72.0 * 3600.0, 6.0 * 3600.0
+    t = float((jt_total - geostr_start) * dt_dim)
+    m = math.sqrt(ug * ug + vg * vg)
+    a = (2.0 * math.pi * t) / T if t < up else (2.0 * math.pi * up) / T
+    return m * math.cos(a), m * math.sin(a)
+
+
+def test_fortran_geostrophic_rotation(tmp_path):
+    """coriolis_flag with geostr_rot from step 3: each step from 3 on turns
+    (ug, vg) before the step; the checkpoint after 8 steps equals the
+    oracle's with the same wind each step."""
+    if not DRIVER.exists():
+        pytest.skip("flang driver not built")
+    P, IC, O = load_params(), load_ic(), load_oracle()
+    nx, ny, nz = 16, 16, 8
+    ug, vg, gstart = 10.0, 2.0, 3
+    f = float(np.float32(1.067e-4))  # the reference's single-precision constant; zi = u_scale = 1
+    prm = P.LesParams(nx=nx, ny=ny, nz=nz, coriolis=1, coriol=f, ug=ug, vg=vg)
+    u, v, w = IC.log_law_ic(nx, ny, nz, prm.lz, prm.z0, 0.1, seed=17)
+    IC.write_vel_files(tmp_path, u, v, w)
+    print("fortran driver: 8 steps, turning geostrophic wind", flush=True)
+    _run(tmp_path, nx, ny, nz, 8, geo=(ug, vg, gstart))
+    got = _read_ckpt(tmp_path / "checkpoint_000000008", nx, ny, nz, 11)
+    o = O.Oracle(prm)
+    o.set_ic(u, v, w)
+    g = (ug, vg)
+    for jt in range(1, 9):
+        if jt >= gstart:
+            g = _geostrophic(*g, jt, gstart, 2.0e-4)
+            o.set_geostrophic(*g)
+        o.step(1)
+    assert g != (ug, vg)
+    for i, n in enumerate(["u", "v", "w"]):
+        assert np.array_equal(got[i], o.get(n)[1:nz + 2]), n
