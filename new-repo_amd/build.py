@@ -76,7 +76,8 @@ def build_fortran() -> pathlib.Path | None:
     bdir = PKG / "_build" / "fortran"
     bdir.mkdir(parents=True, exist_ok=True)
     order = ["les_gpu_abi.f90", "parameters_IO.f90", "system_variables.f90", "compute_modules.f90",
-             "running_diagnostics.f90", "IO_statistics.f90", "tke_budget1D.f90", "IO_main.f90", "LES_engine.f90",
+             "running_diagnostics.f90", "IO_statistics.f90", "tke_budget1D.f90", "IO_main.f90",
+             "initial_conditions.f90", "LES_engine.f90",
              "main.f90"]
     srcs = [str(FSRC / f) for f in order if (FSRC / f).exists()]
     if not srcs:

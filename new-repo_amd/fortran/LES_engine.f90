@@ -2,8 +2,8 @@
 !! module: LES_engine
 !------------------------------------------------------------------------------
 !! Host time loop of the device LES, following LES_main_engine
-!! (src/LES_engine.f90:77-438): read parameters.nlst, initial fields from
-!! vel_u/v/w.dat (ic_type='files') or a checkpoint, then per step the wall
+!! (src/LES_engine.f90:77-438): read parameters.nlst, initial fields
+!! (module initial_conditions) or a checkpoint, then per step the wall
 !! law, velocity gradients, SGS stress, right-hand side, AB2 advance,
 !! pressure and projection, with the reference's running diagnostics
 !! (running_diagnostics.txt), NaN check, running-average statistics (RAV
@@ -22,6 +22,7 @@ module LES_engine
   use compute_sgs
   use compute_wall_law
   use compute_geostrophic
+  use initial_conditions
   use running_diagnositcs
   use mod_IO_statistics
   use tke_budget1D
@@ -115,6 +116,7 @@ contains
        if (stat_flag .and. jt_total > start_stats) call io_stat_averages(jt, jt_total)
        if (tke_flag .and. jt_total > start_tke) call compute_tke_budget(jt)
        if (output .and. mod(jt, step_out) == 0) call output_loop(jt, jt_total)
+       if (mod(jt, scheckpoint) == 0) call scheckpoint_out(jt_total)
        if (mod(jt_total, checkpoint) == 0) call checkpoint_out(jt_total)
     end do
 
@@ -158,45 +160,30 @@ contains
     call les_check(rc, 'check_NaN')
   end subroutine check_NaN
 
-  !! src/initial_conditions.f90:130-208 (ic_type = 'files')
-  subroutine set_initial_conditions()
-    real(rprec), allocatable :: uu(:,:,:), vv(:,:,:), ww(:,:,:)
-    if (trim(ic_type) /= 'files') then
-       write(*,*) 'ABORT - only ic_type = files is supported by the device engine'
-       error stop 1
-    end if
-    allocate(uu(nx, ny, nz), vv(nx, ny, nz), ww(nx, ny, nz))
-    call read_raw('vel_u.dat', uu)
-    call read_raw('vel_v.dat', vv)
-    call read_raw('vel_w.dat', ww)
-    call les_check(les_set_ic(uu, vv, ww), 'set_initial_conditions')
-    deallocate(uu, vv, ww)
-  end subroutine set_initial_conditions
-
-  subroutine read_raw(name, a)
-    character(*), intent(in) :: name
-    real(rprec), intent(out) :: a(:,:,:)
-    integer :: ios
-    open(98, file=name, form='unformatted', access='stream', status='old', action='read', iostat=ios)
-    if (ios /= 0) then
-       write(*,*) 'ABORT - cannot open ', trim(name)
-       error stop 1
-    end if
-    read(98) a
-    close(98)
-  end subroutine read_raw
-
   !! checkpoint file (src/IO_checkpoint.f90:119-183): levels 1..nz+1 of
   !! u, v, w, rhsx, rhsy, rhsz, cs_opt2 and, once the Lagrangian averages
   !! exist, F_LM, F_MM, F_QN, F_NN; float64, Fortran order.
   subroutine checkpoint_out(jt)
     integer, intent(in) :: jt
     character(128) :: filename
+    write(filename,'(A,I9.9)') 'checkpoint_', jt
+    write(*,'(A,A)') '# writing checkpoint file: ', trim(filename)
+    call write_checkpoint_file(filename, jt)
+  end subroutine checkpoint_out
+
+  !! the same content under the fixed name 'scheckpoint', every scheckpoint
+  !! steps of this run (src/IO_checkpoint.f90:100-107; src/LES_engine.f90:387-389)
+  subroutine scheckpoint_out(jt_total)
+    integer, intent(in) :: jt_total
+    call write_checkpoint_file('scheckpoint', jt_total)
+  end subroutine scheckpoint_out
+
+  subroutine write_checkpoint_file(filename, jt)
+    character(*), intent(in) :: filename
+    integer, intent(in) :: jt
     real(rprec), allocatable :: a(:,:,:)
     integer(c_int) :: ids(11)
     integer :: i, n
-    write(filename,'(A,I9.9)') 'checkpoint_', jt
-    write(*,'(A,A)') '# writing checkpoint file: ', trim(filename)
     ids = [u, v, w, rhsx, rhsy, rhsz, cs_opt2, F_LM, F_MM, F_QN, F_NN]
     n = 7
     if (trim(sgs_model) == 'dyn_LagScaleDep' .and. cs_count > 0) then
@@ -211,7 +198,7 @@ contains
     end do
     close(97)
     deallocate(a)
-  end subroutine checkpoint_out
+  end subroutine write_checkpoint_file
 
   subroutine checkpoint_in(jt)
     integer, intent(in) :: jt

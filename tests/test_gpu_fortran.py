@@ -8,7 +8,10 @@ oracle's diagnostics to its printed precision.  With tke_flag the records of
 ta1_profiles/<name>.out hold the oracle's TKE budget sums / tke_period
 (within 1e-12 of the sums of |value|: the plane sums run in another
 order).  With geostr_rot the state equals the oracle's stepped with the
-geostrophic wind turned by the reference's formula bit for bit."""
+geostrophic wind turned by the reference's formula bit for bit.  With
+ic_type 'log_law', 'poiseuille', 'constant' or the Coriolis start the
+state after a step equals the oracle's started from the restatement of the
+reference's initial fields and noise (oracle/ref_ic.py) bit for bit."""
 import pathlib
 import subprocess
 
@@ -23,17 +26,17 @@ DRIVER = PKG_DIR / "les_driver"
 
 NLST = """&simulation_param  p_on_y=1 p_on_z=1 nx={nx} ny={ny} nz={nz} nt={nt} dt=2.0d-4 dim_dt_flag=.false. restart_flag={restart} /
 &output_param      out_path='./' output={output} step_out={step_out} stat_flag={stat} start_stats={start} running_period={period}
-                   tke_flag={tke} start_tke={tstart} tke_period={tperiod} checkpoint={ckpt} scheckpoint=100000 /
+                   tke_flag={tke} start_tke={tstart} tke_period={tperiod} checkpoint={ckpt} scheckpoint={sckpt} /
 &physical_param    XY_pi=.true. lx=2.0d0 ly=2.0d0 lz=1.0d0 u_scale=1.0d0 zi=1.0d0 lbc_dz0=1.0d-4 Re=1.0d5
                    pfx=1.0d0 pfy=0.0d0 coriolis_flag={cor} ug_dim={ug} vg_dim={vg} geostr_rot={geo} geostr_start={gstart} /
 &sgs_param         Cs0=0.16d0 sgs_model='{sgs}' cs_count=5 dyn_init=1 /
 &momentum_bc       lbc_model='std' lbc_special='wall_law' ubc_model='std' ubc_special='std' wall_model='wall_filter' surface_files=.false. /
-&momentum_ic       ic_type='files' u0=0.0d0 v0=0.0d0 w0=0.0d0 u_star=1.0d0 dudz0=0.0d0 z_turb=0.5d0 vnf=1.0d0 /
+&momentum_ic       ic_type='{ic}' u0=0.0d0 v0=0.0d0 w0=0.0d0 u_star=1.0d0 dudz0=0.0d0 z_turb=0.5d0 vnf=1.0d0 /
 {extra}"""
 
 
 def _run(d, nx, ny, nz, nt, restart=False, ckpt=100000, stats=None, step_out=None, sgs="dyn_LagScaleDep", extra="",
-         tke=None, geo=None):
+         tke=None, geo=None, ic="files", sckpt=100000):
     stat, start, period = (".true.", *stats) if stats else (".false.", 0, 100)
     tk, tstart, tperiod = (".true.", *tke) if tke else (".false.", 0, 100)
     # geo = (ug_dim, vg_dim, geostr_start): Coriolis forcing with a turning geostrophic wind
@@ -41,14 +44,14 @@ def _run(d, nx, ny, nz, nt, restart=False, ckpt=100000, stats=None, step_out=Non
         (".false.", "0.0d0", "0.0d0", ".false.", 100000)
     (d / "parameters.nlst").write_text(NLST.format(nx=nx, ny=ny, nz=nz, nt=nt, ckpt=ckpt, sgs=sgs, extra=extra,
                                                    tke=tk, tstart=tstart, tperiod=tperiod, cor=cor, ug=ug, vg=vg,
-                                                   geo=gflag, gstart=gstart,
+                                                   geo=gflag, gstart=gstart, ic=ic, sckpt=sckpt,
                                                    restart=".true." if restart else ".false.",
                                                    stat=stat, start=start, period=period,
                                                    output=".true." if step_out else ".false.",
                                                    step_out=step_out or 1000))
     r = subprocess.run([str(DRIVER)], cwd=d, capture_output=True, text=True, timeout=120)
     print(r.stdout[-800:], r.stderr[-800:], flush=True)
-    assert r.returncode == 0
+    assert r.returncode == 0, f"driver exit status {r.returncode}"
 
 
 def _read_ckpt(path, nx, ny, nz, nvar):
@@ -213,11 +216,19 @@ def test_fortran_geostrophic_rotation(tmp_path):
     ug, vg, gstart = 10.0, 2.0, 3
     f = float(np.float32(1.067e-4))  # the reference's single-precision constant; zi = u_scale = 1
     prm = P.LesParams(nx=nx, ny=ny, nz=nz, coriolis=1, coriol=f, ug=ug, vg=vg)
-    u, v, w = IC.log_law_ic(nx, ny, nz, prm.lz, prm.z0, 0.1, seed=17)
-    IC.write_vel_files(tmp_path, u, v, w)
     print("fortran driver: 8 steps, turning geostrophic wind", flush=True)
-    _run(tmp_path, nx, ny, nz, 8, geo=(ug, vg, gstart))
+    _run(tmp_path, nx, ny, nz, 8, geo=(ug, vg, gstart), ic="log_law")
     got = _read_ckpt(tmp_path / "checkpoint_000000008", nx, ny, nz, 11)
+    # with coriolis_flag the reference starts from the geostrophic wind
+    # whatever ic_type says but 'constant' (src/initial_conditions.f90:81-100),
+    # with the seeded noise (without it the plane-uniform start has S = 0
+    # and the dynamic model's first update divides by M:M = 0)
+    import importlib.util
+    from conftest import ROOT
+    spec = importlib.util.spec_from_file_location("ref_ic", ROOT / "oracle" / "ref_ic.py")
+    R = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(R)
+    u, v, w = R.reference_ic(prm, "log_law", coriolis=True)
     o = O.Oracle(prm)
     o.set_ic(u, v, w)
     g = (ug, vg)
@@ -229,3 +240,53 @@ def test_fortran_geostrophic_rotation(tmp_path):
     assert g != (ug, vg)
     for i, n in enumerate(["u", "v", "w"]):
         assert np.array_equal(got[i], o.get(n)[1:nz + 2]), n
+
+
+@pytest.mark.parametrize("ic", ["log_law", "poiseuille", "constant", "coriolis"])
+def test_fortran_initial_conditions(tmp_path, ic):
+    """set_initial_conditions of the Fortran host (ic_type, or the Coriolis
+    start with coriolis_flag) with the reference's seeded noise, then one
+    step: equal to the oracle's step from oracle/ref_ic.py's fields."""
+    if not DRIVER.exists():
+        pytest.skip("flang driver not built")
+    import importlib.util
+    from conftest import ROOT
+    spec = importlib.util.spec_from_file_location("ref_ic", ROOT / "oracle" / "ref_ic.py")
+    R = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(R)
+    P, O = load_params(), load_oracle()
+    nx, ny, nz = 16, 16, 8
+    cor = ic == "coriolis"
+    kw = dict(coriolis=1, coriol=float(np.float32(1.067e-4)), ug=10.0, vg=2.0) if cor else {}
+    prm = P.LesParams(nx=nx, ny=ny, nz=nz, **kw)
+    print(f"fortran driver: ic {ic}, 1 step", flush=True)
+    _run(tmp_path, nx, ny, nz, 1, ic="log_law" if cor else ic, geo=(10.0, 2.0, 100000) if cor else None)
+    # before the first Lagrangian update the checkpoint holds 7 fields
+    got = _read_ckpt(tmp_path / "checkpoint_000000001", nx, ny, nz, 7)
+    u, v, w = R.reference_ic(prm, "log_law" if cor else ic, coriolis=cor)
+    o = O.Oracle(prm)
+    o.set_ic(u, v, w)
+    o.step(1)
+    for i, n in enumerate(["u", "v", "w"]):
+        assert np.array_equal(got[i], o.get(n)[1:nz + 2]), n
+
+
+def test_fortran_short_checkpoint(tmp_path):
+    """scheckpoint = 4 over 10 steps: the file 'scheckpoint' is rewritten at
+    steps 4 and 8 and holds the step-8 state (src/LES_engine.f90:387-389)."""
+    if not DRIVER.exists():
+        pytest.skip("flang driver not built")
+    P, IC, O = load_params(), load_ic(), load_oracle()
+    nx, ny, nz = 16, 16, 8
+    prm = P.LesParams(nx=nx, ny=ny, nz=nz)
+    u, v, w = IC.log_law_ic(nx, ny, nz, prm.lz, prm.z0, 0.1, seed=17)
+    IC.write_vel_files(tmp_path, u, v, w)
+    print("fortran driver: 10 steps, short checkpoint every 4", flush=True)
+    _run(tmp_path, nx, ny, nz, 10, sckpt=4)
+    got = _read_ckpt(tmp_path / "scheckpoint", nx, ny, nz, 11)
+    o = O.Oracle(prm)
+    o.set_ic(u, v, w)
+    o.step(8)
+    for i, n in enumerate(["u", "v", "w", "cs_opt2"]):
+        j = [0, 1, 2, 6][i]
+        assert np.array_equal(got[j], o.get(n)[1:nz + 2]), n
