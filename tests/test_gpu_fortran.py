@@ -290,3 +290,55 @@ def test_fortran_short_checkpoint(tmp_path):
     for i, n in enumerate(["u", "v", "w", "cs_opt2"]):
         j = [0, 1, 2, 6][i]
         assert np.array_equal(got[j], o.get(n)[1:nz + 2]), n
+
+
+# The values of the reference's sample parameters.nlst (cst_Smag, log-law
+# start with noise over the whole depth, dimensional dt, statistics, TKE
+# budget, snapshots and both checkpoints on), with out_path set to the run
+# folder and the one-GPU pencil grid.
+SAMPLE = """&simulation_param p_on_y=1 p_on_z=1 nx=32 ny=32 nz=32 nt=100 dt=0.2d0 dim_dt_flag=.true. restart_flag=.false. /
+&output_param out_path='./' output=.true. step_out=10 stat_flag=.true. start_stats=50 running_period=10
+              tke_flag=.true. start_tke=50 tke_period=10 checkpoint=100 scheckpoint=10 /
+&physical_param XY_pi=.true. lx=1000.0d0 ly=1000.0d0 lz=1000.0d0 u_scale=0.45d0 zi=1000.0d0 lbc_dz0=0.1d0
+                Re=50000.0d0 pfx=1.0d0 pfy=0.0d0 coriolis_flag=.false. ug_dim=0.0d0 vg_dim=0.0d0 geostr_rot=.false.
+                geostr_start=100000 /
+&sgs_param Cs0=0.16d0 sgs_model='cst_Smag' cs_count=5 dyn_init=1500 /
+&momentum_bc lbc_model='std' lbc_special='wall_law' ubc_model='std' ubc_special='std' wall_model='wall_filter'
+             surface_files=.false. /
+&momentum_ic ic_type='log_law' u0=1.0d0 v0=0.0d0 w0=0.0d0 u_star=1.0d0 dudz0=0.0d0 z_turb=1.0d0 vnf=1.0d0 /
+"""
+
+
+def test_fortran_reference_sample_configuration(tmp_path):
+    """The reference's sample configuration end to end through the Fortran
+    host: 100 steps from its log-law start, every output it switches on
+    written, and the final checkpoint equal to the oracle's run from the
+    restated initial fields bit for bit."""
+    if not DRIVER.exists():
+        pytest.skip("flang driver not built")
+    import importlib.util
+    from conftest import ROOT
+    spec = importlib.util.spec_from_file_location("ref_ic", ROOT / "oracle" / "ref_ic.py")
+    R = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(R)
+    P, O = load_params(), load_oracle()
+    (tmp_path / "parameters.nlst").write_text(SAMPLE)
+    print("fortran driver: the sample configuration, 100 steps", flush=True)
+    r = subprocess.run([str(DRIVER)], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    print(r.stdout[-600:], r.stderr[-600:], flush=True)
+    assert r.returncode == 0
+    prm = P.params_from_namelist(SAMPLE)
+    nx = ny = nz = 32
+    for name in ["checkpoint_000000100", "scheckpoint", "running_diagnostics.txt", "profiling.txt",
+                 "RAV_000000100_3D.dat", "2decomp_sys_000000100.dat"]:
+        assert any(p.name.startswith(name.split(".")[0]) for p in tmp_path.iterdir()), name
+    assert len(list((tmp_path / "ta1_profiles").glob("*.out"))) == 61
+    got = _read_ckpt(tmp_path / "checkpoint_000000100", nx, ny, nz, 7)
+    u, v, w = R.reference_ic(prm, "log_law", u_star=1.0, z_turb=1.0, vnf=1.0)
+    o = O.Oracle(prm)
+    o.set_ic(u, v, w)
+    o.step(100)
+    for i, n in enumerate(["u", "v", "w", "rhsx", "rhsy", "rhsz", "cs_opt2"]):
+        ref = o.get(n)[1:nz + 2]
+        lev = slice(0, nz) if n.startswith("rhs") else slice(0, nz + 1)
+        assert np.array_equal(got[i][lev], ref[lev]), n
