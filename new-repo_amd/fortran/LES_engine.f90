@@ -58,6 +58,8 @@ contains
     end if
     if (tke_flag) call tke_budget_init()
     call les_check(les_init(cparams), 'les_init')
+    ! a roughness map (surface_files): the wall law's z0 per surface point
+    if (surface_files) call les_check(les_upload_layer(LES_Z0, lbc_z0), 'set_bc_surface')
     call running_diagnostics_init()
     if (output) call io_init()
     if (stat_flag) call io_stat_init()

@@ -102,7 +102,7 @@ contains
 
   subroutine set_initial_conditions()
     real(rprec), allocatable :: u(:,:,:), v(:,:,:), w(:,:,:)
-    real(rprec) :: norm, dir_x, dir_y, z, arg
+    real(rprec) :: norm, dir_x, dir_y, z
     integer :: k
 
     if (trim(ic_type) == 'do_nothing' .and. .not. coriolis_flag) return
@@ -129,9 +129,8 @@ contains
        do k = 1, nz
           z = (k - 0.5_rprec)*dz
           if (z >= 0._rprec) then
-             arg = (1._rprec/KvonK)*log(z/(lbc_dz0/zi))
-             u(:,:,k) = dir_x*arg*u_star
-             v(:,:,k) = dir_y*arg*u_star
+             u(:,:,k) = dir_x*((1._rprec/KvonK)*log(z/lbc_z0))*u_star
+             v(:,:,k) = dir_y*((1._rprec/KvonK)*log(z/lbc_z0))*u_star
           end if
        end do
     else if (trim(ic_type) == 'poiseuille') then

@@ -54,6 +54,11 @@ module les_gpu_abi
        integer(c_int), value :: id
        real(c_double), intent(out) :: a(*)
      end function les_download_field
+     integer(c_int) function les_upload_layer(id, a) bind(C, name="les_upload_layer")
+       import :: c_int, c_double
+       integer(c_int), value :: id
+       real(c_double), intent(in) :: a(*)
+     end function les_upload_layer
      integer(c_int) function les_download_layer(id, a) bind(C, name="les_download_layer")
        import :: c_int, c_double
        integer(c_int), value :: id

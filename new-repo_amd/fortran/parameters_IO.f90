@@ -53,6 +53,10 @@ module parameters_IO
   ! the nondimensional geostrophic wind (rotated by compute_geostrophic_correct)
   ! and the dimensional time step (src/parameters_IO.f90:71,83,258-271)
   real(rprec), public :: ug = 0._rprec, vg = 0._rprec, dt_dim
+  ! the nondimensional roughness per surface point (set_bc_surface,
+  ! src/boundary_condition_mpi.f90:100-130): lbc_dz0 / zi, or
+  ! ./input/surface_z0.dat / zi with surface_files
+  real(rprec), allocatable, public :: lbc_z0(:,:)
   logical, public :: Lagran_init = .false.
 
   type(les_params_c), public :: cparams
@@ -146,6 +150,26 @@ contains
     cparams%lx = lx; cparams%ly = ly; cparams%lz = lz; cparams%dt = dt
     cparams%z0 = lbc_dz0/zi; cparams%pfx = pfx; cparams%pfy = pfy; cparams%cs0 = Cs0
     cparams%ubc_u = 0._rprec; cparams%ubc_v = 0._rprec; cparams%ubc_w = 0._rprec
+    if (allocated(lbc_z0)) deallocate(lbc_z0)
+    allocate(lbc_z0(nx, ny))
+    if (surface_files) then
+       block
+         real(rprec) :: z0_in(nx, ny)
+         integer :: rl, ios
+         inquire(iolength=rl) z0_in
+         open(98, file='./input/surface_z0.dat', form='unformatted', access='direct', recl=rl, status='old', &
+              action='read', iostat=ios)
+         if (ios /= 0) then
+            write(*,*) 'ABORT - cannot open ./input/surface_z0.dat'
+            error stop 1
+         end if
+         read(98, rec=1) z0_in
+         close(98)
+         lbc_z0 = z0_in/zi
+       end block
+    else
+       lbc_z0 = lbc_dz0/zi
+    end if
   end subroutine read_parameters
 
   subroutine abort22(msg)

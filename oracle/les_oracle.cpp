@@ -1671,6 +1671,10 @@ int orc_tke_get(double* sums, double* abs_sums) {
 }
 int orc_set_2d(int which, const double* in) {
   if (!g) return 1;
+  if (which == 5) {  // a surface roughness map z0(x, y): the wall law's log(0.5 dz / z0) per point
+    for (size_t q = 0; q < g->np; ++q) g->logterm[q] = std::log((0.5 * g->dz) / in[q]);
+    return 0;
+  }
   std::vector<double>* f = which == 0 ? &g->lbc_u : which == 1 ? &g->lbc_v : which == 2 ? &g->ustar
                          : which == 3 ? &g->bottom_wk : &g->top_wk;
   std::memcpy(f->data(), in, sizeof(double) * g->np);

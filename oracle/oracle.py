@@ -152,7 +152,8 @@ class Oracle:
         return out
 
     def set2d(self, which, arr):
-        idx = {"lbc_u": 0, "lbc_v": 1, "ustar": 2, "bottom_wk": 3, "top_wk": 4}[which]
+        """A surface layer, or "z0": a roughness map (x, y) for the wall law."""
+        idx = {"lbc_u": 0, "lbc_v": 1, "ustar": 2, "bottom_wk": 3, "top_wk": 4, "z0": 5}[which]
         a = np.ascontiguousarray(arr, dtype=np.float64)
         self.L.orc_set_2d(idx, _dp(a))
 

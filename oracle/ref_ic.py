@@ -55,12 +55,14 @@ class Ran3:
         return mj * (1.0 / MBIG), seed
 
 
-def reference_ic(prm, ic_type="log_law", u_star=1.0, z_turb=0.5, vnf=1.0, u0=0.0, v0=0.0, w0=0.0, coriolis=False):
+def reference_ic(prm, ic_type="log_law", u_star=1.0, z_turb=0.5, vnf=1.0, u0=0.0, v0=0.0, w0=0.0, coriolis=False,
+                 z0=None):
     """u, v, w (nz, ny, nx) on levels 1..nz as set_initial_conditions leaves
     them before the boundary conditions; prm supplies nx, ny, nz, lz, z0,
     pfx, pfy, ug, vg and re (Poiseuille's Re).  As in the reference, the
     Coriolis start replaces every type but 'constant' (also 'files', whose
-    fields are then never read), and 'files' gets no noise."""
+    fields are then never read), and 'files' gets no noise.  z0: a
+    roughness map (ny, nx) (surface_files) instead of prm.z0."""
     nx, ny, nz = prm.nx, prm.ny, prm.nz
     dz = prm.lz / nz
     u = np.zeros((nz, ny, nx))
@@ -77,7 +79,10 @@ def reference_ic(prm, ic_type="log_law", u_star=1.0, z_turb=0.5, vnf=1.0, u0=0.0
             if z <= 0.5:
                 v[k - 1] = prm.vg
         elif ic_type == "log_law":
-            arg = (1.0 / 0.4) * math.log(z / prm.z0)
+            if z0 is None:
+                arg = (1.0 / 0.4) * math.log(z / prm.z0)
+            else:
+                arg = (1.0 / 0.4) * np.array([[math.log(z / q) for q in row] for row in z0])
             u[k - 1] = dx * arg * u_star
             v[k - 1] = dy * arg * u_star
         elif ic_type == "poiseuille":

@@ -30,13 +30,13 @@ NLST = """&simulation_param  p_on_y=1 p_on_z=1 nx={nx} ny={ny} nz={nz} nt={nt} d
 &physical_param    XY_pi=.true. lx=2.0d0 ly=2.0d0 lz=1.0d0 u_scale=1.0d0 zi=1.0d0 lbc_dz0=1.0d-4 Re=1.0d5
                    pfx=1.0d0 pfy=0.0d0 coriolis_flag={cor} ug_dim={ug} vg_dim={vg} geostr_rot={geo} geostr_start={gstart} /
 &sgs_param         Cs0=0.16d0 sgs_model='{sgs}' cs_count=5 dyn_init=1 /
-&momentum_bc       lbc_model='std' lbc_special='wall_law' ubc_model='std' ubc_special='std' wall_model='wall_filter' surface_files=.false. /
+&momentum_bc       lbc_model='std' lbc_special='wall_law' ubc_model='std' ubc_special='std' wall_model='wall_filter' surface_files={sfiles} /
 &momentum_ic       ic_type='{ic}' u0=0.0d0 v0=0.0d0 w0=0.0d0 u_star=1.0d0 dudz0=0.0d0 z_turb=0.5d0 vnf=1.0d0 /
 {extra}"""
 
 
 def _run(d, nx, ny, nz, nt, restart=False, ckpt=100000, stats=None, step_out=None, sgs="dyn_LagScaleDep", extra="",
-         tke=None, geo=None, ic="files", sckpt=100000):
+         tke=None, geo=None, ic="files", sckpt=100000, surface_files=False):
     stat, start, period = (".true.", *stats) if stats else (".false.", 0, 100)
     tk, tstart, tperiod = (".true.", *tke) if tke else (".false.", 0, 100)
     # geo = (ug_dim, vg_dim, geostr_start): Coriolis forcing with a turning geostrophic wind
@@ -45,6 +45,7 @@ def _run(d, nx, ny, nz, nt, restart=False, ckpt=100000, stats=None, step_out=Non
     (d / "parameters.nlst").write_text(NLST.format(nx=nx, ny=ny, nz=nz, nt=nt, ckpt=ckpt, sgs=sgs, extra=extra,
                                                    tke=tk, tstart=tstart, tperiod=tperiod, cor=cor, ug=ug, vg=vg,
                                                    geo=gflag, gstart=gstart, ic=ic, sckpt=sckpt,
+                                                   sfiles=".true." if surface_files else ".false.",
                                                    restart=".true." if restart else ".false.",
                                                    stat=stat, start=start, period=period,
                                                    output=".true." if step_out else ".false.",
@@ -342,3 +343,40 @@ def test_fortran_reference_sample_configuration(tmp_path):
         ref = o.get(n)[1:nz + 2]
         lev = slice(0, nz) if n.startswith("rhs") else slice(0, nz + 1)
         assert np.array_equal(got[i][lev], ref[lev]), n
+
+
+def test_fortran_surface_roughness_map(tmp_path):
+    """surface_files: a roughness map from ./input/surface_z0.dat sets the
+    log-law start and the wall law point by point
+    (src/boundary_condition_mpi.f90:100-130); two steps equal the oracle's
+    with the same map bit for bit."""
+    if not DRIVER.exists():
+        pytest.skip("flang driver not built")
+    import importlib.util
+    from conftest import ROOT
+    spec = importlib.util.spec_from_file_location("ref_ic", ROOT / "oracle" / "ref_ic.py")
+    R = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(R)
+    P, O = load_params(), load_oracle()
+    nx, ny, nz = 16, 16, 8
+    prm = P.LesParams(nx=nx, ny=ny, nz=nz)
+    x = np.arange(nx) / nx
+    z0 = np.broadcast_to(1e-4 * (1.0 + 9.0 * (x > 0.5)), (ny, nx)).copy()  # smooth and rough halves
+    (tmp_path / "input").mkdir()
+    z0.astype("<f8").tofile(tmp_path / "input" / "surface_z0.dat")
+    print("fortran driver: roughness map, 2 steps", flush=True)
+    _run(tmp_path, nx, ny, nz, 2, ic="log_law", surface_files=True)
+    got = _read_ckpt(tmp_path / "checkpoint_000000002", nx, ny, nz, 7)
+    u, v, w = R.reference_ic(prm, "log_law", z0=z0)
+    o = O.Oracle(prm)
+    o.set2d("z0", z0)
+    o.set_ic(u, v, w)
+    o.step(2)
+    for i, n in enumerate(["u", "v", "w"]):
+        assert np.array_equal(got[i], o.get(n)[1:nz + 2]), n
+    # the map matters beyond the start: the same start with the uniform z0 in
+    # the wall law steps elsewhere
+    o2 = O.Oracle(prm)
+    o2.set_ic(u, v, w)
+    o2.step(2)
+    assert not np.array_equal(got[0], o2.get("u")[1:nz + 2])
