@@ -1715,7 +1715,7 @@ static void diagnostics(Les& L) {
 // compute_tke_budget: the cleaned pressure through the FP64
 // grad_velocity_filt chain (the filtered field back into tk_pf, in place in
 // the column pass like grad_filt3), then the plane sums of the 61
-// quantities in two launches of 31 and 30, then the window sums.
+// quantities in one pass over the fields, then the window sums.
 template <typename T>
 static void tke_accumulate(Les& L) {
   TkePcleanArgs c;
@@ -1746,11 +1746,8 @@ static void tke_accumulate(Les& L) {
   a.part = L.tk_part; a.np = L.np; a.nz = L.nz; a.cz = 1.0 / L.dz;
   const double rd = (double)L.np * L.nz * (4.0 * 8 + 19.0 * sizeof(T) + 3.0 * 8);
   kt_begin(L, KT_POINTWISE);
-  hipLaunchKernelGGL((k_tke_planes<T, 0, 31>), dim3(LES_TKE_SLICES, L.nz), dim3(LES_BLOCK), 0, L.stream, a);
-  kt_end(L, KT_POINTWISE, rd, "k_tke_planes", sizeof(T), 0, "0-30");
-  kt_begin(L, KT_POINTWISE);
-  hipLaunchKernelGGL((k_tke_planes<T, 31, TK_N>), dim3(LES_TKE_SLICES, L.nz), dim3(LES_BLOCK), 0, L.stream, a);
-  kt_end(L, KT_POINTWISE, rd, "k_tke_planes", sizeof(T), 0, "31-60");
+  hipLaunchKernelGGL(k_tke_sums<T>, dim3(LES_TKE_SLICES, L.nz), dim3(LES_BLOCK), 0, L.stream, a);
+  kt_end(L, KT_POINTWISE, rd, "k_tke_sums", sizeof(T));
   hipLaunchKernelGGL(k_tke_add, dim3(nblk((long)TK_N * L.nz)), dim3(LES_BLOCK), 0, L.stream, (const double*)L.tk_part,
                      L.tk_sum, L.nz, (double)(L.nx * L.ny));
 }

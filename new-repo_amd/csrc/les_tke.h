@@ -108,30 +108,39 @@ __device__ __forceinline__ void tke_point(const TkeArgs<T>& A, int k, long q, do
   x[TK_PDWDX] = Pf * Idwdx; x[TK_PDUDZ] = Pf * Idudz;
 }
 
-// One block per (slice, level) and quantities Q0..Q1-1: each thread sums its
-// points of the slice, then the block's sums are combined in thread order.
-template <typename T, int Q0, int Q1>
-__global__ void __launch_bounds__(LES_BLOCK) k_tke_planes(TkeArgs<T> a) {
-  constexpr int NQ = Q1 - Q0;
-  __shared__ double red[NQ * LES_BLOCK];
+// One block per (slice, level), all quantities in one pass over the fields:
+// each thread sums its points of the slice, then the block's sums are
+// combined in thread order, quantities 0..30 and then 31..60 through one
+// LDS buffer (31 x 256 doubles, 63.5 KB).
+#define LES_TKE_RED 31
+template <typename T>
+__global__ void __launch_bounds__(LES_BLOCK) k_tke_sums(TkeArgs<T> a) {
+  __shared__ double red[LES_TKE_RED * LES_BLOCK];
   const int slice = blockIdx.x, k = blockIdx.y + 1;
   const long q0 = a.np * slice / LES_TKE_SLICES, q1 = a.np * (slice + 1) / LES_TKE_SLICES;
-  double acc[NQ];
+  double acc[TK_N];
 #pragma unroll
-  for (int f = 0; f < NQ; ++f) acc[f] = 0.0;
+  for (int f = 0; f < TK_N; ++f) acc[f] = 0.0;
   for (long q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
     double x[TK_N];
     tke_point(a, k, q, x);
 #pragma unroll
-    for (int f = 0; f < NQ; ++f) acc[f] += x[Q0 + f];
+    for (int f = 0; f < TK_N; ++f) acc[f] += x[f];
   }
+  double* out = a.part + ((long)(k - 1) * LES_TKE_SLICES + slice) * TK_N;
 #pragma unroll
-  for (int f = 0; f < NQ; ++f) red[f * LES_BLOCK + threadIdx.x] = acc[f];
-  __syncthreads();
-  if ((int)threadIdx.x < NQ) {
-    double s = 0.0;
-    for (int t = 0; t < LES_BLOCK; ++t) s += red[threadIdx.x * LES_BLOCK + t];
-    a.part[((long)(k - 1) * LES_TKE_SLICES + slice) * TK_N + Q0 + threadIdx.x] = s;
+  for (int h = 0; h < 2; ++h) {
+    const int f0 = h * LES_TKE_RED, nq = h == 0 ? LES_TKE_RED : TK_N - LES_TKE_RED;
+    if (h) __syncthreads();  // the first half's sums have been read
+#pragma unroll
+    for (int f = 0; f < LES_TKE_RED; ++f)
+      if (f < nq) red[f * LES_BLOCK + threadIdx.x] = acc[f0 + f];
+    __syncthreads();
+    if ((int)threadIdx.x < nq) {
+      double s = 0.0;
+      for (int t = 0; t < LES_BLOCK; ++t) s += red[threadIdx.x * LES_BLOCK + t];
+      out[f0 + threadIdx.x] = s;
+    }
   }
 }
 
