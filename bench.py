@@ -366,8 +366,8 @@ def tke_leg(pkg, P, ic, n=256, calls=10):
     """One call of the TKE budget sums (compute_tke_budget, every step past
     start_tke when tke_flag is set) at 256^3 after two steps, FP64 and
     mixed: milliseconds per call and the algorithmic bytes per point (the
-    cleaned pressure and its FP64 transform chain, then one pass over the
-    fields for the 61 plane sums)."""
+    FP64 transform chain of the cleaned pressure, formed from p, u, v, w on
+    load, then one pass over the fields for the 61 plane sums)."""
     out = {}
     for name, prec, tb in (("fp64", P.PREC_FP64, 8), ("mixed", P.PREC_MIXED, 4)):
         prm = P.LesParams(nx=n, ny=n, nz=n, dt=DT, precision=prec)
@@ -384,7 +384,7 @@ def tke_leg(pkg, P, ic, n=256, calls=10):
             g.sync()
             ms = (time.perf_counter() - t0) / calls * 1e3
         spec = 16.0 * (n // 2 + 1) / n  # one FP64 half spectrum per point
-        bpp = 5 * 8 + (8 + spec) + 4 * spec + (3 * spec + 3 * 8) + (7 * 8 + 16 * tb)
+        bpp = (4 * 8 + spec) + 4 * spec + (3 * spec + 3 * 8) + (7 * 8 + 16 * tb)
         out[name] = {"ms_per_call": ms, "bytes_per_point": bpp,
                      "GBs": bpp * n * n * n / (ms * 1e-3) / 1e9, "calls": calls}
         print(f"# tke budget {name} 256^3: {ms:.2f} ms per call", file=sys.stderr, flush=True)

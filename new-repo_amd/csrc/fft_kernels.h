@@ -21,7 +21,10 @@
 // LOAD_AB2_SCALE_DIV: the AB2 predictor u + dt (t1 r + t2 r_f) of each
 // value (the arithmetic of k_ab2, src/LES_engine.f90:253-283), then
 // LOAD_SCALE_DIV: the pressure right-hand side straight from u, rhs, rhs_f.
-enum { LOAD_SCALE = 0, LOAD_SCALE_DIV = 1, LOAD_AB2_SCALE_DIV = 2 };
+// LOAD_PCLEAN (the PCL instance only): the TKE budget's cleaned pressure
+// p - (u u + v v + W W) / 2, W = (w(k) + w(k+1)) / 2, from p (src) and u, v, w
+// (ab_r), 0 on the planes 0 and nz+1.
+enum { LOAD_SCALE = 0, LOAD_SCALE_DIV = 1, LOAD_AB2_SCALE_DIV = 2, LOAD_PCLEAN = 3 };
 
 template <typename TI, typename T>
 struct RowR2CArgs {
@@ -35,6 +38,7 @@ struct RowR2CArgs {
   const TI* ab_r[3];  // LOAD_AB2_SCALE_DIV: rhs and rhs_f of field f
   const TI* ab_f[3];
   T ab_dt, ab_t1, ab_t2;
+  int pc_nz;          // LOAD_PCLEAN: nz (planes 0 and nz+1 are ghosts)
   // k_rows_r2c_nyq: the FP64 Nyquist sums of the raw input (row sums A,
   // column partials P of each band of 8 rows)
   double *nyq_a, *nyq_p;
@@ -54,7 +58,7 @@ __host__ __device__ constexpr int row_line(int n) {
 
 // One wavefront per row pair: load, transform and unpack without any
 // block-level barrier.  LDS: one line of a.ls points per wavefront.
-template <bool NYQ = false, typename TI, typename T, typename FFT>
+template <bool NYQ = false, bool PCL = false, typename TI, typename T, typename FFT>
 __device__ __forceinline__ void rows_r2c_body(const RowR2CArgs<TI, T>& a, Cplx<T>* z, int lane, long job, FFT&& fft,
                                               double* part = nullptr) {
   const int nx = a.nx, lhx = a.lhx;
@@ -79,7 +83,22 @@ __device__ __forceinline__ void rows_r2c_body(const RowR2CArgs<TI, T>& a, Cplx<T
       v0 = v0 + a.ab_dt * (a.ab_t1 * (T)r[o] + a.ab_t2 * (T)rf[o]);
       v1 = v1 + a.ab_dt * (a.ab_t1 * (T)r[o + nx] + a.ab_t2 * (T)rf[o + nx]);
     }
-    if (a.mode != LOAD_SCALE) {
+    if constexpr (PCL) {
+      if (k == 0 || k == a.pc_nz + 1) {
+        v0 = (T)0;
+        v1 = (T)0;
+      } else {
+        const long o = ((long)k * a.ny + 2 * jp) * nx + n, pl = (long)a.ny * nx;
+        const TI* __restrict__ u = a.ab_r[0];
+        const TI* __restrict__ v = a.ab_r[1];
+        const TI* __restrict__ w = a.ab_r[2];
+        const T h0 = (T)0.5 * ((T)w[o] + (T)w[o + pl]);
+        const T h1 = (T)0.5 * ((T)w[o + nx] + (T)w[o + nx + pl]);
+        v0 = v0 - (T)0.5 * (((T)u[o] * (T)u[o] + (T)v[o] * (T)v[o]) + h0 * h0);
+        v1 = v1 - (T)0.5 * (((T)u[o + nx] * (T)u[o + nx] + (T)v[o + nx] * (T)v[o + nx]) + h1 * h1);
+      }
+    }
+    if (a.mode == LOAD_SCALE_DIV || a.mode == LOAD_AB2_SCALE_DIV) {
       v0 = a.scale * (v0 / a.div);
       v1 = a.scale * (v1 / a.div);
     } else {
@@ -112,7 +131,7 @@ __device__ __forceinline__ void rows_r2c_body(const RowR2CArgs<TI, T>& a, Cplx<T
 }
 
 // FP32: held to 64 VGPRs (8 waves per SIMD instead of 7; two VGPRs spill).
-template <typename TI, typename T>
+template <typename TI, typename T, bool PCL = false>
 __global__ void __launch_bounds__(LES_BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 8 : 1)))
 k_rows_r2c(RowR2CArgs<TI, T> a, FftPlan pl, const Cplx<T>* __restrict__ tw) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -123,9 +142,9 @@ k_rows_r2c(RowR2CArgs<TI, T> a, FftPlan pl, const Cplx<T>* __restrict__ tw) {
   if (pl.n == 256) {
     TwReg<T, 256> tr;
     tw_load(tr, tw + pl.twf, lane);
-    rows_r2c_body(a, z, lane, job, [&](Cplx<T>* p, bool inv) { wave_fft_t<T, 256, row_layout<T>(256)>(p, tr, inv, lane); });
+    rows_r2c_body<false, PCL>(a, z, lane, job, [&](Cplx<T>* p, bool inv) { wave_fft_t<T, 256, row_layout<T>(256)>(p, tr, inv, lane); });
   } else {
-    rows_r2c_body(a, z, lane, job, [&](Cplx<T>* p, bool inv) { wave_fft(p, pl, tw, inv, lane); });
+    rows_r2c_body<false, PCL>(a, z, lane, job, [&](Cplx<T>* p, bool inv) { wave_fft(p, pl, tw, inv, lane); });
   }
 }
 

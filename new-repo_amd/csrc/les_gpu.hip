@@ -159,14 +159,22 @@ static void rows_r2c(Les& L, const TI* const* src, Cplx<T>* const* dst, int nf, 
     for (int f = 0; f < nf && f < 3; ++f) { a.ab_r[f] = ab[f]; a.ab_f[f] = ab[3 + f]; }
     a.ab_dt = abc[0]; a.ab_t1 = abc[1]; a.ab_t2 = abc[2];
   }
+  if (mode == LOAD_PCLEAN) {  // one field; ab = u, v, w
+    for (int f = 0; f < 3; ++f) a.ab_r[f] = ab[f];
+    a.pc_nz = L.nz;
+  }
   long jobs = (long)npl * (a.ny / 2);
   dim3 grid((unsigned)((jobs + kWavesPerBlock - 1) / kWavesPerBlock), nf);
   size_t sh = (size_t)kWavesPerBlock * a.ls * sizeof(Cplx<T>);
   kt_begin(L, KT_ROWS_R2C);
-  hipLaunchKernelGGL((k_rows_r2c<TI, T>), grid, dim3(LES_BLOCK), sh, L.stream, a, L.px, twpool<T>(L));
-  const double in_fields = mode == LOAD_AB2_SCALE_DIV ? 3.0 : 1.0;
+  if (mode == LOAD_PCLEAN)
+    hipLaunchKernelGGL((k_rows_r2c<TI, T, true>), grid, dim3(LES_BLOCK), sh, L.stream, a, L.px, twpool<T>(L));
+  else
+    hipLaunchKernelGGL((k_rows_r2c<TI, T>), grid, dim3(LES_BLOCK), sh, L.stream, a, L.px, twpool<T>(L));
+  const double in_fields = mode == LOAD_AB2_SCALE_DIV ? 3.0 : mode == LOAD_PCLEAN ? 4.0 : 1.0;
   kt_end(L, KT_ROWS_R2C, (double)nf * npl * a.ny * (in_fields * L.nx * sizeof(TI) + (double)L.lhx * sizeof(Cplx<T>)),
-         "k_rows_r2c", sizeof(TI), sizeof(T), mode == LOAD_AB2_SCALE_DIV ? "AB2 predictor" : nullptr);
+         "k_rows_r2c", sizeof(TI), sizeof(T),
+         mode == LOAD_AB2_SCALE_DIV ? "AB2 predictor" : mode == LOAD_PCLEAN ? "cleaned pressure" : nullptr);
 }
 
 // The grad_velocity_filt forward transform of one velocity component in the
@@ -1712,24 +1720,21 @@ static void diagnostics(Les& L) {
   hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(LES_BLOCK), 0, L.stream, b);
 }
 
-// compute_tke_budget: the cleaned pressure through the FP64
-// grad_velocity_filt chain (the filtered field back into tk_pf, in place in
-// the column pass like grad_filt3), then the plane sums of the 61
-// quantities in one pass over the fields, then the window sums.
+// compute_tke_budget: the cleaned pressure (formed on load by the forward
+// row transform) through the FP64 grad_velocity_filt chain (the filtered
+// spectrum in place in the column pass like grad_filt3, the filtered field
+// into tk_pf), then the plane sums of the 61 quantities in one pass over the
+// fields, then the window sums.
 template <typename T>
 static void tke_accumulate(Les& L) {
-  TkePcleanArgs c;
-  c.u = D(L, LES_U); c.v = D(L, LES_V); c.w = D(L, LES_W); c.p = D(L, LES_P);
-  c.pf = L.tk_pf; c.np = L.np; c.nz = L.nz;
-  kt_begin(L, KT_POINTWISE);
-  hipLaunchKernelGGL(k_tke_pclean, dim3(nblk(L.np * (L.nz + 2))), dim3(LES_BLOCK), 0, L.stream, c);
-  kt_end(L, KT_POINTWISE, (double)L.np * L.nz * 5.0 * 8 + 2.0 * L.np * 8, "k_tke_pclean", 8);
   Cplx<double>* s0 = (Cplx<double>*)L.tk_spec[0];
   Cplx<double>* s1 = (Cplx<double>*)L.tk_spec[1];
   Cplx<double>* s2 = (Cplx<double>*)L.tk_spec[2];
-  const double* src[1] = {L.tk_pf};
+  // the cleaned pressure formed on load (LOAD_PCLEAN)
+  const double* src[1] = {D(L, LES_P)};
+  const double* uvw[3] = {D(L, LES_U), D(L, LES_V), D(L, LES_W)};
   Cplx<double>* dst[1] = {s0};
-  rows_r2c<double, double>(L, src, dst, 1, 0, L.nzg, 1.0 / (double)(L.nx * L.ny));
+  rows_r2c<double, double>(L, src, dst, 1, 0, L.nzg, 1.0 / (double)(L.nx * L.ny), LOAD_PCLEAN, 1.0, 0, uvw);
   cols<double>(L, COP_GRAD, s0, nullptr, s1, s2, s0, 3, 1, 1, 0, L.nzg);
   const Cplx<double>* cs[3] = {s1, s2, s0};
   double* outs[3] = {L.tk_px, L.tk_py, L.tk_pf};

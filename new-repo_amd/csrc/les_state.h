@@ -87,7 +87,7 @@ struct Les {
   // TKE budget (allocated by les_tke_init)
   double* tk_sum;         // LES_NTKE x nz window sums
   double* tk_part;        // plane partial sums
-  double* tk_pf;          // cleaned pressure, then filtered (nx, ny, nzg)
+  double* tk_pf;          // filtered cleaned pressure (nx, ny, nzg)
   double* tk_px;          // its x derivative
   double* tk_py;          // its y derivative
   double* tk_spec[3];     // FP64 spectra of the chain

@@ -34,26 +34,6 @@ enum {
 static_assert(TK_N == LES_NTKE, "TKE budget quantities");
 #define LES_TKE_SLICES 16
 
-// Pf = p - (u u + v v + W W) / 2 on levels 1..nz; 0 on planes 0 and nz+1
-struct TkePcleanArgs {
-  const double *u, *v, *w, *p;
-  double* pf;
-  long np;
-  int nz;
-};
-
-__global__ void k_tke_pclean(TkePcleanArgs a) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= a.np * (a.nz + 2)) return;
-  const int k = (int)(t / a.np);
-  if (k == 0 || k == a.nz + 1) {
-    a.pf[t] = 0.0;
-    return;
-  }
-  const double h = 0.5 * (a.w[t] + a.w[t + a.np]);
-  a.pf[t] = a.p[t] - 0.5 * ((a.u[t] * a.u[t] + a.v[t] * a.v[t]) + h * h);
-}
-
 template <typename T>
 struct TkeArgs {
   const double *u, *v, *w;
