@@ -74,7 +74,8 @@ def test_fft2d_matches_oracle_bitwise():
 
 # 256x256: the production transform lengths (256 and the 3/2-padded 384);
 # 96x48: mixed radices, padded lengths 144 and 72 with two radix-3 stages
-@pytest.mark.parametrize("grid", [(16, 16, 8), (32, 32, 16), (96, 48, 6), (256, 256, 4)])
+# nz = 40: two whole 16-level chunks of the tridiagonal sweeps and a partial one
+@pytest.mark.parametrize("grid", [(16, 16, 8), (32, 32, 16), (96, 48, 6), (256, 256, 4), (32, 32, 40)])
 def test_step_matches_oracle(grid):
     pkg, O = load_pkg(), load_oracle()
     nx, ny, nz = grid
@@ -130,10 +131,11 @@ def test_operators_match_oracle_individually():
 # and a separate boundary pass.  Both from the same fields and wall-law
 # layers: u, v, w (every plane, ghosts included) agree to FP64 round-off,
 # dudz, dvdz, dwdz to FP32 rounding of the same FP64 differences.  12x24x3
-# leaves a partial wavefront of columns and one-plane segments; the lid
+# leaves a partial wavefront of columns and one-plane segments, 24x32x37
+# segments of unequal length (38 planes in 8); the lid
 # conditions are Dirichlet there (u = 0.3, v = -0.2, w = 0.05), Neumann in
 # the others.
-@pytest.mark.parametrize("grid", [(12, 24, 3), (32, 32, 16), (256, 256, 16)])
+@pytest.mark.parametrize("grid", [(12, 24, 3), (32, 32, 16), (256, 256, 16), (24, 32, 37)])
 def test_mixed_velocity_gradients_match_fp64(grid):
     pkg, P = load_pkg(), load_params()
     nx, ny, nz = grid
