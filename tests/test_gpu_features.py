@@ -18,7 +18,7 @@ FO = importlib.util.module_from_spec(spec)
 spec.loader.exec_module(FO)
 
 
-@pytest.mark.parametrize("grid", [(32, 32, 16), (256, 256, 16)])
+@pytest.mark.parametrize("grid", [(32, 32, 16), (256, 256, 16), (24, 32, 37)])
 def test_features_match_the_restatement(grid):
     torch = pytest.importorskip("torch")
     pkg, P, ic, O = load_pkg(), load_params(), load_ic(), load_oracle()

@@ -192,8 +192,11 @@ def test_fused_inference_with_many_tiles_per_wave():
     assert np.all(np.isfinite(y)) and np.abs(y[-48:] - ref[-48:]).max() < 1e-2 * np.abs(ref).max()
 
 
-@pytest.mark.parametrize("grid", [(32, 32, 16), (256, 256, 16)])
-def test_surrogate_update_is_the_network_on_the_step_features(grid):
+# 24 x 32 x 37: radix-3 rows, unequal segments; its finer dz tightens the
+# explicit-diffusion limit (h^2 / delta^2 3.7 times smaller than at 32 x 32 x
+# 16), so a smaller dt keeps the limit out of the comparison
+@pytest.mark.parametrize("grid, dt", [((32, 32, 16), 1e-4), ((256, 256, 16), 1e-4), ((24, 32, 37), 1e-5)])
+def test_surrogate_update_is_the_network_on_the_step_features(grid, dt):
     # A surrogate coefficient update (features and inference in one kernel)
     # against the CPU forward pass of the network (bf16 roundings where the
     # device rounds) on the features les_sgs_samples reports for the same
@@ -202,7 +205,7 @@ def test_surrogate_update_is_the_network_on_the_step_features(grid):
     # of the in-step inference loops over several 16-point tiles.
     torch = pytest.importorskip("torch")
     pkg, P, ic = load_pkg(), load_params(), load_ic()
-    prm = P.LesParams(nx=grid[0], ny=grid[1], nz=grid[2], precision=P.PREC_LEARNED, judge_every=1000, dt=1e-4,
+    prm = P.LesParams(nx=grid[0], ny=grid[1], nz=grid[2], precision=P.PREC_LEARNED, judge_every=1000, dt=dt,
                       learned_zmin=4.0)
     u, v, w = ic.log_law_ic(prm.nx, prm.ny, prm.nz, prm.lz, prm.z0, 0.1, seed=21)
     np_, nz = prm.nx * prm.ny, prm.nz
