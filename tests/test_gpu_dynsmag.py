@@ -35,7 +35,7 @@ def _levels(name, a, nz):
     return a[1:nz + 2]
 
 
-@pytest.mark.parametrize("grid", [(32, 32, 16), (96, 48, 6), (256, 256, 4)])
+@pytest.mark.parametrize("grid", [(32, 32, 16), (96, 48, 6), (256, 256, 4), (24, 32, 21)])
 def test_dynsmag_steps_match_oracle(grid):
     pkg, O = load_pkg(), load_oracle()
     nx, ny, nz = grid

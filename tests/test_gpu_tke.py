@@ -51,7 +51,7 @@ def _compare(dev, orc, ab, tol):
     print(f"  worst: {worst[0]} {worst[1]:.2e}", flush=True)
 
 
-@pytest.mark.parametrize("grid", [(32, 32, 16), (256, 256, 8)])
+@pytest.mark.parametrize("grid", [(32, 32, 16), (256, 256, 8), (24, 32, 13)])
 def test_tke_sums_match_oracle(grid):
     print(f"tke budget FP64 {grid}", flush=True)
     P = load_params()
